@@ -1,0 +1,214 @@
+// gn_device.hpp -- device building blocks shared by the HIP kernels of libgnurbs.
+#pragma once
+#include "gn_internal.hpp"
+
+namespace gn {
+
+// Knot-span lookup.  Returns the unique m with k[m] <= u < k[m+1], clamped to [p, cp-1] exactly
+// as findSpan (src/NURBS/Evaluate.cu:504-533): u >= k[cp] -> cp-1, u <= k[p] -> p.  A NaN makes
+// the reference's while-loop exit at its first midpoint, (p + cp) / 2; that is reproduced too.
+// Any search that finds the unique bracketing span returns the reference's index, so the loop
+// here is a plain branch-light bisection on an invariant k[lo] <= u < k[hi].
+__device__ __forceinline__ int find_span(const double *__restrict__ k, int p, int cp, double u) {
+    if (u >= k[cp]) return cp - 1;
+    if (u <= k[p]) return p;
+    if (u != u) return (p + cp) >> 1;
+    int lo = p, hi = cp;
+    while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        if (u < k[mid]) hi = mid; else lo = mid;
+    }
+    return lo;
+}
+
+template <typename T>
+struct Hom {  // homogeneous 4-vector
+    T x, y, z, w;
+};
+
+template <typename T>
+struct HomDer {  // homogeneous point and partial derivatives in the local span parameters
+    Hom<T> R, Ru, Rv, Ruu, Ruv, Rvv;
+};
+
+template <typename T>
+__device__ __forceinline__ void hzero(Hom<T> &a) { a.x = a.y = a.z = a.w = T(0); }
+template <typename T>
+__device__ __forceinline__ void hfma(Hom<T> &acc, T s, const Hom<T> &m) {
+    acc.x = fma(s, m.x, acc.x);
+    acc.y = fma(s, m.y, acc.y);
+    acc.z = fma(s, m.z, acc.z);
+    acc.w = fma(s, m.w, acc.w);
+}
+
+template <typename T>
+__device__ __forceinline__ Hom<T> load_hom(const T *__restrict__ p) {
+    Hom<T> h;
+    h.x = p[0]; h.y = p[1]; h.z = p[2]; h.w = p[3];
+    return h;
+}
+template <>
+__device__ __forceinline__ Hom<float> load_hom<float>(const float *__restrict__ p) {
+    float4 v = *reinterpret_cast<const float4 *>(p);
+    Hom<float> h;
+    h.x = v.x; h.y = v.y; h.z = v.z; h.w = v.w;
+    return h;
+}
+template <>
+__device__ __forceinline__ Hom<double> load_hom<double>(const double *__restrict__ p) {
+    double2 a = reinterpret_cast<const double2 *>(p)[0];
+    double2 b = reinterpret_cast<const double2 *>(p)[1];
+    Hom<double> h;
+    h.x = a.x; h.y = a.y; h.z = b.x; h.w = b.y;
+    return h;
+}
+
+// Contraction of one span patch M[r][s][c] (layout [K1][K2][4]) with the power bases of the
+// local parameters (t, s):  R = sum_rs t^r M[r][s] s^s  and its partial derivatives up to
+// ORDER (MatrixSurfacePoint / MatrixSurfaceRd*, src/NURBS/Evaluate.cu:159-502).  The loop runs
+// over s outermost so that only the 4-vectors of one patch column are live at a time.
+// K1/K2 are compile-time bounds; k1/k2 the runtime sizes (equal to K1/K2 when EXACT).
+template <int K1, int K2, int ORDER, bool EXACT, typename T>
+__device__ __forceinline__ void contract(const T *__restrict__ patch, int k1, int k2, T t, T s,
+                                         HomDer<T> &o) {
+    if (EXACT) { k1 = K1; k2 = K2; }
+    T U[K1], dU[K1], ddU[K1], V[K2], dV[K2], ddV[K2];
+    {
+        T now = T(1);
+#pragma unroll
+        for (int r = 0; r < K1; r++) { U[r] = now; now *= t; }
+        dU[0] = T(0);
+        ddU[0] = T(0);
+#pragma unroll
+        for (int r = 1; r < K1; r++) dU[r] = T(r) * U[r - 1];
+#pragma unroll
+        for (int r = 0; r < K1; r++) ddU[r] = r >= 2 ? T(r * (r - 1)) * U[r - 2] : T(0);
+        now = T(1);
+#pragma unroll
+        for (int c = 0; c < K2; c++) { V[c] = now; now *= s; }
+        dV[0] = T(0);
+#pragma unroll
+        for (int c = 1; c < K2; c++) dV[c] = T(c) * V[c - 1];
+#pragma unroll
+        for (int c = 0; c < K2; c++) ddV[c] = c >= 2 ? T(c * (c - 1)) * V[c - 2] : T(0);
+    }
+    hzero(o.R);
+    if (ORDER >= 1) { hzero(o.Ru); hzero(o.Rv); }
+    if (ORDER >= 2) { hzero(o.Ruu); hzero(o.Ruv); hzero(o.Rvv); }
+#pragma unroll
+    for (int c = 0; c < K2; c++) {
+        if (!EXACT && c >= k2) break;
+        Hom<T> a, a1, a2;
+        hzero(a);
+        hzero(a1);
+        hzero(a2);
+#pragma unroll
+        for (int r = 0; r < K1; r++) {
+            if (!EXACT && r >= k1) break;
+            Hom<T> m = load_hom<T>(patch + (r * k2 + c) * 4);
+            hfma(a, U[r], m);
+            if (ORDER >= 1 && r >= 1) hfma(a1, dU[r], m);
+            if (ORDER >= 2 && r >= 2) hfma(a2, ddU[r], m);
+        }
+        hfma(o.R, V[c], a);
+        if (ORDER >= 1) {
+            hfma(o.Ru, V[c], a1);
+            if (c >= 1) hfma(o.Rv, dV[c], a);
+        }
+        if (ORDER >= 2) {
+            hfma(o.Ruu, V[c], a2);
+            if (c >= 1) hfma(o.Ruv, dV[c], a1);
+            if (c >= 2) hfma(o.Rvv, ddV[c], a);
+        }
+    }
+}
+
+template <typename T>
+struct Cart {  // Cartesian point and derivatives in the global parameters
+    T S[3], Su[3], Sv[3], Suu[3], Suv[3], Svv[3];
+};
+
+// Rational quotient (the old-version rational derivatives of src/NURBS/Evaluate.cu:535-930,
+// written in the equivalent Leibniz form S_u = (A_u - w_u S)/w, ...) and the chain factors
+// 1/(k[m+1]-k[m]) of the span (the reference multiplies by (k[last]-k[0])/(k[m+1]-k[m]),
+// which is the same for knots on [0, 1]).
+template <int ORDER, typename T>
+__device__ __forceinline__ void quotient(const HomDer<T> &h, T ihu, T ihv, Cart<T> &o) {
+    T iw = T(1) / h.R.w;
+    o.S[0] = h.R.x * iw;
+    o.S[1] = h.R.y * iw;
+    o.S[2] = h.R.z * iw;
+    if (ORDER < 1) return;
+    const T Ru[3] = {h.Ru.x, h.Ru.y, h.Ru.z}, Rv[3] = {h.Rv.x, h.Rv.y, h.Rv.z};
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        o.Su[c] = (Ru[c] - h.Ru.w * o.S[c]) * iw;
+        o.Sv[c] = (Rv[c] - h.Rv.w * o.S[c]) * iw;
+    }
+    if (ORDER >= 2) {
+        const T Ruu[3] = {h.Ruu.x, h.Ruu.y, h.Ruu.z}, Ruv[3] = {h.Ruv.x, h.Ruv.y, h.Ruv.z},
+                Rvv[3] = {h.Rvv.x, h.Rvv.y, h.Rvv.z};
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            T uu = (Ruu[c] - T(2) * h.Ru.w * o.Su[c] - h.Ruu.w * o.S[c]) * iw;
+            T uv = (Ruv[c] - h.Ru.w * o.Sv[c] - h.Rv.w * o.Su[c] - h.Ruv.w * o.S[c]) * iw;
+            T vv = (Rvv[c] - T(2) * h.Rv.w * o.Sv[c] - h.Rvv.w * o.S[c]) * iw;
+            o.Suu[c] = uu * (ihu * ihu);
+            o.Suv[c] = uv * (ihu * ihv);
+            o.Svv[c] = vv * (ihv * ihv);
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        o.Su[c] *= ihu;
+        o.Sv[c] *= ihv;
+    }
+}
+
+// Full evaluation at global (u, v): span lookup against FP64 knots, local parameters in FP64,
+// contraction and quotient in T (double or float).
+template <int K1, int K2, int ORDER, bool EXACT, typename T>
+__device__ __forceinline__ void eval_point(const GnSurfDev &d, const double *__restrict__ ku,
+                                           const double *__restrict__ kv, double u, double v,
+                                           Cart<T> &o, int *pm = nullptr, int *pn = nullptr) {
+    const int m = find_span(ku, d.p, d.nu, u);
+    const int n = find_span(kv, d.q, d.nv, v);
+    const int im = m - d.p, in = n - d.q;
+    const double ihu = d.ihu[im], ihv = d.ihv[in];
+    const T t = T((u - ku[m]) * ihu);
+    const T s = T((v - kv[n]) * ihv);
+    const int k1 = d.p + 1, k2 = d.q + 1;
+    const T *tab = nullptr;
+    if constexpr (sizeof(T) == 8) tab = reinterpret_cast<const T *>(d.tab64);
+    else tab = reinterpret_cast<const T *>(d.tab32);
+    const T *patch = tab + (size_t)(im * d.sv + in) * (size_t)(k1 * k2 * 4);
+    HomDer<T> h;
+    contract<K1, K2, ORDER, EXACT, T>(patch, k1, k2, t, s, h);
+    quotient<ORDER, T>(h, T(ihu), T(ihv), o);
+    if (pm) *pm = m;
+    if (pn) *pn = n;
+}
+
+// Dispatch helper: calls F.template operator()<K1, K2, EXACT>() for the degree pair of a
+// surface.  Degrees 2..5 in each direction get exact instantiations; everything else uses the
+// padded <8, 8> form with runtime sizes.
+template <typename F>
+__host__ int dispatch_degrees(int p, int q, F &&f) {
+#define GN_CASE(P, Q) \
+    if (p == P && q == Q) return f.template operator()<P + 1, Q + 1, true>();
+    GN_CASE(2, 2) GN_CASE(2, 3) GN_CASE(2, 4) GN_CASE(2, 5)
+    GN_CASE(3, 2) GN_CASE(3, 3) GN_CASE(3, 4) GN_CASE(3, 5)
+    GN_CASE(4, 2) GN_CASE(4, 3) GN_CASE(4, 4) GN_CASE(4, 5)
+    GN_CASE(5, 2) GN_CASE(5, 3) GN_CASE(5, 4) GN_CASE(5, 5)
+#undef GN_CASE
+    return f.template operator()<GN_MAXK, GN_MAXK, false>();
+}
+
+__host__ __device__ inline unsigned grid_for(size_t n, int block, unsigned cap) {
+    size_t g = (n + block - 1) / block;
+    if (g > cap) g = cap;
+    if (g == 0) g = 1;
+    return (unsigned)g;
+}
+
+}  // namespace gn

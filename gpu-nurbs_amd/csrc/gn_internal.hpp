@@ -1,0 +1,118 @@
+// gn_internal.hpp -- shared definitions of libgnurbs (host and device).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "gnurbs.h"
+
+#define GN_MAXK 8  // k = degree + 1; degrees 1..7 (MAX_DEGREE of src/NURBS/Evaluate.cuh:15)
+
+// ---------------------------------------------------------------------------------------------
+// Device view of one surface.  Span slots are indexed by (m - p, n - q) for knot spans
+// m in [p, nu-1], n in [q, nv-1]; each slot holds the patch M_u * P_w * M_v^T as
+// [r][s][c] with c = (x*w, y*w, z*w, w) fastest, so one (r, s) entry is one 4-vector.
+// ---------------------------------------------------------------------------------------------
+struct GnSurfDev {
+    int p, q, nu, nv;       // degrees and control-net size
+    int su, sv;             // span slots per direction: nu - p, nv - q
+    const double *ku, *kv;  // knots (nu + p + 1, nv + q + 1)
+    const double *ihu, *ihv;  // 1 / (k[m+1] - k[m]) per slot (0 for empty spans)
+    const double *tab64;    // su*sv*k1*k2*4 doubles
+    const float *tab32;     // same, FP32
+};
+
+// Tree node record, FP64, 13 values (matches gn_get_tree)
+struct GnTreeNode {
+    double uv_lb[2], uv_rt[2], lb[3], rt[3], c[3];
+};
+
+// Quadtree search structure used by the nearest-leaf seed search: levels 0..D of 4^l nodes in
+// Morton order; boxes contain every leaf centre below them.
+struct GnSearchDev {
+    int depth;                // D = 3*(levels-1)
+    const double *box;        // per node 6 doubles lo.xyz hi.xyz, level offsets (4^l-1)/3
+    const double *centre;     // per leaf (Morton order) centre point xyz
+    const double *seed_uv;    // per leaf (Morton order) seed (u, v)
+    const int32_t *ref_index; // per leaf (Morton order) reference tree index
+};
+
+// ---------------------------------------------------------------------------------------------
+// error handling
+// ---------------------------------------------------------------------------------------------
+namespace gn {
+void set_error(const std::string &msg);
+int fail(int code, const std::string &msg);
+int hip_fail(hipError_t e, const char *what);
+}  // namespace gn
+
+#define GN_HIP(call)                                              \
+    do {                                                          \
+        hipError_t _e = (call);                                   \
+        if (_e != hipSuccess) return gn::hip_fail(_e, #call);     \
+    } while (0)
+
+// ---------------------------------------------------------------------------------------------
+// host-side surface object
+// ---------------------------------------------------------------------------------------------
+struct gn_surface {
+    int p, q, nu, nv;
+    std::vector<double> ku, kv, xyz, w;  // host copies of the definition
+    // device buffers
+    double *d_ku = nullptr, *d_kv = nullptr, *d_ihu = nullptr, *d_ihv = nullptr;
+    double *d_xyz = nullptr, *d_w = nullptr;
+    double *d_tab64 = nullptr;
+    float *d_tab32 = nullptr;
+    size_t tab_entries = 0;  // su*sv*k1*k2*4
+    // tree
+    int tree_levels = 0;
+    GnTreeNode *d_tree = nullptr;
+    uint8_t *d_valid = nullptr, *d_divided = nullptr;
+    double *d_sbox = nullptr, *d_scentre = nullptr, *d_sseed = nullptr;
+    int32_t *d_sref = nullptr;
+    size_t device_bytes = 0;
+
+    GnSurfDev dev() const {
+        GnSurfDev d;
+        d.p = p; d.q = q; d.nu = nu; d.nv = nv;
+        d.su = nu - p; d.sv = nv - q;
+        d.ku = d_ku; d.kv = d_kv; d.ihu = d_ihu; d.ihv = d_ihv;
+        d.tab64 = d_tab64; d.tab32 = d_tab32;
+        return d;
+    }
+    GnSearchDev search() const {
+        GnSearchDev s;
+        s.depth = 3 * (tree_levels - 1);
+        s.box = d_sbox; s.centre = d_scentre; s.seed_uv = d_sseed; s.ref_index = d_sref;
+        return s;
+    }
+};
+
+// ---------------------------------------------------------------------------------------------
+// kernel launchers (defined in the .hip files)
+// ---------------------------------------------------------------------------------------------
+namespace gn {
+int launch_span_table(const gn_surface *s, const double *d_xyz, const double *d_w,
+                      hipStream_t st);
+int launch_find_span(const GnSurfDev &d, int dir, const double *u, size_t n, int32_t *span,
+                     hipStream_t st);
+int launch_eval64(const GnSurfDev &d, const double *u, const double *v, size_t n, int order,
+                  double *out, hipStream_t st);
+int launch_eval32(const GnSurfDev &d, const float *u, const float *v, size_t n, int order,
+                  float *out, hipStream_t st);
+int launch_eval_grid32(const GnSurfDev &d, const float *us, int nu_pts, const float *vs,
+                       int nv_pts, int order, float *out, hipStream_t st);
+int launch_tree_level(const GnSurfDev &d, int level, int flags, GnTreeNode *tree,
+                      uint8_t *valid, uint8_t *divided, hipStream_t st);
+int launch_search_build(const GnSurfDev &d, int levels, const GnTreeNode *tree, double *sbox,
+                        double *scentre, double *sseed, int32_t *sref, hipStream_t st);
+int launch_seed(const GnSearchDev &sd, const double *x, const double *y, const double *z,
+                size_t n, int32_t *leaf, double *su, double *sv, hipStream_t st);
+int launch_invert(const GnSurfDev &d, const GnSearchDev &sd, const GnTreeNode *tree,
+                  const uint8_t *valid, int tree_levels, const double *x, const double *y,
+                  const double *z, size_t n, const gn_invert_opts &o, double *u, double *v,
+                  uint8_t *status, uint8_t *it32, uint8_t *it64, hipStream_t st);
+}  // namespace gn

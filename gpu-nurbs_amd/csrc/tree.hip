@@ -1,0 +1,287 @@
+// tree.hip -- subdivision-tree build and nearest-leaf seed search.
+//
+// Tree build follows GetPoints / GetSecondDerivatives / Max_Sequential_Addressing_Shared /
+// TightBoundingBox / NeedDivide (src/kernel/kernel.cu:375-431, 544-621, 707-733, 859-901):
+// node (bx, by) of level L (g = 8^L per side) covers [bx/g, (bx+1)/g] x [by/g, (by+1)/g], is
+// sampled on a 9 x 9 grid, and gets the AABB of the samples enlarged by
+// K = (M1 + 2 M2 + M3) sqrt(3) / (8 (8g)^2), M* = max over samples of |S_uu|inf, |S_uv|inf,
+// |S_vv|inf, then is clamped to its parent's box.  The reference runs five launches per level
+// (samples, three derivative passes, two LDS reductions) with a device-wide sync after each; here
+// one wave per node evaluates its 81 samples (lanes 0..63, then lanes 0..16 again) and reduces
+// them with cross-lane shuffles in a single launch per level.
+//
+// Seed search (new): a quadtree over the finest level, built bottom-up, whose boxes contain the
+// leaf centre points below them.  Each lane descends greedily for an initial bound, then walks
+// the tree depth-first without a stack (Morton-code successor), pruning every box whose squared
+// distance exceeds the best centre distance.  The result is the exact nearest leaf centre.
+#include "gn_device.hpp"
+
+namespace gn {
+
+__device__ __forceinline__ double wave_max(double x) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x = fmax(x, __shfl_xor(x, o, 64));
+    return x;
+}
+__device__ __forceinline__ double wave_min(double x) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x = fmin(x, __shfl_xor(x, o, 64));
+    return x;
+}
+
+__device__ __forceinline__ bool need_divide(const double lb[3], const double rt[3]) {
+    // NeedDivide, src/kernel/kernel.cu:859-878; Threshold 0.5 (src/kernel/kernel.cuh:28)
+    double dx = rt[0] - lb[0], dy = rt[1] - lb[1], dz = rt[2] - lb[2], t;
+    if (dx > dy) { t = dx; dx = dy; dy = t; }
+    if (dx > dz) { t = dx; dx = dz; dz = t; }
+    if ((dx < 0.5 * dy) && (dx < 0.5 * dz)) return false;
+    return true;
+}
+
+// One 64-lane workgroup per node of level `level`.
+template <int K1, int K2, bool EXACT>
+__global__ __launch_bounds__(64) void k_tree_level(GnSurfDev d, int level, int flags,
+                                                   GnTreeNode *__restrict__ tree,
+                                                   uint8_t *__restrict__ valid,
+                                                   uint8_t *__restrict__ divided) {
+    long g = 1, base = 0;
+    for (int l = 0; l < level; l++) { base += g * g; g *= 8; }
+    const long idx = blockIdx.x;  // bx * g + by
+    const long bx = idx / g, by = idx % g;
+    const long ti = base + idx;
+    const double base_unit = 1.0 / g, unit = base_unit / 8;
+    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    double M1 = 0, M2 = 0, M3 = 0;
+    for (int sidx = threadIdx.x; sidx < 81; sidx += 64) {
+        const int tx = sidx / 9, ty = sidx % 9;
+        const double su = base_unit * bx + tx * unit, sv = base_unit * by + ty * unit;
+        Cart<double> o;
+        eval_point<K1, K2, 2, EXACT, double>(d, d.ku, d.kv, su, sv, o);
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            lo[c] = fmin(lo[c], o.S[c]);
+            hi[c] = fmax(hi[c], o.S[c]);
+        }
+        M1 = fmax(M1, fmax(fabs(o.Suu[0]), fmax(fabs(o.Suu[1]), fabs(o.Suu[2]))));
+        M2 = fmax(M2, fmax(fabs(o.Suv[0]), fmax(fabs(o.Suv[1]), fabs(o.Suv[2]))));
+        M3 = fmax(M3, fmax(fabs(o.Svv[0]), fmax(fabs(o.Svv[1]), fabs(o.Svv[2]))));
+    }
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        lo[c] = wave_min(lo[c]);
+        hi[c] = wave_max(hi[c]);
+    }
+    M1 = wave_max(M1);
+    M2 = wave_max(M2);
+    M3 = wave_max(M3);
+    if (threadIdx.x != 0) return;
+    const double K = (M1 + 2 * M2 + M3) * sqrt(3.0) / (8.0 * 8.0 * (double)g * 8.0 * (double)g);
+    GnTreeNode nd;
+    const double unitg = 1.0 / g;
+    nd.uv_lb[0] = bx * unitg;
+    nd.uv_lb[1] = by * unitg;
+    nd.uv_rt[0] = (bx + 1) * unitg;
+    nd.uv_rt[1] = (by + 1) * unitg;
+    for (int c = 0; c < 3; c++) {
+        nd.lb[c] = lo[c] - K;
+        nd.rt[c] = hi[c] + K;
+    }
+    long parent = -1;
+    if (level > 0) {
+        if (flags & GN_TREE_REF_PARENT) parent = (ti - 1) / 64;
+        else {
+            long gp = g / 8, bp = base - gp * gp;
+            parent = bp + (bx / 8) * gp + by / 8;
+        }
+    }
+    uint8_t vld = 1;
+    if (parent >= 0) {
+        const GnTreeNode pn = tree[parent];
+        for (int c = 0; c < 3; c++) {
+            nd.rt[c] = fmin(nd.rt[c], pn.rt[c]);
+            nd.lb[c] = fmax(nd.lb[c], pn.lb[c]);
+        }
+        vld = valid[parent] && divided[parent];
+    }
+    Cart<double> oc;
+    eval_point<K1, K2, 0, EXACT, double>(d, d.ku, d.kv, (nd.uv_lb[0] + nd.uv_rt[0]) / 2,
+                                          (nd.uv_lb[1] + nd.uv_rt[1]) / 2, oc);
+    for (int c = 0; c < 3; c++) nd.c[c] = oc.S[c];
+    tree[ti] = nd;
+    valid[ti] = vld;
+    divided[ti] = need_divide(nd.lb, nd.rt) ? 1 : 0;
+}
+
+template <typename L>
+struct TreeLaunch {
+    const GnSurfDev &d;
+    int level, flags;
+    GnTreeNode *tree;
+    uint8_t *valid, *divided;
+    hipStream_t st;
+    template <int K1, int K2, bool EXACT>
+    int operator()() {
+        long g = 1;
+        for (int l = 0; l < level; l++) g *= 8;
+        k_tree_level<K1, K2, EXACT><<<(unsigned)(g * g), 64, 0, st>>>(d, level, flags, tree, valid,
+                                                                      divided);
+        GN_HIP(hipGetLastError());
+        return GN_OK;
+    }
+};
+
+int launch_tree_level(const GnSurfDev &d, int level, int flags, GnTreeNode *tree,
+                      uint8_t *valid, uint8_t *divided, hipStream_t st) {
+    return dispatch_degrees(d.p, d.q, TreeLaunch<void>{d, level, flags, tree, valid, divided, st});
+}
+
+// ---------------------------------------------------------------------------------------------
+// quadtree search structure
+// ---------------------------------------------------------------------------------------------
+__host__ __device__ inline long qoff(int l) { return ((1L << (2 * l)) - 1) / 3; }
+
+__device__ inline uint32_t morton2(uint32_t x, uint32_t y) {  // x bits odd, y bits even
+    uint32_t c = 0;
+    for (int b = 0; b < 16; b++) c |= ((x >> b) & 1u) << (2 * b + 1) | ((y >> b) & 1u) << (2 * b);
+    return c;
+}
+
+// leaves: one thread per finest-level node
+__global__ void k_search_leaves(int levels, const GnTreeNode *__restrict__ tree,
+                                double *__restrict__ sbox, double *__restrict__ scentre,
+                                double *__restrict__ sseed, int32_t *__restrict__ sref) {
+    long g = 1, base = 0;
+    for (int l = 0; l < levels - 1; l++) { base += g * g; g *= 8; }
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= g * g) return;
+    const long bx = idx / g, by = idx % g;
+    const int D = 3 * (levels - 1);
+    const GnTreeNode nd = tree[base + idx];
+    const uint32_t c = morton2((uint32_t)bx, (uint32_t)by);
+    double *b = sbox + (qoff(D) + c) * 6;
+    for (int k = 0; k < 3; k++) {
+        b[k] = fmin(nd.lb[k], nd.c[k]);
+        b[3 + k] = fmax(nd.rt[k], nd.c[k]);
+        scentre[c * 3 + k] = nd.c[k];
+    }
+    sseed[c * 2 + 0] = (nd.uv_lb[0] + nd.uv_rt[0]) / 2;
+    sseed[c * 2 + 1] = (nd.uv_lb[1] + nd.uv_rt[1]) / 2;
+    sref[c] = (int32_t)(base + idx);
+}
+
+// internal level l from level l+1
+__global__ void k_search_level(int l, double *__restrict__ sbox) {
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= (1L << (2 * l))) return;
+    const double *ch = sbox + (qoff(l + 1) + 4 * c) * 6;
+    double b[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+    for (int a = 0; a < 4; a++)
+        for (int k = 0; k < 3; k++) {
+            b[k] = fmin(b[k], ch[a * 6 + k]);
+            b[3 + k] = fmax(b[3 + k], ch[a * 6 + 3 + k]);
+        }
+    double *o = sbox + (qoff(l) + c) * 6;
+    for (int k = 0; k < 6; k++) o[k] = b[k];
+}
+
+int launch_search_build(const GnSurfDev &d, int levels, const GnTreeNode *tree, double *sbox,
+                        double *scentre, double *sseed, int32_t *sref, hipStream_t st) {
+    (void)d;
+    const int D = 3 * (levels - 1);
+    const long leaves = 1L << (2 * D);
+    k_search_leaves<<<grid_for(leaves, 256, 1u << 30), 256, 0, st>>>(levels, tree, sbox, scentre,
+                                                                       sseed, sref);
+    for (int l = D - 1; l >= 0; l--)
+        k_search_level<<<grid_for(1L << (2 * l), 256, 1u << 30), 256, 0, st>>>(l, sbox);
+    GN_HIP(hipGetLastError());
+    return GN_OK;
+}
+
+__device__ __forceinline__ double box_d2(const double *__restrict__ b, const double P[3]) {
+    double s = 0;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        double e = fmax(fmax(b[k] - P[k], P[k] - b[3 + k]), 0.0);
+        s = fma(e, e, s);
+    }
+    return s;
+}
+
+// Exact nearest leaf centre (ties to the lower reference index).  Returns the Morton index.
+__device__ uint32_t nearest_leaf(const GnSearchDev &sd, const double P[3]) {
+    const int D = sd.depth;
+    if (D == 0) return 0;
+    // greedy descent for an initial bound
+    uint32_t c = 0;
+    for (int l = 0; l < D; l++) {
+        double bd = INFINITY;
+        uint32_t bc = 4 * c;
+        for (int a = 0; a < 4; a++) {
+            double dd = box_d2(sd.box + (qoff(l + 1) + 4 * c + a) * 6, P);
+            if (dd < bd) { bd = dd; bc = 4 * c + a; }
+        }
+        c = bc;
+    }
+    auto cdist = [&](uint32_t leaf) {
+        const double *q = sd.centre + leaf * 3;
+        double dx = P[0] - q[0], dy = P[1] - q[1], dz = P[2] - q[2];
+        return fma(dz, dz, fma(dy, dy, dx * dx));
+    };
+    uint32_t best_c = c;
+    double best = cdist(c);
+    // stackless depth-first walk with pruning
+    int l = 1;
+    c = 0;
+    while (true) {
+        const double lb = box_d2(sd.box + (qoff(l) + c) * 6, P);
+        bool descend = false;
+        if (lb <= best) {
+            if (l == D) {
+                const double dd = cdist(c);
+                if (dd < best || (dd == best && sd.ref_index[c] < sd.ref_index[best_c])) {
+                    best = dd;
+                    best_c = c;
+                }
+            } else {
+                descend = true;
+            }
+        }
+        if (descend) {
+            c <<= 2;
+            l++;
+            continue;
+        }
+        while (l > 1 && (c & 3u) == 3u) {
+            c >>= 2;
+            l--;
+        }
+        if (l == 1 && (c & 3u) == 3u) break;
+        c++;
+    }
+    return best_c;
+}
+
+__global__ __launch_bounds__(256) void k_seed(GnSearchDev sd, const double *__restrict__ x,
+                                              const double *__restrict__ y,
+                                              const double *__restrict__ z, size_t n,
+                                              int32_t *__restrict__ leaf, double *__restrict__ su,
+                                              double *__restrict__ sv) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x) {
+        const double P[3] = {x[i], y[i], z[i]};
+        const uint32_t c = nearest_leaf(sd, P);
+        if (leaf) leaf[i] = sd.ref_index[c];
+        su[i] = sd.seed_uv[c * 2 + 0];
+        sv[i] = sd.seed_uv[c * 2 + 1];
+    }
+}
+
+int launch_seed(const GnSearchDev &sd, const double *x, const double *y, const double *z,
+                size_t n, int32_t *leaf, double *su, double *sv, hipStream_t st) {
+    if (n == 0) return GN_OK;
+    k_seed<<<grid_for(n, 256, 256 * 32), 256, 0, st>>>(sd, x, y, z, n, leaf, su, sv);
+    GN_HIP(hipGetLastError());
+    return GN_OK;
+}
+
+}  // namespace gn

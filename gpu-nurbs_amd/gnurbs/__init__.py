@@ -1,0 +1,305 @@
+"""gnurbs -- Python binding of libgnurbs, matrix-form NURBS evaluation and inversion on MI355X.
+
+The binding is a thin ctypes layer over the C ABI declared in include/gnurbs.h; every call goes
+straight to the HIP kernels in libgnurbs.so (built in-tree next to this file).  torch supplies
+device memory and the current HIP stream; no computation happens in Python and there is no CPU
+fallback: if the library is missing or no GPU is visible, calls raise.
+
+Method names follow the reference's entry points (Qiang-Zou/GPU-NURBS): ``readNu``
+(src/main.cu:45-113), ``findSpan`` (src/NURBS/Evaluate.cu:504-533), ``GetMatrixN``
+(src/NURBS/Evaluate.cu:3-88), ``EvaluateRandom``/``Evaluate`` (src/NURBS/Evaluate.cu:1023-1038 and
+the derivative functions 535-930), ``BuildTree`` (src/kernel/kernel.cu:375-431, 544-621),
+``NewInverse`` (src/kernel/kernel.cu:433-469 with GaussNewton 151-249).
+"""
+from __future__ import annotations
+
+import atexit
+import ctypes
+import os
+import weakref
+from typing import Optional
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgnurbs.so")
+
+GN_OK, GN_EINVAL, GN_ENOMEM, GN_EDEVICE, GN_EIO, GN_ESTATE, GN_EBUDGET, GN_ENOTFOUND, GN_EAGAIN = range(9)
+GN_FP64, GN_FP32, GN_MIXED = 0, 1, 2
+PT_CONVERGED, PT_CONVERGED_DIST, PT_MAXITER, PT_SINGULAR, PT_NONFINITE, PT_NOSEED = range(6)
+SEED_NEAREST_LEAF, SEED_REF_FIRST_HIT, SEED_GIVEN = 0, 1, 2
+TREE_REF_PARENT = 1
+NEWTON_FP32, NEWTON_FP64 = 1, 2
+
+_c_int, _c_double, _c_size, _c_void = ctypes.c_int, ctypes.c_double, ctypes.c_size_t, ctypes.c_void_p
+
+
+class InvertOpts(ctypes.Structure):
+    """gn_invert_opts (include/gnurbs.h)."""
+
+    _fields_ = [
+        ("tol_dist", _c_double),
+        ("tol_step", _c_double),
+        ("max_iter", _c_int),
+        ("clamp", _c_int),
+        ("precision", _c_int),
+        ("fp32_iters", _c_int),
+        ("fp64_tail", _c_int),
+        ("tol_step32", _c_double),
+        ("seed_mode", _c_int),
+        ("newton", _c_int),
+    ]
+
+
+class GnError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"gnurbs error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+# (name, restype, argtypes) of every C ABI export this binding uses
+_SIGS = [
+    ("gn_version", ctypes.c_char_p, []),
+    ("gn_last_error", ctypes.c_char_p, []),
+    ("gn_invert_opts_default", None, [ctypes.POINTER(InvertOpts)]),
+    ("gn_surface_create", _c_int, [_c_int, _c_int, _c_int, _c_int, _c_void, _c_void, _c_void, _c_void, _c_void, ctypes.POINTER(_c_void)]),
+    ("gn_surface_from_nu", _c_int, [ctypes.c_char_p, _c_void, ctypes.POINTER(_c_void)]),
+    ("gn_surface_destroy", _c_int, [_c_void]),
+    ("gn_surface_info", _c_int, [_c_void, ctypes.POINTER(_c_int), ctypes.POINTER(_c_int), ctypes.POINTER(_c_int), ctypes.POINTER(_c_int)]),
+    ("gn_surface_device_bytes", _c_size, [_c_void]),
+    ("gn_find_span", _c_int, [_c_void, _c_int, _c_void, _c_size, _c_void, _c_void]),
+    ("gn_build_span_table", _c_int, [_c_void, _c_void]),
+    ("gn_get_span_matrix", _c_int, [_c_void, _c_int, _c_int, _c_void]),
+    ("gn_evaluate", _c_int, [_c_void, _c_void, _c_void, _c_size, _c_int, _c_void, _c_void]),
+    ("gn_evaluate_f32", _c_int, [_c_void, _c_void, _c_void, _c_size, _c_int, _c_void, _c_void]),
+    ("gn_evaluate_grid_f32", _c_int, [_c_void, _c_void, _c_int, _c_void, _c_int, _c_int, _c_void, _c_void]),
+    ("gn_build_tree", _c_int, [_c_void, _c_int, _c_int, _c_void]),
+    ("gn_get_tree", _c_int, [_c_void, _c_void, _c_void, _c_void, _c_size]),
+    ("gn_seed", _c_int, [_c_void, _c_void, _c_void, _c_void, _c_size, _c_void, _c_void, _c_void, _c_void]),
+    ("gn_invert", _c_int, [_c_void, _c_void, _c_void, _c_void, _c_size, ctypes.POINTER(InvertOpts), _c_void, _c_void, _c_void, _c_void]),
+    ("gn_invert_iters", _c_int, [_c_void, _c_void, _c_void, _c_void, _c_size, ctypes.POINTER(InvertOpts), _c_void, _c_void, _c_void, _c_void, _c_void, _c_void]),
+]
+
+
+def lib():
+    """Load libgnurbs.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with __graft_entry__.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, res, args in _SIGS:
+            if not hasattr(L, name):
+                continue
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc: int):
+    if rc != GN_OK:
+        raise GnError(rc, lib().gn_last_error().decode())
+
+
+def _torch():
+    import torch
+
+    if not torch.cuda.is_available():
+        raise RuntimeError("gnurbs needs a visible GPU (HIP device); none is available")
+    return torch
+
+
+def stream_handle(stream=None) -> int:
+    torch = _torch()
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return s.cuda_stream
+
+
+def to_device(x, dtype):
+    """numpy array / tensor -> contiguous device tensor of `dtype` (a torch dtype)."""
+    torch = _torch()
+    if isinstance(x, np.ndarray):
+        x = torch.from_numpy(np.ascontiguousarray(x))
+    return x.to(device="cuda", dtype=dtype).contiguous()
+
+
+def default_opts(**kw) -> InvertOpts:
+    o = InvertOpts()
+    lib().gn_invert_opts_default(ctypes.byref(o))
+    for k, v in kw.items():
+        if not hasattr(o, k):
+            raise AttributeError(k)
+        setattr(o, k, v)
+    return o
+
+
+def tree_count(levels: int) -> int:
+    return sum(64 ** l for l in range(levels))
+
+
+_live = weakref.WeakSet()
+
+
+@atexit.register
+def _release_all():
+    # free device objects while the HIP runtime is certainly still up
+    for obj in list(_live):
+        try:
+            obj.close()
+        except Exception:
+            pass
+
+
+class Surface:
+    """A rational B-spline surface resident in HBM (gn_surface).
+
+    Construction mirrors the reference's surface tuple (degree_u, degree_v, cp_u, cp_v, knots_u,
+    knots_v, control_points, weights) passed to GetMatrixN (src/NURBS/Evaluate.cu:3-4); control
+    points are u-major, shape (cp_u, cp_v, 3).
+    """
+
+    def __init__(self, degree_u, degree_v, knots_u, knots_v, control_points, weights, stream=None, _handle=None):
+        self._h = None
+        if _handle is not None:
+            self._h = _handle
+        else:
+            cp = np.ascontiguousarray(control_points, dtype=np.float64)
+            w = np.ascontiguousarray(weights, dtype=np.float64)
+            ku = np.ascontiguousarray(knots_u, dtype=np.float64)
+            kv = np.ascontiguousarray(knots_v, dtype=np.float64)
+            nu, nv = w.shape
+            h = _c_void()
+            _check(lib().gn_surface_create(int(degree_u), int(degree_v), nu, nv, ku.ctypes.data, kv.ctypes.data,
+                                           cp.ctypes.data, w.ctypes.data, _c_void(stream_handle(stream)), ctypes.byref(h)))
+            self._h = h
+        p, q, nu, nv = _c_int(), _c_int(), _c_int(), _c_int()
+        _check(lib().gn_surface_info(self._h, ctypes.byref(p), ctypes.byref(q), ctypes.byref(nu), ctypes.byref(nv)))
+        self.p, self.q, self.nu, self.nv = p.value, q.value, nu.value, nv.value
+        self.tree_levels = 0
+        _live.add(self)
+
+    @classmethod
+    def readNu(cls, path: str, stream=None) -> "Surface":
+        h = _c_void()
+        _check(lib().gn_surface_from_nu(path.encode(), _c_void(stream_handle(stream)), ctypes.byref(h)))
+        return cls(0, 0, None, None, None, None, _handle=h)
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if self._h:
+            lib().gn_surface_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def device_bytes(self) -> int:
+        return int(lib().gn_surface_device_bytes(self._h))
+
+    # ---- span lookup / span table -------------------------------------------------------
+    def findSpan(self, direction: int, u, stream=None):
+        torch = _torch()
+        ud = to_device(u, torch.float64)
+        out = torch.empty(ud.numel(), dtype=torch.int32, device="cuda")
+        _check(lib().gn_find_span(self._h, direction, ud.data_ptr(), ud.numel(), out.data_ptr(), _c_void(stream_handle(stream))))
+        return out
+
+    def GetMatrixN(self, i: int, j: int) -> np.ndarray:
+        """Span matrix of knot spans (i, j) in the reference order [c][r][s] (FP64)."""
+        out = np.zeros((4, self.p + 1, self.q + 1), dtype=np.float64)
+        _check(lib().gn_get_span_matrix(self._h, i, j, out.ctypes.data))
+        return out
+
+    # ---- evaluation ----------------------------------------------------------------------
+    def Evaluate(self, u, v, order: int = 0, precision: int = GN_FP64, out=None, stream=None):
+        """Point and derivatives; returns a device tensor of shape (3*(1, 3, 6)[order], n)."""
+        torch = _torch()
+        nblk = (3, 9, 18)[order] if 0 <= order <= 2 else 3  # bad orders are rejected by the C ABI
+        if precision == GN_FP64:
+            ud, vd = to_device(u, torch.float64), to_device(v, torch.float64)
+            n = ud.numel()
+            out = out if out is not None else torch.empty((nblk, n), dtype=torch.float64, device="cuda")
+            _check(lib().gn_evaluate(self._h, ud.data_ptr(), vd.data_ptr(), n, order, out.data_ptr(), _c_void(stream_handle(stream))))
+        else:
+            ud, vd = to_device(u, torch.float32), to_device(v, torch.float32)
+            n = ud.numel()
+            out = out if out is not None else torch.empty((nblk, n), dtype=torch.float32, device="cuda")
+            _check(lib().gn_evaluate_f32(self._h, ud.data_ptr(), vd.data_ptr(), n, order, out.data_ptr(), _c_void(stream_handle(stream))))
+        return out
+
+    EvaluateRandom = Evaluate
+
+    def EvaluateGrid(self, us, vs, order: int = 0, out=None, stream=None):
+        """FP32 MFMA span-GEMM evaluation on the grid us x vs; returns (nblk, len(us)*len(vs))."""
+        torch = _torch()
+        ud, vd = to_device(us, torch.float32), to_device(vs, torch.float32)
+        nblk = (3, 9, 18)[order] if 0 <= order <= 2 else 3
+        out = out if out is not None else torch.empty((nblk, ud.numel() * vd.numel()), dtype=torch.float32, device="cuda")
+        _check(lib().gn_evaluate_grid_f32(self._h, ud.data_ptr(), ud.numel(), vd.data_ptr(), vd.numel(), order,
+                                          out.data_ptr(), _c_void(stream_handle(stream))))
+        return out
+
+    # ---- tree / seeds / inversion ----------------------------------------------------------
+    def BuildTree(self, levels: int = 3, flags: int = 0, stream=None):
+        _check(lib().gn_build_tree(self._h, levels, flags, _c_void(stream_handle(stream))))
+        self.tree_levels = levels
+
+    def GetTree(self):
+        cnt = tree_count(self.tree_levels)
+        nodes = np.zeros((cnt, 13), dtype=np.float64)
+        valid = np.zeros(cnt, dtype=np.uint8)
+        divided = np.zeros(cnt, dtype=np.uint8)
+        _check(lib().gn_get_tree(self._h, nodes.ctypes.data, valid.ctypes.data, divided.ctypes.data, cnt))
+        return nodes, valid, divided
+
+    def Seed(self, x, y, z, stream=None):
+        torch = _torch()
+        xd, yd, zd = (to_device(a, torch.float64) for a in (x, y, z))
+        n = xd.numel()
+        leaf = torch.empty(n, dtype=torch.int32, device="cuda")
+        su = torch.empty(n, dtype=torch.float64, device="cuda")
+        sv = torch.empty(n, dtype=torch.float64, device="cuda")
+        _check(lib().gn_seed(self._h, xd.data_ptr(), yd.data_ptr(), zd.data_ptr(), n, leaf.data_ptr(), su.data_ptr(),
+                             sv.data_ptr(), _c_void(stream_handle(stream))))
+        return leaf, su, sv
+
+    def NewInverse(self, x, y, z, opts: Optional[InvertOpts] = None, seeds=None, out=None, stream=None,
+                   iters=None):
+        """Invert points (x, y, z); returns (u, v, status) device tensors, plus the per-point FP32
+        and FP64 iteration counts when iters=True."""
+        torch = _torch()
+        xd, yd, zd = (to_device(a, torch.float64) for a in (x, y, z))
+        n = xd.numel()
+        opts = opts if opts is not None else default_opts()
+        if out is not None:
+            u, v, st = out
+        else:
+            u = torch.empty(n, dtype=torch.float64, device="cuda")
+            v = torch.empty(n, dtype=torch.float64, device="cuda")
+            st = torch.empty(n, dtype=torch.uint8, device="cuda")
+        if seeds is not None:
+            u.copy_(to_device(seeds[0], torch.float64))
+            v.copy_(to_device(seeds[1], torch.float64))
+            opts.seed_mode = SEED_GIVEN
+        if iters is None:
+            _check(lib().gn_invert(self._h, xd.data_ptr(), yd.data_ptr(), zd.data_ptr(), n, ctypes.byref(opts),
+                                   u.data_ptr(), v.data_ptr(), st.data_ptr(), _c_void(stream_handle(stream))))
+            return u, v, st
+        it32 = torch.empty(n, dtype=torch.uint8, device="cuda")
+        it64 = torch.empty(n, dtype=torch.uint8, device="cuda")
+        _check(lib().gn_invert_iters(self._h, xd.data_ptr(), yd.data_ptr(), zd.data_ptr(), n, ctypes.byref(opts),
+                                     u.data_ptr(), v.data_ptr(), st.data_ptr(), it32.data_ptr(), it64.data_ptr(),
+                                     _c_void(stream_handle(stream))))
+        return u, v, st, it32, it64
+
+    Invert = NewInverse

@@ -1,0 +1,165 @@
+/*
+ * gnurbs.h -- C ABI of libgnurbs: matrix-form NURBS evaluation and inverse evaluation on one
+ * AMD Instinct MI355X (gfx950).
+ *
+ * The reference (Qiang-Zou/GPU-NURBS) has no library ABI: its operator API is a set of CUDA
+ * kernels launched by src/main.cu.  Each entry point below names the reference interface it
+ * replaces.  Conventions kept from the reference:
+ *   - the caller owns query and result buffers, which are device memory (src/main.cu:176-233);
+ *   - control points are u-major, cp[(iu*nv + iv)*3 + c] (src/main.cu:97-110), weights separate;
+ *   - span indices are the knot indices returned by findSpan (src/NURBS/Evaluate.cu:504-533);
+ *   - tree node indices are base_L + bx*8^L + by with base_L = (64^L - 1)/63
+ *     (src/kernel/kernel.cu:406-407, 546-548; src/kernel/kernel.cuh:13-28).
+ * Differences: errors are returned as gn_status codes, never exit(); per-point inversion outcome
+ * is a status byte instead of an uninitialised flag; all work runs on the caller's stream.
+ *
+ * All functions are thread-safe for distinct objects; a surface may be used concurrently by
+ * several threads for read-only calls (evaluate, invert) once its tables are built.
+ */
+#ifndef GNURBS_H
+#define GNURBS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct gn_surface gn_surface; /* opaque: knots, span tables and tree, resident in HBM */
+typedef struct gn_curve gn_curve;     /* opaque: rational B-spline curve, resident in HBM    */
+typedef struct gn_batch gn_batch;     /* opaque: packed span tables of many surfaces          */
+typedef struct gn_server gn_server;   /* opaque: continuous-batching inversion server         */
+typedef void *gn_stream;              /* hipStream_t; NULL is the null stream                 */
+
+typedef enum {
+    GN_OK = 0,
+    GN_EINVAL = 1,     /* bad argument (sizes, degrees, knots, null pointers)           */
+    GN_ENOMEM = 2,     /* device or host allocation failed                              */
+    GN_EDEVICE = 3,    /* HIP runtime error                                             */
+    GN_EIO = 4,        /* file could not be read or parsed                              */
+    GN_ESTATE = 5,     /* object not ready (e.g. tree not built)                        */
+    GN_EBUDGET = 6,    /* request can never fit the server's memory budget              */
+    GN_ENOTFOUND = 7,  /* unknown request or object id                                  */
+    GN_EAGAIN = 8      /* request not finished yet (gn_server_poll)                     */
+} gn_status;
+
+typedef enum { GN_FP64 = 0, GN_FP32 = 1, GN_MIXED = 2 } gn_precision;
+
+/* per-point inversion outcome (status byte) */
+enum {
+    GN_PT_CONVERGED = 0,      /* step below tol_step                                      */
+    GN_PT_CONVERGED_DIST = 1, /* |S - P| below tol_dist (the reference's rule)            */
+    GN_PT_MAXITER = 2,        /* iteration cap reached                                    */
+    GN_PT_SINGULAR = 3,       /* det(J^T J) == 0                                          */
+    GN_PT_NONFINITE = 4,      /* NaN/Inf met                                              */
+    GN_PT_NOSEED = 5          /* first-hit seeding found no box containing P              */
+};
+
+typedef enum {
+    GN_SEED_NEAREST_LEAF = 0, /* leaf of the tree whose centre point is nearest to P       */
+    GN_SEED_REF_FIRST_HIT = 1,/* reference NewInverse: index-order scan, first box hit     */
+    GN_SEED_GIVEN = 2         /* caller supplies seeds in u_out / v_out                    */
+} gn_seed_mode;
+
+/* tree build flags */
+enum { GN_TREE_REF_PARENT = 1 /* parent = (index-1)/64 as src/kernel/kernel.cu:380,599 */ };
+
+typedef struct {
+    double tol_dist;    /* stop when |S - P| < tol_dist, checked before the update; <= 0 off  */
+    double tol_step;    /* stop when max(|du|,|dv|) < tol_step after an update; <= 0 off      */
+    int max_iter;       /* FP64 (plain) iteration cap; the reference uses 10                  */
+    int clamp;          /* clamp (u,v) to the knot domain after every update                  */
+    int precision;      /* GN_FP64 (plain) or GN_MIXED                                        */
+    int fp32_iters;     /* GN_MIXED: cap of FP32 bulk iterations                              */
+    int fp64_tail;      /* GN_MIXED: FP64 closing iterations                                  */
+    double tol_step32;  /* GN_MIXED: FP32 bulk stops below this step                          */
+    int seed_mode;      /* gn_seed_mode                                                       */
+    int newton;         /* bitmask of GN_NEWTON_*: iterations that add the residual-curvature  */
+                        /* term sum_k r_k d2S_k to J^T J (exact Hessian of |S-P|^2/2, falling  */
+                        /* back to J^T J where that is not positive definite)                  */
+} gn_invert_opts;
+
+enum { GN_NEWTON_FP32 = 1, /* FP32 bulk iterations of GN_MIXED                                */
+       GN_NEWTON_FP64 = 2  /* FP64 iterations (GN_FP64, and the closing iterations of GN_MIXED) */
+};
+
+/* Defaults: the reference iteration in plain FP64 (tol_dist 0, tol_step 1e-10, max_iter 20,
+ * clamp 1, nearest-leaf seeds, newton 0); mixed: <= 20 FP32 iterations to a 1e-6 step, then
+ * <= 3 FP64 closing iterations. */
+void gn_invert_opts_default(gn_invert_opts *opts);
+
+const char *gn_last_error(void);       /* message of the last failure on this thread */
+const char *gn_version(void);
+
+/* ---- surfaces ---------------------------------------------------------------------------- */
+
+/* Tuple constructor: (degree_u, degree_v, cp_u, cp_v, knots_u, knots_v, control_points,
+ * weights) as passed to GetMatrixN (src/NURBS/Evaluate.cu:3-4).  Host pointers; copied.
+ * Degrees 1..7, knots non-decreasing.  Builds the FP64 and FP32 span tables on `stream`. */
+int gn_surface_create(int degree_u, int degree_v, int cp_u, int cp_v, const double *knots_u,
+                      const double *knots_v, const double *xyz, const double *weights,
+                      gn_stream stream, gn_surface **out);
+/* readNu (src/main.cu:45-113): NU/CP/KNu/KNv/DP text format. */
+int gn_surface_from_nu(const char *path, gn_stream stream, gn_surface **out);
+int gn_surface_destroy(gn_surface *s);
+int gn_surface_info(const gn_surface *s, int *degree_u, int *degree_v, int *cp_u, int *cp_v);
+/* HBM bytes held by the object (span tables, knots, tree). */
+size_t gn_surface_device_bytes(const gn_surface *s);
+
+/* findSpan (src/NURBS/Evaluate.cu:504-533) for n parameters: dir 0 = u, 1 = v.
+ * u, span: device pointers. */
+int gn_find_span(const gn_surface *s, int dir, const double *u, size_t n, int32_t *span,
+                 gn_stream stream);
+
+/* GetMatrixN (src/NURBS/Evaluate.cu:3-88): rebuilds the span tables (done by create). */
+int gn_build_span_table(gn_surface *s, gn_stream stream);
+/* One span matrix in the reference's MatrixN order [c][r][s] (c = xw, yw, zw, w), FP64,
+ * host buffer of 4*(p+1)*(q+1) doubles; (i, j) are knot-span indices.  Synchronous. */
+int gn_get_span_matrix(const gn_surface *s, int i, int j, double *out);
+
+/* EvaluateRandom / MatrixSurfacePoint / MatrixSurfaceDerivative{U,V,UU,UV,VV}
+ * (src/NURBS/Evaluate.cu:159-227, 535-930, 1023-1038).  u, v, out: device pointers.
+ * out is SoA: blocks of n values S.x S.y S.z | Su.xyz | Sv.xyz | Suu.xyz | Suv.xyz | Svv.xyz;
+ * max_order 0, 1, 2 writes 3, 9, 18 blocks. */
+int gn_evaluate(const gn_surface *s, const double *u, const double *v, size_t n, int max_order,
+                double *out, gn_stream stream);
+/* FP32 form: parameters are floats (span lookup still compares against FP64 knots). */
+int gn_evaluate_f32(const gn_surface *s, const float *u, const float *v, size_t n,
+                    int max_order, float *out, gn_stream stream);
+/* Tensor-product grid (us[nu_pts] x vs[nv_pts]), FP32 MFMA span-GEMM form; out SoA blocks of
+ * nu_pts*nv_pts values, point (a, b) at a*nv_pts + b.  us, vs ascending. */
+int gn_evaluate_grid_f32(const gn_surface *s, const float *us, int nu_pts, const float *vs,
+                         int nv_pts, int max_order, float *out, gn_stream stream);
+
+/* ---- subdivision tree (GetPoints / GetSecondDerivatives / Max_Sequential_Addressing_Shared /
+ * TightBoundingBox, src/kernel/kernel.cu:375-431, 544-621, 707-733) ------------------------- */
+
+/* levels: 8x8-fan-out levels (reference level = 2 builds levels 0 and 1; 3 gives 64x64 leaves,
+ * i.e. a quadtree of depth 6).  All nodes are computed; valid/divided follow the reference. */
+int gn_build_tree(gn_surface *s, int levels, int flags, gn_stream stream);
+/* Host copy of the tree: nodes[13*count] = uv_lb(2) uv_rt(2) box_lb(3) box_rt(3) centre(3);
+ * valid[count], divided[count]; count = (64^levels - 1)/63.  Synchronous. */
+int gn_get_tree(const gn_surface *s, double *nodes, uint8_t *valid, uint8_t *divided,
+                size_t count);
+/* Seed search alone: leaf index (reference numbering) and seed (u,v) per point. */
+int gn_seed(const gn_surface *s, const double *x, const double *y, const double *z, size_t n,
+            int32_t *leaf, double *seed_u, double *seed_v, gn_stream stream);
+
+/* ---- inversion (NewInverse + GaussNewton, src/kernel/kernel.cu:151-249, 433-469) ----------- */
+
+/* x, y, z: query points (device, SoA).  u_out, v_out, status: device.  With GN_SEED_GIVEN the
+ * seeds are read from u_out/v_out.  Requires gn_build_tree unless seeds are given. */
+int gn_invert(const gn_surface *s, const double *x, const double *y, const double *z, size_t n,
+              const gn_invert_opts *opts, double *u_out, double *v_out, uint8_t *status,
+              gn_stream stream);
+/* gn_invert plus per-point iteration counts (saturated at 255) of the FP32 bulk and of the FP64
+ * iterations; either pointer may be NULL.  Used for algorithmic flop accounting. */
+int gn_invert_iters(const gn_surface *s, const double *x, const double *y, const double *z,
+                    size_t n, const gn_invert_opts *opts, double *u_out, double *v_out,
+                    uint8_t *status, uint8_t *iters_fp32, uint8_t *iters_fp64, gn_stream stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GNURBS_H */

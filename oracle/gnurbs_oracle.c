@@ -1,0 +1,633 @@
+/*
+ * gnurbs_oracle.c -- CPU FP64 oracle for matrix-form NURBS evaluation and inversion.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker: tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py load it (through ctypes, as oracle/liboracle.so) to compare the HIP
+ * product path against.  Nothing in the product library links, calls or falls back to it.
+ *
+ * It restates, in plain C, the algorithms of Qiang-Zou/GPU-NURBS (paths relative to the reference
+ * root).  Every function names the file:line it follows.  Deliberate differences from the reference:
+ *   - derivative chain factor is 1/(k[m+1]-k[m]); the reference uses (k[last]-k[0])/(k[m+1]-k[m])
+ *     (src/NURBS/Evaluate.cu:120, src/kernel/kernel.cu:167-168).  Identical for knots on [0,1].
+ *   - only domain spans m in [p, n-1] are built (the reference also visits, and skips, the
+ *     zero-length end slots; src/NURBS/Evaluate.cu:10-11).
+ *   - the tree uses the geometric parent at every level unless OR_TREE_REF_PARENT is set, which
+ *     reproduces (tree_index-1)/64 of src/kernel/kernel.cu:380,599.
+ *   - Gauss-Newton reports a singular 2x2 system instead of dividing by det == 0
+ *     (src/kernel/kernel.cu:222-226 has no guard).
+ *
+ * Pinning (every exported function is checked by tests/test_oracle.py):
+ *   or_find_span        brute-force linear scan, boundary cases
+ *   or_nrc / basis      independent Cox-de Boor power-basis build (or_basis_matrix_cdb), Bernstein
+ *                       closed form, partition of unity; scipy BSpline
+ *   or_get_matrix_n     contraction against scipy NURBS evaluation; known-answer points of
+ *                       src/kernel/kernel.cu:451-454 on model/center_bk.nu and model/test_face2.nu
+ *   or_eval             scipy NURBS value, finite differences of order k -> k+1, Leibniz form,
+ *                       closed forms (bilinear patch, exact quarter cylinder)
+ *   or_gauss_newton     round trips on on-surface points, orthogonality of the residual
+ *   or_build_tree       containment of dense samples, index formula, NeedDivide truth table
+ *   or_seed_*           brute force
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define OR_MAXK 8 /* MAX_DEGREE of src/NURBS/Evaluate.cuh:15 (k = degree + 1) */
+
+/* ------------------------------------------------------------------------------------------ */
+/* a1: knot-span lookup -- src/NURBS/Evaluate.cu:504-533                                        */
+/* ------------------------------------------------------------------------------------------ */
+int or_find_span(int degree, int cp, const double *knots, double u)
+{
+    int n = cp - 1;
+    if (u >= knots[n + 1]) return n;
+    if (u <= knots[degree]) return degree;
+    int low = degree, high = n + 1;
+    int mid = (int)((low + high) / 2.0);
+    while (u < knots[mid] || u >= knots[mid + 1]) {
+        if (u < knots[mid]) high = mid;
+        else low = mid;
+        mid = (int)((low + high) / 2.0);
+    }
+    return mid;
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* a2, a3: basis-matrix recursion -- src/NURBS/Evaluate.cu:90-108                                */
+/* ------------------------------------------------------------------------------------------ */
+double or_nabla(const double *knot, int d, int i, int k)
+{
+    int index = i + d - 1;
+    return knot[index + k] - knot[index];
+}
+
+double or_nrc(const double *knot, int m, int r, int c, int j, int i, int d)
+{
+    if (m == 0) return ((i + c - 1) == (j + d) && r == 1) ? 1.0 : 0.0;
+    double den = or_nabla(knot, d, j, d - m + 1);
+    double a = or_nabla(knot, d, i, 1) / den;
+    double b = or_nabla(knot, d, j, i - j) / den;
+    return a * or_nrc(knot, m - 1, r - 1, c, j, i, d) + b * or_nrc(knot, m - 1, r, c, j, i, d) +
+           (1 - b) * or_nrc(knot, m - 1, r, c, j - 1, i, d) +
+           (-a) * or_nrc(knot, m - 1, r - 1, c, j - 1, i, d);
+}
+
+/* M[r*k + s] = coefficient of t^r in N_{span-degree+s, degree}(k[span] + t*(k[span+1]-k[span])),
+ * built the way GetMatrixN fills Nu (src/NURBS/Evaluate.cu:15-20). */
+void or_basis_matrix_nrc(int degree, const double *knots, int span, double *M)
+{
+    int k = degree + 1;
+    int ij = span - (degree - 1);
+    for (int r = 0; r < k; r++)
+        for (int s = 0; s < k; s++) M[r * k + s] = or_nrc(knots, degree, r + 1, s + 1, ij, ij, degree);
+}
+
+/* Independent path (not in the reference): Cox-de Boor recursion carried out on power-basis
+ * polynomials in the local parameter t of the span.  0/0 is taken as 0. */
+void or_basis_matrix_cdb(int degree, const double *knots, int span, double *M)
+{
+    int k = degree + 1;
+    double h = knots[span + 1] - knots[span];
+    /* N[a][e]: polynomial coefficients (power e) of N_{span-d+a, d} at the current degree d */
+    double N[OR_MAXK][OR_MAXK], T[OR_MAXK][OR_MAXK];
+    memset(N, 0, sizeof N);
+    N[0][0] = 1.0;
+    for (int d = 1; d <= degree; d++) {
+        memset(T, 0, sizeof T);
+        for (int a = 0; a <= d; a++) {
+            int i = span - d + a; /* global basis index */
+            /* left term: (u - k_i)/(k_{i+d} - k_i) * N_{i,d-1}; N_{i,d-1} is N[a-1] at degree d-1 */
+            if (a >= 1) {
+                double den = knots[i + d] - knots[i];
+                if (den != 0.0) {
+                    double c0 = (knots[span] - knots[i]) / den, c1 = h / den;
+                    for (int e = 0; e < d; e++) {
+                        T[a][e] += c0 * N[a - 1][e];
+                        T[a][e + 1] += c1 * N[a - 1][e];
+                    }
+                }
+            }
+            /* right term: (k_{i+d+1} - u)/(k_{i+d+1} - k_{i+1}) * N_{i+1,d-1} = N[a] at degree d-1 */
+            if (a <= d - 1) {
+                double den = knots[i + d + 1] - knots[i + 1];
+                if (den != 0.0) {
+                    double c0 = (knots[i + d + 1] - knots[span]) / den, c1 = -h / den;
+                    for (int e = 0; e < d; e++) {
+                        T[a][e] += c0 * N[a][e];
+                        T[a][e + 1] += c1 * N[a][e];
+                    }
+                }
+            }
+        }
+        memcpy(N, T, sizeof N);
+    }
+    for (int r = 0; r < k; r++)
+        for (int s = 0; s < k; s++) M[r * k + s] = N[s][r];
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* a4, a5: span table -- MatrixNPosition src/NURBS/Evaluate.cu:110-112, GetMatrixN 3-88          */
+/* ------------------------------------------------------------------------------------------ */
+int or_matrix_n_position(int i, int j, int p, int q, int nu, int nv, int c)
+{
+    (void)nu;
+    return i * (q + nv) * 4 * (p + 1) * (q + 1) + j * 4 * (p + 1) * (q + 1) + c * (p + 1) * (q + 1);
+}
+
+/* MatrixN in the reference's padded layout [(p+nu)][(q+nv)][4][k1][k2]; cp is u-major xyz
+ * (cp[(iu*nv+iv)*3+c], src/main.cu:97-110).  valid[i*(q+nv)+j] = 1 for built slots. */
+void or_get_matrix_n(int p, int q, int nu, int nv, const double *ku, const double *kv,
+                     const double *cp, const double *w, int use_cdb, double *MatrixN,
+                     unsigned char *valid)
+{
+    int k1 = p + 1, k2 = q + 1;
+    size_t slots = (size_t)(p + nu) * (q + nv);
+    memset(MatrixN, 0, slots * 4 * k1 * k2 * sizeof(double));
+    memset(valid, 0, slots);
+    for (int i = p; i < nu; i++) {
+        if (ku[i] == ku[i + 1]) continue;
+        double Nu[OR_MAXK * OR_MAXK];
+        if (use_cdb) or_basis_matrix_cdb(p, ku, i, Nu);
+        else or_basis_matrix_nrc(p, ku, i, Nu);
+        for (int j = q; j < nv; j++) {
+            if (kv[j] == kv[j + 1]) continue;
+            double Nv[OR_MAXK * OR_MAXK], NvT[OR_MAXK * OR_MAXK];
+            if (use_cdb) or_basis_matrix_cdb(q, kv, j, Nv);
+            else or_basis_matrix_nrc(q, kv, j, Nv);
+            for (int m = 0; m < k2; m++)
+                for (int n = 0; n < k2; n++) NvT[n * k2 + m] = Nv[m * k2 + n];
+            for (int c = 0; c < 4; c++) {
+                double mid[OR_MAXK * OR_MAXK], tmp[OR_MAXK * OR_MAXK];
+                for (int s = 0; s < k1; s++)
+                    for (int l = 0; l < k2; l++) {
+                        size_t g = (size_t)(s + i - p) * nv + (l + j - q);
+                        double wt = w[g];
+                        mid[s * k2 + l] = (c == 3) ? wt : cp[g * 3 + c] * wt;
+                    }
+                /* tmp = mid * NvT (src/NURBS/Evaluate.cu:58-61) */
+                for (int a = 0; a < k1; a++)
+                    for (int b = 0; b < k2; b++) {
+                        double sum = 0;
+                        for (int l = 0; l < k2; l++) sum += mid[a * k2 + l] * NvT[l * k2 + b];
+                        tmp[a * k2 + b] = sum;
+                    }
+                /* out = Nu * tmp (src/NURBS/Evaluate.cu:63-66) */
+                double *out = MatrixN + or_matrix_n_position(i, j, p, q, nu, nv, c);
+                for (int a = 0; a < k1; a++)
+                    for (int b = 0; b < k2; b++) {
+                        double sum = 0;
+                        for (int l = 0; l < k1; l++) sum += Nu[a * k1 + l] * tmp[l * k2 + b];
+                        out[a * k2 + b] = sum;
+                    }
+            }
+            valid[(size_t)i * (q + nv) + j] = 1;
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* a6-a9: evaluation -- MatrixSurfacePoint src/NURBS/Evaluate.cu:159-227, old-version rational   */
+/* derivatives 535-930, EvaluateRandom 1023-1038, homogenousToCartesian src/util/ExtraMath.cu:3-5 */
+/* ------------------------------------------------------------------------------------------ */
+typedef struct {
+    int p, q, nu, nv;
+    const double *ku, *kv, *MatrixN;
+} or_surf;
+
+/* y[i] = sum_l a[i*n + l] * b[l]   (matrix_mult with k == 1, src/util/Matrix.cu:43-52) */
+static void mv(const double *a, const double *b, double *y, int m, int n)
+{
+    for (int i = 0; i < m; i++) {
+        double s = 0;
+        for (int l = 0; l < n; l++) s += a[i * n + l] * b[l];
+        y[i] = s;
+    }
+}
+/* y[j] = sum_l a[l] * b[l*k + j]   (matrix_mult with m == 1) */
+static void vm(const double *a, const double *b, double *y, int n, int k)
+{
+    for (int j = 0; j < k; j++) {
+        double s = 0;
+        for (int l = 0; l < n; l++) s += a[l] * b[l * k + j];
+        y[j] = s;
+    }
+}
+static double dot(const double *a, const double *b, int n)
+{
+    double s = 0;
+    for (int l = 0; l < n; l++) s += a[l] * b[l];
+    return s;
+}
+
+static void monomials(double t, int k, double *V, double *dV, double *ddV)
+{
+    double now = 1;
+    for (int i = 0; i < k; i++) { V[i] = now; now *= t; }
+    now = 1;
+    dV[0] = 0;
+    for (int i = 0; i < k - 1; i++) { dV[i + 1] = (i + 1) * now; now *= t; }
+    now = 1;
+    ddV[0] = 0;
+    if (k > 1) ddV[1] = 0;
+    for (int i = 0; i < k - 2; i++) { ddV[i + 2] = (i + 2) * (i + 1) * now; now *= t; }
+}
+
+typedef struct { double x, y, z, w; } d4;
+
+/* Homogeneous contraction R = U^T * M_c * V for the four components c (MatrixSurfacePoint). */
+static d4 contract(const double *mid, int k1, int k2, const double *U, const double *V)
+{
+    double t0[OR_MAXK], t1[OR_MAXK], t2[OR_MAXK], t3[OR_MAXK];
+    int kk = k1 * k2;
+    mv(mid + 0 * kk, V, t0, k1, k2);
+    mv(mid + 1 * kk, V, t1, k1, k2);
+    mv(mid + 2 * kk, V, t2, k1, k2);
+    mv(mid + 3 * kk, V, t3, k1, k2);
+    d4 r = {dot(U, t0, k1), dot(U, t1, k1), dot(U, t2, k1), dot(U, t3, k1)};
+    return r;
+}
+/* VV variant contracts U first (src/NURBS/Evaluate.cu:883-912) */
+static d4 contract_ufirst(const double *mid, int k1, int k2, const double *U, const double *V)
+{
+    double t0[OR_MAXK], t1[OR_MAXK], t2[OR_MAXK], t3[OR_MAXK];
+    int kk = k1 * k2;
+    vm(U, mid + 0 * kk, t0, k1, k2);
+    vm(U, mid + 1 * kk, t1, k1, k2);
+    vm(U, mid + 2 * kk, t2, k1, k2);
+    vm(U, mid + 3 * kk, t3, k1, k2);
+    d4 r = {dot(t0, V, k2), dot(t1, V, k2), dot(t2, V, k2), dot(t3, V, k2)};
+    return r;
+}
+
+typedef struct {
+    double S[3], Su[3], Sv[3], Suu[3], Suv[3], Svv[3];
+    double R[4];
+    int m, n;
+} or_pt;
+
+/* Point and Cartesian derivatives up to `order` (0..2) at global (u, v). */
+static void eval_one(const or_surf *sf, double uu, double vv, int order, or_pt *o)
+{
+    int p = sf->p, q = sf->q, k1 = p + 1, k2 = q + 1;
+    int m = or_find_span(p, sf->nu, sf->ku, uu);
+    int n = or_find_span(q, sf->nv, sf->kv, vv);
+    double hu = sf->ku[m + 1] - sf->ku[m], hv = sf->kv[n + 1] - sf->kv[n];
+    double u = (uu - sf->ku[m]) / hu;
+    double v = (vv - sf->kv[n]) / hv;
+    double spanU = 1.0 / hu, spanV = 1.0 / hv;
+    const double *mid = sf->MatrixN + or_matrix_n_position(m, n, p, q, sf->nu, sf->nv, 0);
+    double U[OR_MAXK], dU[OR_MAXK], ddU[OR_MAXK], V[OR_MAXK], dV[OR_MAXK], ddV[OR_MAXK];
+    monomials(u, k1, U, dU, ddU);
+    monomials(v, k2, V, dV, ddV);
+    d4 R = contract(mid, k1, k2, U, V);
+    o->m = m;
+    o->n = n;
+    o->R[0] = R.x; o->R[1] = R.y; o->R[2] = R.z; o->R[3] = R.w;
+    double Rc[3] = {R.x, R.y, R.z};
+    for (int c = 0; c < 3; c++) o->S[c] = Rc[c] / R.w;
+    if (order < 1) return;
+    double h = R.w;
+    /* MatrixSurfaceDerivativeU / V, src/NURBS/Evaluate.cu:535-665 */
+    d4 RdU = contract(mid, k1, k2, dU, V);
+    d4 RdV = contract(mid, k1, k2, U, dV);
+    double RdUc[3] = {RdU.x, RdU.y, RdU.z}, RdVc[3] = {RdV.x, RdV.y, RdV.z};
+    for (int c = 0; c < 3; c++) {
+        o->Su[c] = (Rc[c] * (-RdU.w / pow(h, 2)) + RdUc[c] / h) * spanU;
+        o->Sv[c] = (Rc[c] * (-RdV.w / pow(h, 2)) + RdVc[c] / h) * spanV;
+    }
+    if (order < 2) return;
+    /* MatrixSurfaceDerivativeUU / UV / VV, src/NURBS/Evaluate.cu:667-930 */
+    d4 RdUU = contract(mid, k1, k2, ddU, V);
+    d4 RdUV = contract(mid, k1, k2, dU, dV);
+    d4 RdVV = contract_ufirst(mid, k1, k2, U, ddV);
+    double RdUUc[3] = {RdUU.x, RdUU.y, RdUU.z}, RdUVc[3] = {RdUV.x, RdUV.y, RdUV.z},
+           RdVVc[3] = {RdVV.x, RdVV.y, RdVV.z};
+    for (int c = 0; c < 3; c++) {
+        double huu = Rc[c] * ((2 * pow(RdU.w, 2) / pow(h, 3)) - (RdUU.w / pow(h, 2))) +
+                     2 * RdUc[c] * (-RdU.w / pow(h, 2)) + RdUUc[c] / h;
+        double hvv = Rc[c] * ((2 * pow(RdV.w, 2) / pow(h, 3)) - (RdVV.w / pow(h, 2))) +
+                     2 * RdVc[c] * (-RdV.w / pow(h, 2)) + RdVVc[c] / h;
+        double huv = (RdUVc[c] * h - RdV.w * RdUc[c] - RdVc[c] * RdU.w - Rc[c] * RdUV.w) / pow(h, 2) +
+                     (2 * Rc[c] * RdU.w * RdV.w) / pow(h, 3);
+        o->Suu[c] = huu * spanU * spanU;
+        o->Svv[c] = hvv * spanV * spanV;
+        o->Suv[c] = huv * spanU * spanV;
+    }
+}
+
+/* Batch evaluation (EvaluateRandom, src/NURBS/Evaluate.cu:1023-1038, extended to order <= 2).
+ * out is SoA: [S.x S.y S.z | Su.xyz | Sv.xyz | Suu.xyz | Suv.xyz | Svv.xyz], each block n long;
+ * only the first 3*(1 + 2*(order>=1) + 3*(order>=2)) blocks are written. */
+void or_eval(int p, int q, int nu, int nv, const double *ku, const double *kv,
+             const double *MatrixN, const double *u, const double *v, long n, int order,
+             double *out, int nthreads)
+{
+    or_surf sf = {p, q, nu, nv, ku, kv, MatrixN};
+#ifdef _OPENMP
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel for schedule(static) num_threads(nthreads)
+#else
+    (void)nthreads;
+#endif
+    for (long i = 0; i < n; i++) {
+        or_pt o;
+        eval_one(&sf, u[i], v[i], order, &o);
+        for (int c = 0; c < 3; c++) {
+            out[(0 + c) * n + i] = o.S[c];
+            if (order >= 1) {
+                out[(3 + c) * n + i] = o.Su[c];
+                out[(6 + c) * n + i] = o.Sv[c];
+            }
+            if (order >= 2) {
+                out[(9 + c) * n + i] = o.Suu[c];
+                out[(12 + c) * n + i] = o.Suv[c];
+                out[(15 + c) * n + i] = o.Svv[c];
+            }
+        }
+    }
+}
+
+/* Homogeneous point R (4 values per point, AoS) -- used by the tree build (GetPoints). */
+void or_eval_homogeneous(int p, int q, int nu, int nv, const double *ku, const double *kv,
+                         const double *MatrixN, const double *u, const double *v, long n,
+                         double *R4)
+{
+    or_surf sf = {p, q, nu, nv, ku, kv, MatrixN};
+    for (long i = 0; i < n; i++) {
+        or_pt o;
+        eval_one(&sf, u[i], v[i], 0, &o);
+        for (int c = 0; c < 4; c++) R4[i * 4 + c] = o.R[c];
+    }
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* a20: Gauss-Newton -- src/kernel/kernel.cu:151-249                                            */
+/* ------------------------------------------------------------------------------------------ */
+enum { OR_CONV_STEP = 0, OR_CONV_DIST = 1, OR_MAXITER = 2, OR_SINGULAR = 3, OR_NONFINITE = 4,
+       OR_NOSEED = 5 };
+
+typedef struct {
+    double tol_dist; /* stop when |S - P| < tol_dist, before the update (reference rule) */
+    double tol_step; /* stop when max(|du|,|dv|) < tol_step, after the update (<= 0: off) */
+    int max_iter;    /* reference: 10 */
+    int clamp;       /* clamp (u,v) to the knot domain after each update */
+    int newton;      /* 1: add the residual-curvature term to J^T J (exact Hessian) */
+} or_gn_opts;
+
+/* Returns status; (u,v) is updated in place; *iters gets the number of iterations begun. */
+static int gn_one(const or_surf *sf, const double P[3], const or_gn_opts *op, double *pu, double *pv,
+                  int *iters)
+{
+    double cu = *pu, cv = *pv;
+    double ulo = sf->ku[sf->p], uhi = sf->ku[sf->nu], vlo = sf->kv[sf->q], vhi = sf->kv[sf->nv];
+    int st = OR_MAXITER, it;
+    for (it = 0; it < op->max_iter; it++) {
+        or_pt o;
+        eval_one(sf, cu, cv, op->newton ? 2 : 1, &o);
+        double d[3] = {o.S[0] - P[0], o.S[1] - P[1], o.S[2] - P[2]};
+        double dist = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+        if (!(dist == dist)) { st = OR_NONFINITE; break; }
+        if (dist < op->tol_dist) { st = OR_CONV_DIST; break; }
+        /* J = [Su Sv]; (J^T J)^-1 J^T (S - P), src/kernel/kernel.cu:195-239 */
+        double a = 0, b = 0, c = 0, gu = 0, gv = 0;
+        for (int k = 0; k < 3; k++) {
+            a += o.Su[k] * o.Su[k];
+            b += o.Su[k] * o.Sv[k];
+            c += o.Sv[k] * o.Sv[k];
+            gu += o.Su[k] * d[k];
+            gv += o.Sv[k] * d[k];
+        }
+        if (op->newton) {
+            double a2 = a, b2 = b, c2 = c;
+            for (int k = 0; k < 3; k++) {
+                a2 += d[k] * o.Suu[k];
+                b2 += d[k] * o.Suv[k];
+                c2 += d[k] * o.Svv[k];
+            }
+            if (a2 > 0 && a2 * c2 - b2 * b2 > 0) { a = a2; b = b2; c = c2; }
+        }
+        double det = a * c - b * b;
+        if (!(det == det)) { st = OR_NONFINITE; break; }
+        if (det == 0) { st = OR_SINGULAR; break; }
+        double du = (c * gu - b * gv) / det;
+        double dv = (a * gv - b * gu) / det;
+        cu -= du;
+        cv -= dv;
+        if (op->clamp) {
+            cu = cu < ulo ? ulo : (cu > uhi ? uhi : cu);
+            cv = cv < vlo ? vlo : (cv > vhi ? vhi : cv);
+        }
+        if (!(cu == cu) || !(cv == cv)) { st = OR_NONFINITE; break; }
+        if (op->tol_step > 0 && fabs(du) < op->tol_step && fabs(dv) < op->tol_step) {
+            st = OR_CONV_STEP;
+            it++;
+            break;
+        }
+    }
+    *pu = cu;
+    *pv = cv;
+    if (iters) *iters = it;
+    return st;
+}
+
+/* Batch Gauss-Newton from given seeds.  xyz SoA (x[n], y[n], z[n]). */
+void or_gauss_newton(int p, int q, int nu, int nv, const double *ku, const double *kv,
+                     const double *MatrixN, const double *x, const double *y, const double *z,
+                     long n, double tol_dist, double tol_step, int max_iter, int clamp, int newton,
+                     const double *seed_u, const double *seed_v, double *u_out, double *v_out,
+                     unsigned char *status, int *iters, int nthreads)
+{
+    or_surf sf = {p, q, nu, nv, ku, kv, MatrixN};
+    or_gn_opts op = {tol_dist, tol_step, max_iter, clamp, newton};
+#ifdef _OPENMP
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 256) num_threads(nthreads)
+#else
+    (void)nthreads;
+#endif
+    for (long i = 0; i < n; i++) {
+        double P[3] = {x[i], y[i], z[i]};
+        double cu = seed_u[i], cv = seed_v[i];
+        int it = 0;
+        int st = gn_one(&sf, P, &op, &cu, &cv, &it);
+        u_out[i] = cu;
+        v_out[i] = cv;
+        if (status) status[i] = (unsigned char)st;
+        if (iters) iters[i] = it;
+    }
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* a13-a18: subdivision tree -- GetPoints src/kernel/kernel.cu:404-431, GetSecondDerivatives     */
+/* 375-402, Max_Sequential_Addressing_Shared 707-733, TightBoundingBox 544-621, NeedDivide       */
+/* 859-878, min/max(TreeNode) 885-901, constants src/kernel/kernel.cuh:13-39                     */
+/* ------------------------------------------------------------------------------------------ */
+#define OR_GAP 8
+#define OR_PPE (OR_GAP + 1)
+#define OR_FAN (OR_GAP * OR_GAP)
+#define OR_THRESHOLD 0.5
+#define OR_TREE_REF_PARENT 1
+
+long or_tree_base(int level)
+{
+    long b = 0, g = 1;
+    for (int l = 0; l < level; l++) { b += g; g *= OR_FAN; }
+    return b;
+}
+
+static int need_divide(const double lb[3], const double rt[3])
+{
+    double dx = rt[0] - lb[0], dy = rt[1] - lb[1], dz = rt[2] - lb[2], t;
+    if (dx > dy) { t = dx; dx = dy; dy = t; }
+    if (dx > dz) { t = dx; dx = dz; dz = t; }
+    if ((dx < OR_THRESHOLD * dy) && (dx < OR_THRESHOLD * dz)) return 0;
+    return 1;
+}
+int or_need_divide(const double *lb, const double *rt) { return need_divide(lb, rt); }
+
+/* Node record (13 doubles): uv_lb(2) uv_rt(2) lb(3) rt(3) centre point C(3).  Every node of
+ * `levels` levels is computed (as if every parent were divided); valid[] marks the nodes the
+ * reference itself would build (root, and children of divided nodes), divided[] = NeedDivide. */
+void or_build_tree(int p, int q, int nu, int nv, const double *ku, const double *kv,
+                   const double *MatrixN, int levels, int flags, double *nodes,
+                   unsigned char *valid, unsigned char *divided)
+{
+    or_surf sf = {p, q, nu, nv, ku, kv, MatrixN};
+    long g = 1;
+    for (int L = 0; L < levels; L++, g *= OR_GAP) {
+        long base = or_tree_base(L);
+        for (long bx = 0; bx < g; bx++)
+            for (long by = 0; by < g; by++) {
+                long idx = bx * g + by, ti = base + idx;
+                double base_unit = 1.0 / g, unit = base_unit / OR_GAP;
+                double mx = -INFINITY, lo[3] = {INFINITY, INFINITY, INFINITY},
+                       hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+                double M1 = 0, M2 = 0, M3 = 0;
+                for (int tx = 0; tx < OR_PPE; tx++)
+                    for (int ty = 0; ty < OR_PPE; ty++) {
+                        double su = base_unit * bx + tx * unit, sv = base_unit * by + ty * unit;
+                        or_pt o;
+                        eval_one(&sf, su, sv, 2, &o);
+                        for (int c = 0; c < 3; c++) {
+                            double s = o.R[c] / o.R[3];
+                            if (s > hi[c]) hi[c] = s;
+                            if (s < lo[c]) lo[c] = s;
+                        }
+                        double e1 = fmax(fabs(o.Suu[0]), fmax(fabs(o.Suu[1]), fabs(o.Suu[2])));
+                        double e2 = fmax(fabs(o.Suv[0]), fmax(fabs(o.Suv[1]), fabs(o.Suv[2])));
+                        double e3 = fmax(fabs(o.Svv[0]), fmax(fabs(o.Svv[1]), fabs(o.Svv[2])));
+                        if (e1 > M1) M1 = e1;
+                        if (e2 > M2) M2 = e2;
+                        if (e3 > M3) M3 = e3;
+                    }
+                (void)mx;
+                double K = (M1 + 2 * M2 + M3) * sqrt(3.0) /
+                           (8.0 * OR_GAP * (double)g * OR_GAP * (double)g);
+                double *nd = nodes + ti * 13;
+                double unitg = 1.0 / g;
+                nd[0] = bx * unitg;
+                nd[1] = by * unitg;
+                nd[2] = (bx + 1) * unitg;
+                nd[3] = (by + 1) * unitg;
+                for (int c = 0; c < 3; c++) {
+                    nd[4 + c] = lo[c] - K;
+                    nd[7 + c] = hi[c] + K;
+                }
+                long parent = -1;
+                if (L > 0) {
+                    if (flags & OR_TREE_REF_PARENT) parent = (ti - 1) / OR_FAN;
+                    else parent = or_tree_base(L - 1) + (bx / OR_GAP) * (g / OR_GAP) + by / OR_GAP;
+                }
+                if (parent >= 0) {
+                    const double *pn = nodes + parent * 13;
+                    for (int c = 0; c < 3; c++) {
+                        if (pn[7 + c] < nd[7 + c]) nd[7 + c] = pn[7 + c];
+                        if (pn[4 + c] > nd[4 + c]) nd[4 + c] = pn[4 + c];
+                    }
+                }
+                valid[ti] = (parent < 0) ? 1 : (unsigned char)(valid[parent] && divided[parent]);
+                divided[ti] = (unsigned char)need_divide(nd + 4, nd + 7);
+                /* centre point: the surface at the uv-rect centre used as the GN seed
+                 * (src/kernel/kernel.cu:156) */
+                double cu = (nd[0] + nd[2]) / 2, cv = (nd[1] + nd[3]) / 2;
+                or_pt o;
+                eval_one(&sf, cu, cv, 0, &o);
+                for (int c = 0; c < 3; c++) nd[10 + c] = o.S[c];
+            }
+    }
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* a19: seed search -- NewInverse src/kernel/kernel.cu:433-464, NewBoundingBoxCheck 466-469      */
+/* ------------------------------------------------------------------------------------------ */
+static int box_has(const double *nd, const double P[3])
+{
+    return (P[0] >= nd[4]) && (P[1] >= nd[5]) && (P[2] >= nd[6]) && (P[0] <= nd[7]) &&
+           (P[1] <= nd[8]) && (P[2] <= nd[9]);
+}
+
+/* Reference inversion: scan valid nodes in index order; the first closed box holding P seeds
+ * Gauss-Newton at the node's uv centre; stop at the first success (NewInverse). */
+void or_invert_first_hit(int p, int q, int nu, int nv, const double *ku, const double *kv,
+                         const double *MatrixN, const double *nodes, const unsigned char *valid,
+                         long n_nodes, const double *x, const double *y, const double *z, long n,
+                         double tol_dist, double tol_step, int max_iter, int clamp,
+                         double *u_out, double *v_out, unsigned char *status, int *node_out,
+                         int nthreads)
+{
+    or_surf sf = {p, q, nu, nv, ku, kv, MatrixN};
+    or_gn_opts op = {tol_dist, tol_step, max_iter, clamp, 0};
+#ifdef _OPENMP
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 64) num_threads(nthreads)
+#else
+    (void)nthreads;
+#endif
+    for (long i = 0; i < n; i++) {
+        double P[3] = {x[i], y[i], z[i]};
+        int st = OR_NOSEED, hit = -1;
+        double ru = NAN, rv = NAN;
+        for (long t = 0; t < n_nodes; t++) {
+            if (!valid[t]) continue;
+            const double *nd = nodes + t * 13;
+            if (!box_has(nd, P)) continue;
+            double cu = (nd[0] + nd[2]) / 2, cv = (nd[1] + nd[3]) / 2;
+            int s = gn_one(&sf, P, &op, &cu, &cv, NULL);
+            hit = (int)t;
+            ru = cu;
+            rv = cv;
+            st = s;
+            if (s == OR_CONV_DIST || s == OR_CONV_STEP) break;
+        }
+        u_out[i] = ru;
+        v_out[i] = rv;
+        status[i] = (unsigned char)st;
+        if (node_out) node_out[i] = hit;
+    }
+}
+
+/* Nearest leaf: the leaf (level `levels-1` node) whose centre point is closest to P, ties to
+ * the lower index; brute force over all leaves. */
+void or_seed_nearest_leaf(const double *nodes, int levels, const double *x, const double *y,
+                          const double *z, long n, int *leaf_out, double *seed_u, double *seed_v)
+{
+    long base = or_tree_base(levels - 1), cnt = or_tree_base(levels) - base;
+    for (long i = 0; i < n; i++) {
+        double best = INFINITY;
+        long bi = -1;
+        for (long t = 0; t < cnt; t++) {
+            const double *nd = nodes + (base + t) * 13;
+            double dx = x[i] - nd[10], dy = y[i] - nd[11], dz = z[i] - nd[12];
+            double d2 = dx * dx + dy * dy + dz * dz;
+            if (d2 < best) { best = d2; bi = t; }
+        }
+        leaf_out[i] = (int)(base + bi);
+        const double *nd = nodes + (base + bi) * 13;
+        seed_u[i] = (nd[0] + nd[2]) / 2;
+        seed_v[i] = (nd[1] + nd[3]) / 2;
+    }
+}
