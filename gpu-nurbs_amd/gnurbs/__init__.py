@@ -89,6 +89,13 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"{LIB_PATH} is missing: build it with __graft_entry__.build()")
+        # torch ships its own libamdhip64.so.7 (same SONAME as /opt/rocm's).  Loading torch first
+        # makes libgnurbs bind to that already-loaded runtime, so the process has one HIP runtime;
+        # loading ours first would pair the system runtime with torch's HSA library.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = ctypes.CDLL(LIB_PATH)
         for name, res, args in _SIGS:
             if not hasattr(L, name):
