@@ -51,7 +51,7 @@ void gn_invert_opts_default(gn_invert_opts *o) {
     o->fp32_iters = 20;
     o->fp64_tail = 3;
     o->tol_step32 = 1e-6;
-    o->seed_mode = GN_SEED_NEAREST_LEAF;
+    o->seed_mode = GN_SEED_GREEDY_LEAF;
     o->newton = 0;
 }
 
@@ -62,10 +62,12 @@ static void free_tree(gn_surface *s) {
     (void)hipFree(s->d_sbox);
     (void)hipFree(s->d_scentre);
     (void)hipFree(s->d_sseed);
+    (void)hipFree(s->d_sbox32);
     (void)hipFree(s->d_sref);
     s->d_tree = nullptr;
     s->d_valid = s->d_divided = nullptr;
     s->d_sbox = s->d_scentre = s->d_sseed = nullptr;
+    s->d_sbox32 = nullptr;
     s->d_sref = nullptr;
     s->tree_levels = 0;
 }
@@ -288,6 +290,7 @@ int gn_build_tree(gn_surface *s, int levels, int flags, gn_stream stream) {
         (rc = dalloc(&s->d_valid, (size_t)count, acc)) ||
         (rc = dalloc(&s->d_divided, (size_t)count, acc)) ||
         (rc = dalloc(&s->d_sbox, (size_t)qnodes * 6, acc)) ||
+        (rc = dalloc(&s->d_sbox32, (size_t)qnodes * 6, acc)) ||
         (rc = dalloc(&s->d_scentre, (size_t)leaves * 3, acc)) ||
         (rc = dalloc(&s->d_sseed, (size_t)leaves * 2, acc)) ||
         (rc = dalloc(&s->d_sref, (size_t)leaves, acc))) {
@@ -299,8 +302,8 @@ int gn_build_tree(gn_surface *s, int levels, int flags, gn_stream stream) {
         rc = gn::launch_tree_level(d, l, flags, s->d_tree, s->d_valid, s->d_divided, st);
         if (rc) { free_tree(s); return rc; }
     }
-    rc = gn::launch_search_build(d, levels, s->d_tree, s->d_sbox, s->d_scentre, s->d_sseed,
-                                 s->d_sref, st);
+    rc = gn::launch_search_build(d, levels, s->d_tree, s->d_sbox, s->d_sbox32, s->d_scentre,
+                                 s->d_sseed, s->d_sref, st);
     if (rc) { free_tree(s); return rc; }
     s->tree_levels = levels;
     s->device_bytes += acc;
@@ -321,12 +324,13 @@ int gn_get_tree(const gn_surface *s, double *nodes, uint8_t *valid, uint8_t *div
     return GN_OK;
 }
 
-int gn_seed(const gn_surface *s, const double *x, const double *y, const double *z, size_t n,
-            int32_t *leaf, double *su, double *sv, gn_stream stream) {
+int gn_seed(const gn_surface *s, int exact, const double *x, const double *y, const double *z,
+            size_t n, int32_t *leaf, double *su, double *sv, gn_stream stream) {
     if (!s) return fail(GN_EINVAL, "null surface");
     if (!s->tree_levels) return fail(GN_ESTATE, "tree not built");
     if (n && (!x || !y || !z || !su || !sv)) return fail(GN_EINVAL, "null buffer");
-    return gn::launch_seed(s->search(), x, y, z, n, leaf, su, sv, (hipStream_t)stream);
+    return gn::launch_seed(s->search(), exact ? 0 : 1, x, y, z, n, leaf, su, sv,
+                           (hipStream_t)stream);
 }
 
 int gn_invert(const gn_surface *s, const double *x, const double *y, const double *z, size_t n,
@@ -345,7 +349,7 @@ int gn_invert_iters(const gn_surface *s, const double *x, const double *y, const
     if (n && (!x || !y || !z || !u || !v || !status)) return fail(GN_EINVAL, "null buffer");
     if (o.precision != GN_FP64 && o.precision != GN_MIXED)
         return fail(GN_EINVAL, "precision must be GN_FP64 or GN_MIXED");
-    if (o.seed_mode < GN_SEED_NEAREST_LEAF || o.seed_mode > GN_SEED_GIVEN)
+    if (o.seed_mode < GN_SEED_NEAREST_LEAF || o.seed_mode > GN_SEED_GREEDY_LEAF)
         return fail(GN_EINVAL, "bad seed_mode");
     if (o.max_iter < 0 || o.fp32_iters < 0 || o.fp64_tail < 0 || o.max_iter > 10000 ||
         o.fp32_iters > 10000 || o.fp64_tail > 10000)

@@ -128,13 +128,17 @@ struct Cart {  // Cartesian point and derivatives in the global parameters
     T S[3], Su[3], Sv[3], Suu[3], Suv[3], Svv[3];
 };
 
+// Reciprocal: correctly rounded in FP64; the hardware reciprocal (1 ulp) in FP32.
+__device__ __forceinline__ double recip(double x) { return 1.0 / x; }
+__device__ __forceinline__ float recip(float x) { return __builtin_amdgcn_rcpf(x); }
+
 // Rational quotient (the old-version rational derivatives of src/NURBS/Evaluate.cu:535-930,
 // written in the equivalent Leibniz form S_u = (A_u - w_u S)/w, ...) and the chain factors
 // 1/(k[m+1]-k[m]) of the span (the reference multiplies by (k[last]-k[0])/(k[m+1]-k[m]),
 // which is the same for knots on [0, 1]).
 template <int ORDER, typename T>
 __device__ __forceinline__ void quotient(const HomDer<T> &h, T ihu, T ihv, Cart<T> &o) {
-    T iw = T(1) / h.R.w;
+    T iw = recip(h.R.w);
     o.S[0] = h.R.x * iw;
     o.S[1] = h.R.y * iw;
     o.S[2] = h.R.z * iw;
@@ -187,6 +191,44 @@ __device__ __forceinline__ void eval_point(const GnSurfDev &d, const double *__r
     quotient<ORDER, T>(h, T(ihu), T(ihv), o);
     if (pm) *pm = m;
     if (pn) *pn = n;
+}
+
+// Per-lane cache of the current span pair: a parameter that stays inside [k[m], k[m+1]) has
+// exactly the span m (the bracketing span is unique), so the search is skipped.
+struct SpanCache {
+    double ulo = INFINITY, uhi = -INFINITY, vlo = INFINITY, vhi = -INFINITY;
+    double ihu = 0, ihv = 0;
+    int im = 0, in = 0;  // span slots m - p, n - q
+};
+
+template <int K1, int K2, int ORDER, bool EXACT, typename T>
+__device__ __forceinline__ void eval_point_cached(const GnSurfDev &d, const double *__restrict__ ku,
+                                                  const double *__restrict__ kv, double u, double v,
+                                                  SpanCache &sc, Cart<T> &o) {
+    if (!(u >= sc.ulo && u < sc.uhi)) {
+        const int m = find_span(ku, d.p, d.nu, u);
+        sc.ulo = ku[m];
+        sc.uhi = ku[m + 1];
+        sc.im = m - d.p;
+        sc.ihu = d.ihu[sc.im];
+    }
+    if (!(v >= sc.vlo && v < sc.vhi)) {
+        const int n = find_span(kv, d.q, d.nv, v);
+        sc.vlo = kv[n];
+        sc.vhi = kv[n + 1];
+        sc.in = n - d.q;
+        sc.ihv = d.ihv[sc.in];
+    }
+    const T t = T((u - sc.ulo) * sc.ihu);
+    const T s = T((v - sc.vlo) * sc.ihv);
+    const int k1 = d.p + 1, k2 = d.q + 1;
+    const T *tab = nullptr;
+    if constexpr (sizeof(T) == 8) tab = reinterpret_cast<const T *>(d.tab64);
+    else tab = reinterpret_cast<const T *>(d.tab32);
+    const T *patch = tab + (size_t)(sc.im * d.sv + sc.in) * (size_t)(k1 * k2 * 4);
+    HomDer<T> h;
+    contract<K1, K2, ORDER, EXACT, T>(patch, k1, k2, t, s, h);
+    quotient<ORDER, T>(h, T(sc.ihu), T(sc.ihv), o);
 }
 
 // Dispatch helper: calls F.template operator()<K1, K2, EXACT>() for the degree pair of a

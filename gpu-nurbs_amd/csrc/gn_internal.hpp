@@ -35,6 +35,7 @@ struct GnTreeNode {
 struct GnSearchDev {
     int depth;                // D = 3*(levels-1)
     const double *box;        // per node 6 doubles lo.xyz hi.xyz, level offsets (4^l-1)/3
+    const float *box32;       // the same boxes in FP32, rounded outward and widened by a margin
     const double *centre;     // per leaf (Morton order) centre point xyz
     const double *seed_uv;    // per leaf (Morton order) seed (u, v)
     const int32_t *ref_index; // per leaf (Morton order) reference tree index
@@ -72,6 +73,7 @@ struct gn_surface {
     GnTreeNode *d_tree = nullptr;
     uint8_t *d_valid = nullptr, *d_divided = nullptr;
     double *d_sbox = nullptr, *d_scentre = nullptr, *d_sseed = nullptr;
+    float *d_sbox32 = nullptr;
     int32_t *d_sref = nullptr;
     size_t device_bytes = 0;
 
@@ -86,7 +88,8 @@ struct gn_surface {
     GnSearchDev search() const {
         GnSearchDev s;
         s.depth = 3 * (tree_levels - 1);
-        s.box = d_sbox; s.centre = d_scentre; s.seed_uv = d_sseed; s.ref_index = d_sref;
+        s.box = d_sbox; s.box32 = d_sbox32; s.centre = d_scentre; s.seed_uv = d_sseed;
+        s.ref_index = d_sref;
         return s;
     }
 };
@@ -108,9 +111,12 @@ int launch_eval_grid32(const GnSurfDev &d, const float *us, int nu_pts, const fl
 int launch_tree_level(const GnSurfDev &d, int level, int flags, GnTreeNode *tree,
                       uint8_t *valid, uint8_t *divided, hipStream_t st);
 int launch_search_build(const GnSurfDev &d, int levels, const GnTreeNode *tree, double *sbox,
-                        double *scentre, double *sseed, int32_t *sref, hipStream_t st);
-int launch_seed(const GnSearchDev &sd, const double *x, const double *y, const double *z,
-                size_t n, int32_t *leaf, double *su, double *sv, hipStream_t st);
+                        float *sbox32, double *scentre, double *sseed, int32_t *sref,
+                        hipStream_t st);
+// mode 0: exact nearest leaf centre; mode 1: greedy descent + 3x3 leaf neighbourhood
+int launch_seed(const GnSearchDev &sd, int mode, const double *x, const double *y,
+                const double *z, size_t n, int32_t *leaf, double *su, double *sv,
+                hipStream_t st);
 int launch_invert(const GnSurfDev &d, const GnSearchDev &sd, const GnTreeNode *tree,
                   const uint8_t *valid, int tree_levels, const double *x, const double *y,
                   const double *z, size_t n, const gn_invert_opts &o, double *u, double *v,

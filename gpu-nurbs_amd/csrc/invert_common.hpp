@@ -1,0 +1,251 @@
+// invert_common.hpp -- Gauss-Newton step and the persistent lane-refill driver shared by the
+// inversion kernels (GaussNewton, src/kernel/kernel.cu:151-249).
+#pragma once
+#include "gn_device.hpp"
+
+namespace gn {
+
+constexpr int kInvBlock = 256;
+constexpr int kLdsKnotsInv = 1024;
+constexpr int kChunk = 64;       // points claimed per atomic by one wave
+constexpr int kStepContinue = -1;
+constexpr int kMaxSlices = 64;   // claim counters per launch
+constexpr int kCounterStride = 16;  // 16 x 8 bytes = one 128-byte line per counter
+
+// kMaxSlices zeroed counters (kCounterStride apart) for one launch on stream st
+int work_counters(hipStream_t st, unsigned long long **out);
+int cu_count();
+
+struct InvArgs {
+    const double *x, *y, *z;
+    size_t n;
+    double *u, *v;
+    uint8_t *status;
+    uint8_t *it32, *it64;  // optional per-point iteration counts (FP32 bulk, FP64), or null
+    gn_invert_opts o;
+};
+
+__device__ __forceinline__ uint8_t sat8(int x) { return (uint8_t)(x > 255 ? 255 : x); }
+
+__device__ __forceinline__ void stage_knots_inv(const GnSurfDev &d, double *sk, const double *&ku,
+                                                const double *&kv) {
+    const int nku = d.nu + d.p + 1, nkv = d.nv + d.q + 1;
+    ku = d.ku;
+    kv = d.kv;
+    if (nku + nkv <= kLdsKnotsInv) {
+        for (int i = threadIdx.x; i < nku; i += blockDim.x) sk[i] = d.ku[i];
+        for (int i = threadIdx.x; i < nkv; i += blockDim.x) sk[nku + i] = d.kv[i];
+        __syncthreads();
+        ku = sk;
+        kv = sk + nku;
+    }
+}
+
+// One Gauss-Newton update in arithmetic T on the parameter state (cu, cv) (always FP64).
+// Returns kStepContinue, or the point's final status: GN_PT_CONVERGED_DIST (|S-P| < tol_dist,
+// checked before the update as the reference does), GN_PT_CONVERGED (step below tol_step after
+// the update), GN_PT_SINGULAR (det(J^T J) == 0) or GN_PT_NONFINITE.
+// With NEWTON the residual-curvature term sum_k r_k d2S_k is added to J^T J when the sum stays
+// positive definite (exact Hessian of |S - P|^2 / 2), otherwise the plain J^T J is used.
+template <int K1, int K2, bool EXACT, typename T, bool NEWTON>
+__device__ __forceinline__ int gn_step(const GnSurfDev &d, const double *__restrict__ ku,
+                                       const double *__restrict__ kv, T Px, T Py, T Pz,
+                                       double &cu, double &cv, double tol_dist, double tol_step,
+                                       int clamp, SpanCache &sc) {
+    Cart<T> o;
+    eval_point_cached<K1, K2, NEWTON ? 2 : 1, EXACT, T>(d, ku, kv, cu, cv, sc, o);
+    const T r0 = o.S[0] - Px, r1 = o.S[1] - Py, r2 = o.S[2] - Pz;
+    const T rr = r0 * r0 + r1 * r1 + r2 * r2;
+    if (!(rr == rr)) return GN_PT_NONFINITE;
+    if (tol_dist > 0 && sqrt(rr) < T(tol_dist)) return GN_PT_CONVERGED_DIST;
+    T a = o.Su[0] * o.Su[0] + o.Su[1] * o.Su[1] + o.Su[2] * o.Su[2];
+    T b = o.Su[0] * o.Sv[0] + o.Su[1] * o.Sv[1] + o.Su[2] * o.Sv[2];
+    T c = o.Sv[0] * o.Sv[0] + o.Sv[1] * o.Sv[1] + o.Sv[2] * o.Sv[2];
+    const T gu = o.Su[0] * r0 + o.Su[1] * r1 + o.Su[2] * r2;
+    const T gv = o.Sv[0] * r0 + o.Sv[1] * r1 + o.Sv[2] * r2;
+    if (NEWTON) {
+        const T a2 = a + r0 * o.Suu[0] + r1 * o.Suu[1] + r2 * o.Suu[2];
+        const T b2 = b + r0 * o.Suv[0] + r1 * o.Suv[1] + r2 * o.Suv[2];
+        const T c2 = c + r0 * o.Svv[0] + r1 * o.Svv[1] + r2 * o.Svv[2];
+        if (a2 > T(0) && a2 * c2 - b2 * b2 > T(0)) { a = a2; b = b2; c = c2; }
+    }
+    const T det = a * c - b * b;
+    if (!(det == det)) return GN_PT_NONFINITE;
+    if (det == T(0)) return GN_PT_SINGULAR;
+    const T idet = recip(det);
+    const T du = (c * gu - b * gv) * idet;
+    const T dv = (a * gv - b * gu) * idet;
+    cu -= double(du);
+    cv -= double(dv);
+    if (clamp) {
+        cu = fmin(fmax(cu, ku[d.p]), ku[d.nu]);
+        cv = fmin(fmax(cv, kv[d.q]), kv[d.nv]);
+    }
+    if (!(cu == cu) || !(cv == cv)) return GN_PT_NONFINITE;
+    if (tol_step > 0 && fabs(du) < T(tol_step) && fabs(dv) < T(tol_step)) return GN_PT_CONVERGED;
+    return kStepContinue;
+}
+
+// Up to max_iter steps from (cu, cv); *iters = updates performed.
+template <int K1, int K2, bool EXACT, typename T, bool NEWTON>
+__device__ __forceinline__ int gn_loop(const GnSurfDev &d, const double *__restrict__ ku,
+                                       const double *__restrict__ kv, const double P[3],
+                                       double &cu, double &cv, int max_iter, double tol_dist,
+                                       double tol_step, int clamp, int &iters) {
+    int it = 0, st = GN_PT_MAXITER;
+    SpanCache sc;
+    while (it < max_iter) {
+        const int s = gn_step<K1, K2, EXACT, T, NEWTON>(d, ku, kv, T(P[0]), T(P[1]), T(P[2]), cu,
+                                                        cv, tol_dist, tol_step, clamp, sc);
+        if (s == kStepContinue || s == GN_PT_CONVERGED) it++;
+        if (s != kStepContinue) { st = s; break; }
+    }
+    iters = it;
+    return st;
+}
+
+__device__ __forceinline__ unsigned long long bcast64(unsigned long long x, int src) {
+    const unsigned lo = __shfl((unsigned)(x & 0xffffffffull), src, 64);
+    const unsigned hi = __shfl((unsigned)(x >> 32), src, 64);
+    return ((unsigned long long)hi << 32) | lo;
+}
+
+// Persistent lane-refill Gauss-Newton.  Every wave keeps its 64 lanes busy: a lane whose point
+// has stopped takes the next point of the wave's current chunk of kChunk points.  The batch is
+// cut into `slices` contiguous slices with one claim counter each (separate 128-byte lines);
+// wave w claims chunks from slice w % slices until that slice is used up, then exits.  Atomics on
+// one address serialise across the chip (about 12 ns each), so spreading the claims over up to 64
+// counters keeps them off the critical path.  Each pass of the loop either advances every active
+// point by one step (at most max_iter each) or hands out new points (at most n), so it ends.
+//   BULK = true : FP32 bulk of the mixed mode; writes the bulk state (u, v, status, it32); a
+//                 breakdown (singular / non-finite) writes the seed back so the FP64 pass
+//                 restarts from it.
+//   BULK = false: final (plain) mode; writes u, v, status, it64.
+template <int K1, int K2, bool EXACT, typename T, bool NEWTON, bool BULK>
+__global__ __launch_bounds__(kInvBlock) void k_gn_refill(GnSurfDev d, InvArgs a,
+                                                         unsigned long long *counters, int slices,
+                                                         int max_iter, double tol_dist,
+                                                         double tol_step) {
+    __shared__ double sk[kLdsKnotsInv];
+    const double *ku, *kv;
+    stage_knots_inv(d, sk, ku, kv);
+    const int lane = threadIdx.x & 63;
+    const unsigned long long lt = (1ull << lane) - 1ull;
+    const int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    const int slice = wave % slices;
+    const unsigned long long s_lo = (unsigned long long)(a.n * (size_t)slice / slices);
+    const unsigned long long s_hi = (unsigned long long)(a.n * (size_t)(slice + 1) / slices);
+    unsigned long long *counter = counters + (size_t)slice * kCounterStride;
+    unsigned long long q_lo = 0, q_hi = 0;  // wave-uniform chunk
+    bool exhausted = false;                 // wave-uniform
+    long long idx = -1;
+    double cu = 0, cv = 0, su = 0, sv = 0;
+    T Px = 0, Py = 0, Pz = 0;
+    int it = 0;
+    SpanCache sc;
+    while (true) {
+        while (!exhausted) {
+            const unsigned long long idle = __ballot(idx < 0);
+            if (idle == 0ull) break;
+            if (q_lo >= q_hi) {
+                unsigned long long base = 0;
+                if (lane == 0) base = atomicAdd(counter, (unsigned long long)kChunk);
+                base = s_lo + bcast64(base, 0);
+                if (base >= s_hi) { exhausted = true; break; }
+                q_lo = base;
+                q_hi = base + kChunk < s_hi ? base + kChunk : s_hi;
+            }
+            const unsigned long long avail = q_hi - q_lo;
+            if (idx < 0) {
+                const unsigned long long rank = (unsigned long long)__popcll(idle & lt);
+                if (rank < avail) {
+                    idx = (long long)(q_lo + rank);
+                    Px = T(a.x[idx]);
+                    Py = T(a.y[idx]);
+                    Pz = T(a.z[idx]);
+                    su = cu = a.u[idx];
+                    sv = cv = a.v[idx];
+                    it = 0;
+                    sc = SpanCache();
+                }
+            }
+            const unsigned long long nidle = (unsigned long long)__popcll(idle);
+            q_lo += nidle < avail ? nidle : avail;
+        }
+        if (__ballot(idx >= 0) == 0ull) break;
+        if (idx >= 0) {
+            int st;
+            if (it >= max_iter) {
+                st = GN_PT_MAXITER;
+            } else {
+                st = gn_step<K1, K2, EXACT, T, NEWTON>(d, ku, kv, Px, Py, Pz, cu, cv, tol_dist,
+                                                       tol_step, a.o.clamp, sc);
+                if (st == kStepContinue || st == GN_PT_CONVERGED) it++;
+                if (st == kStepContinue && it >= max_iter) st = GN_PT_MAXITER;
+            }
+            if (st != kStepContinue) {
+                if (BULK && (st == GN_PT_SINGULAR || st == GN_PT_NONFINITE)) {
+                    cu = su;
+                    cv = sv;
+                }
+                a.u[idx] = cu;
+                a.v[idx] = cv;
+                a.status[idx] = (uint8_t)st;
+                if (BULK) {
+                    if (a.it32) a.it32[idx] = sat8(it);
+                } else {
+                    if (a.it32) a.it32[idx] = 0;
+                    if (a.it64) a.it64[idx] = sat8(it);
+                }
+                idx = -1;
+            }
+        }
+    }
+}
+
+// Closing FP64 iterations of the mixed mode, one lane per point: up to fp64_tail iterations from
+// the bulk result, or up to max_iter from the seed when the FP32 bulk broke down.
+template <int K1, int K2, bool EXACT, bool NEWTON>
+__global__ __launch_bounds__(kInvBlock) void k_gn_tail(GnSurfDev d, InvArgs a) {
+    __shared__ double sk[kLdsKnotsInv];
+    const double *ku, *kv;
+    stage_knots_inv(d, sk, ku, kv);
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < a.n;
+         i += (size_t)gridDim.x * blockDim.x) {
+        const double P[3] = {a.x[i], a.y[i], a.z[i]};
+        double cu = a.u[i], cv = a.v[i];
+        const int sb = a.status[i];
+        const bool restart = sb == GN_PT_SINGULAR || sb == GN_PT_NONFINITE;
+        int it;
+        const int st = gn_loop<K1, K2, EXACT, double, NEWTON>(
+            d, ku, kv, P, cu, cv, restart ? a.o.max_iter : a.o.fp64_tail, a.o.tol_dist,
+            a.o.tol_step, a.o.clamp, it);
+        a.u[i] = cu;
+        a.v[i] = cv;
+        a.status[i] = (uint8_t)st;
+        if (a.it64) a.it64[i] = sat8(it);
+    }
+}
+
+// Resident workgroups per CU of a kernel (runtime occupancy calculator, cached per kernel).
+int blocks_per_cu(const void *kernel, int block);
+
+// Persistent grid: as many 256-lane workgroups as can be resident at once, never more than the
+// batch needs; and the number of claim slices for that grid (16 or more waves per slice).
+inline void refill_shape(const void *kernel, size_t n, unsigned &grid, int &slices) {
+    const size_t want = (n + kInvBlock - 1) / kInvBlock;
+    const size_t cap = (size_t)cu_count() * blocks_per_cu(kernel, kInvBlock);
+    grid = (unsigned)(want < cap ? (want ? want : 1) : cap);
+    const int waves = (int)grid * (kInvBlock / 64);
+    slices = waves / 16;
+    if (slices < 1) slices = 1;
+    if (slices > kMaxSlices) slices = kMaxSlices;
+}
+
+// launchers, one translation unit each (invert_fp64.hip, invert_mixed.hip, invert_ref.hip)
+int launch_gn_plain(const GnSurfDev &d, const InvArgs &a, hipStream_t st);
+int launch_gn_mixed(const GnSurfDev &d, const InvArgs &a, hipStream_t st);
+int launch_gn_first_hit(const GnSurfDev &d, const InvArgs &a, const GnTreeNode *tree,
+                        const uint8_t *valid, long count, hipStream_t st);
+
+}  // namespace gn

@@ -1,0 +1,79 @@
+// runtime.hip -- small per-device runtime services of libgnurbs:
+//   - claim counters for the persistent (lane-refill) kernels: a ring of slots per device, each
+//     slot kMaxSlices 64-bit counters on their own 128-byte lines; every launch takes the next
+//     slot and zeroes it with a small fill kernel on the caller's stream (a kernel, not a memset,
+//     so the sequence is also valid inside a captured hipGraph).  Two launches share a slot only
+//     if more than kCounterSlots launches are in flight at once.
+//   - the compute-unit count used to size persistent grids.
+#include <atomic>
+#include <mutex>
+#include <unordered_map>
+
+#include "invert_common.hpp"
+
+namespace gn {
+
+constexpr int kCounterSlots = 256;
+constexpr int kMaxDevices = 64;
+constexpr size_t kSlotWords = (size_t)kMaxSlices * kCounterStride;
+
+__global__ void k_zero_slot(unsigned long long *p) {
+    if (threadIdx.x < kMaxSlices) p[threadIdx.x * kCounterStride] = 0ull;
+}
+
+namespace {
+std::mutex g_mu;
+unsigned long long *g_counters[kMaxDevices] = {};
+int g_cus[kMaxDevices] = {};
+std::atomic<unsigned> g_next{0};
+}  // namespace
+
+int work_counters(hipStream_t st, unsigned long long **out) {
+    int dev = 0;
+    GN_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(GN_EDEVICE, "device index out of range");
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        if (!g_counters[dev]) {
+            void *p = nullptr;
+            GN_HIP(hipMalloc(&p, sizeof(unsigned long long) * kCounterSlots * kSlotWords));
+            GN_HIP(hipMemset(p, 0, sizeof(unsigned long long) * kCounterSlots * kSlotWords));
+            GN_HIP(hipDeviceSynchronize());
+            g_counters[dev] = (unsigned long long *)p;
+        }
+    }
+    const unsigned slot = g_next.fetch_add(1, std::memory_order_relaxed) % kCounterSlots;
+    unsigned long long *c = g_counters[dev] + (size_t)slot * kSlotWords;
+    k_zero_slot<<<1, 64, 0, st>>>(c);
+    GN_HIP(hipGetLastError());
+    *out = c;
+    return GN_OK;
+}
+
+int blocks_per_cu(const void *kernel, int block) {
+    static std::mutex mu;
+    static std::unordered_map<const void *, int> cache;
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = cache.find(kernel);
+    if (it != cache.end()) return it->second;
+    int b = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, kernel, block, 0) != hipSuccess || b <= 0)
+        b = 1;
+    cache[kernel] = b;
+    return b;
+}
+
+int cu_count() {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return 256;
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!g_cus[dev]) {
+        int n = 0;
+        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0)
+            n = 256;
+        g_cus[dev] = n;
+    }
+    return g_cus[dev];
+}
+
+}  // namespace gn

@@ -27,7 +27,7 @@ LIB_PATH = os.path.join(_HERE, "libgnurbs.so")
 GN_OK, GN_EINVAL, GN_ENOMEM, GN_EDEVICE, GN_EIO, GN_ESTATE, GN_EBUDGET, GN_ENOTFOUND, GN_EAGAIN = range(9)
 GN_FP64, GN_FP32, GN_MIXED = 0, 1, 2
 PT_CONVERGED, PT_CONVERGED_DIST, PT_MAXITER, PT_SINGULAR, PT_NONFINITE, PT_NOSEED = range(6)
-SEED_NEAREST_LEAF, SEED_REF_FIRST_HIT, SEED_GIVEN = 0, 1, 2
+SEED_NEAREST_LEAF, SEED_REF_FIRST_HIT, SEED_GIVEN, SEED_GREEDY_LEAF = 0, 1, 2, 3
 TREE_REF_PARENT = 1
 NEWTON_FP32, NEWTON_FP64 = 1, 2
 
@@ -77,7 +77,7 @@ _SIGS = [
     ("gn_evaluate_grid_f32", _c_int, [_c_void, _c_void, _c_int, _c_void, _c_int, _c_int, _c_void, _c_void]),
     ("gn_build_tree", _c_int, [_c_void, _c_int, _c_int, _c_void]),
     ("gn_get_tree", _c_int, [_c_void, _c_void, _c_void, _c_void, _c_size]),
-    ("gn_seed", _c_int, [_c_void, _c_void, _c_void, _c_void, _c_size, _c_void, _c_void, _c_void, _c_void]),
+    ("gn_seed", _c_int, [_c_void, _c_int, _c_void, _c_void, _c_void, _c_size, _c_void, _c_void, _c_void, _c_void]),
     ("gn_invert", _c_int, [_c_void, _c_void, _c_void, _c_void, _c_size, ctypes.POINTER(InvertOpts), _c_void, _c_void, _c_void, _c_void]),
     ("gn_invert_iters", _c_int, [_c_void, _c_void, _c_void, _c_void, _c_size, ctypes.POINTER(InvertOpts), _c_void, _c_void, _c_void, _c_void, _c_void, _c_void]),
 ]
@@ -269,14 +269,15 @@ class Surface:
         _check(lib().gn_get_tree(self._h, nodes.ctypes.data, valid.ctypes.data, divided.ctypes.data, cnt))
         return nodes, valid, divided
 
-    def Seed(self, x, y, z, stream=None):
+    def Seed(self, x, y, z, exact=True, stream=None):
         torch = _torch()
         xd, yd, zd = (to_device(a, torch.float64) for a in (x, y, z))
         n = xd.numel()
         leaf = torch.empty(n, dtype=torch.int32, device="cuda")
         su = torch.empty(n, dtype=torch.float64, device="cuda")
         sv = torch.empty(n, dtype=torch.float64, device="cuda")
-        _check(lib().gn_seed(self._h, xd.data_ptr(), yd.data_ptr(), zd.data_ptr(), n, leaf.data_ptr(), su.data_ptr(),
+        _check(lib().gn_seed(self._h, 1 if exact else 0, xd.data_ptr(), yd.data_ptr(), zd.data_ptr(), n,
+                             leaf.data_ptr(), su.data_ptr(),
                              sv.data_ptr(), _c_void(stream_handle(stream))))
         return leaf, su, sv
 

@@ -59,7 +59,9 @@ enum {
 typedef enum {
     GN_SEED_NEAREST_LEAF = 0, /* leaf of the tree whose centre point is nearest to P       */
     GN_SEED_REF_FIRST_HIT = 1,/* reference NewInverse: index-order scan, first box hit     */
-    GN_SEED_GIVEN = 2         /* caller supplies seeds in u_out / v_out                    */
+    GN_SEED_GIVEN = 2,        /* caller supplies seeds in u_out / v_out                    */
+    GN_SEED_GREEDY_LEAF = 3   /* greedy box-distance descent, then the nearest leaf centre */
+                              /* among the 3 x 3 leaves around it (fixed work per point)   */
 } gn_seed_mode;
 
 /* tree build flags */
@@ -85,7 +87,7 @@ enum { GN_NEWTON_FP32 = 1, /* FP32 bulk iterations of GN_MIXED                  
 };
 
 /* Defaults: the reference iteration in plain FP64 (tol_dist 0, tol_step 1e-10, max_iter 20,
- * clamp 1, nearest-leaf seeds, newton 0); mixed: <= 20 FP32 iterations to a 1e-6 step, then
+ * clamp 1, greedy-leaf seeds, newton 0); mixed: <= 20 FP32 iterations to a 1e-6 step, then
  * <= 3 FP64 closing iterations. */
 void gn_invert_opts_default(gn_invert_opts *opts);
 
@@ -142,9 +144,10 @@ int gn_build_tree(gn_surface *s, int levels, int flags, gn_stream stream);
  * valid[count], divided[count]; count = (64^levels - 1)/63.  Synchronous. */
 int gn_get_tree(const gn_surface *s, double *nodes, uint8_t *valid, uint8_t *divided,
                 size_t count);
-/* Seed search alone: leaf index (reference numbering) and seed (u,v) per point. */
-int gn_seed(const gn_surface *s, const double *x, const double *y, const double *z, size_t n,
-            int32_t *leaf, double *seed_u, double *seed_v, gn_stream stream);
+/* Seed search alone: leaf index (reference numbering) and seed (u,v) per point; exact != 0
+ * selects GN_SEED_NEAREST_LEAF, exact == 0 GN_SEED_GREEDY_LEAF. */
+int gn_seed(const gn_surface *s, int exact, const double *x, const double *y, const double *z,
+            size_t n, int32_t *leaf, double *seed_u, double *seed_v, gn_stream stream);
 
 /* ---- inversion (NewInverse + GaussNewton, src/kernel/kernel.cu:151-249, 433-469) ----------- */
 

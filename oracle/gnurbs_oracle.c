@@ -28,6 +28,7 @@
  *   or_build_tree       containment of dense samples, index formula, NeedDivide truth table
  *   or_seed_*           brute force
  */
+#define _GNU_SOURCE
 #include <math.h>
 #include <stdint.h>
 #include <stdlib.h>
@@ -630,4 +631,102 @@ void or_seed_nearest_leaf(const double *nodes, int levels, const double *x, cons
         seed_u[i] = (nd[0] + nd[2]) / 2;
         seed_v[i] = (nd[1] + nd[3]) / 2;
     }
+}
+
+/* Greedy-leaf seed (the library's GN_SEED_GREEDY_LEAF, not in the reference): quadtree over the
+ * finest level with boxes = leaf box U leaf centre, unions upward; FP32 copies rounded outward and
+ * widened by (1 + max|root box|) * 2^-20; greedy descent by FP32 squared box distance (ties to
+ * the lower child, children ordered (x,y) = (0,0),(0,1),(1,0),(1,1)); then the nearest leaf
+ * centre (FP64, ties to the lower reference index) among the 3 x 3 leaves around it. */
+void or_seed_greedy_leaf(const double *nodes, int levels, const double *x, const double *y,
+                         const double *z, long n, int *leaf_out, double *seed_u, double *seed_v)
+{
+    int D = 3 * (levels - 1);
+    long side = 1L << D, base = or_tree_base(levels - 1);
+    /* boxes per quad level, row-major (ix*side_l + iy) */
+    double **bx = malloc(sizeof(double *) * (D + 1));
+    for (int l = 0; l <= D; l++) bx[l] = malloc(sizeof(double) * 6 * (1L << (2 * l)));
+    for (long ix = 0; ix < side; ix++)
+        for (long iy = 0; iy < side; iy++) {
+            const double *nd = nodes + (base + ix * side + iy) * 13;
+            double *b = bx[D] + (ix * side + iy) * 6;
+            for (int k = 0; k < 3; k++) {
+                b[k] = fmin(nd[4 + k], nd[10 + k]);
+                b[3 + k] = fmax(nd[7 + k], nd[10 + k]);
+            }
+        }
+    for (int l = D - 1; l >= 0; l--) {
+        long s = 1L << l;
+        for (long ix = 0; ix < s; ix++)
+            for (long iy = 0; iy < s; iy++) {
+                double *b = bx[l] + (ix * s + iy) * 6;
+                for (int k = 0; k < 3; k++) { b[k] = INFINITY; b[3 + k] = -INFINITY; }
+                for (int a = 0; a < 4; a++) {
+                    long cx = 2 * ix + (a >> 1), cy = 2 * iy + (a & 1);
+                    const double *cb = bx[l + 1] + (cx * (2 * s) + cy) * 6;
+                    for (int k = 0; k < 3; k++) {
+                        b[k] = fmin(b[k], cb[k]);
+                        b[3 + k] = fmax(b[3 + k], cb[3 + k]);
+                    }
+                }
+            }
+    }
+    double mag = 0;
+    for (int k = 0; k < 6; k++) mag = fmax(mag, fabs(bx[0][k]));
+    double margin = (1.0 + mag) * ldexp(1.0, -20);
+    float **b32 = malloc(sizeof(float *) * (D + 1));
+    for (int l = 0; l <= D; l++) {
+        long cnt = 1L << (2 * l);
+        b32[l] = malloc(sizeof(float) * 6 * cnt);
+        for (long i = 0; i < cnt; i++)
+            for (int k = 0; k < 3; k++) {
+                double lo = bx[l][i * 6 + k] - margin, hi = bx[l][i * 6 + 3 + k] + margin;
+                float lo32 = (float)lo, hi32 = (float)hi;
+                if ((double)lo32 > lo) lo32 = nextafterf(lo32, -INFINITY);
+                if ((double)hi32 < hi) hi32 = nextafterf(hi32, INFINITY);
+                b32[l][i * 6 + k] = lo32;
+                b32[l][i * 6 + 3 + k] = hi32;
+            }
+    }
+    for (long i = 0; i < n; i++) {
+        double P[3] = {x[i], y[i], z[i]};
+        float p[3] = {(float)P[0], (float)P[1], (float)P[2]};
+        long ix = 0, iy = 0;
+        for (int l = 0; l < D; l++) {
+            float bd = INFINITY;
+            long bxx = 2 * ix, byy = 2 * iy;
+            long s = 1L << (l + 1);
+            for (int a = 0; a < 4; a++) {
+                long cx = 2 * ix + (a >> 1), cy = 2 * iy + (a & 1);
+                const float *b = b32[l + 1] + (cx * s + cy) * 6;
+                float sum = 0.f;
+                for (int k = 0; k < 3; k++) {
+                    float e = fmaxf(fmaxf(b[k] - p[k], p[k] - b[3 + k]), 0.f);
+                    sum = fmaf(e, e, sum);
+                }
+                if (sum < bd) { bd = sum; bxx = cx; byy = cy; }
+            }
+            ix = bxx;
+            iy = byy;
+        }
+        double best = INFINITY;
+        long bi = base + ix * side + iy;
+        for (int dx = -1; dx <= 1; dx++)
+            for (int dy = -1; dy <= 1; dy++) {
+                long xx = ix + dx, yy = iy + dy;
+                if (xx < 0 || yy < 0 || xx >= side || yy >= side) continue;
+                long t = base + xx * side + yy;
+                const double *q = nodes + t * 13 + 10;
+                double ex = P[0] - q[0], ey = P[1] - q[1], ez = P[2] - q[2];
+                double dd = fma(ez, ez, fma(ey, ey, ex * ex));
+                if (dd < best || (dd == best && t < bi)) { best = dd; bi = t; }
+            }
+        leaf_out[i] = (int)bi;
+        const double *nd = nodes + bi * 13;
+        seed_u[i] = (nd[0] + nd[2]) / 2;
+        seed_v[i] = (nd[1] + nd[3]) / 2;
+    }
+    for (int l = 0; l <= D; l++) { free(bx[l]); free(b32[l]); }
+    free(bx);
+    free(b32);
 }

@@ -50,6 +50,7 @@ def lib():
         L.or_invert_first_hit.argtypes = [_I, _I, _I, _I, _P, _P, _P, _P, _P, _L, _P, _P, _P, _L,
                                           _D, _D, _I, _I, _P, _P, _P, _P, _I]
         L.or_seed_nearest_leaf.argtypes = [_P, _I, _P, _P, _P, _L, _P, _P, _P]
+        L.or_seed_greedy_leaf.argtypes = [_P, _I, _P, _P, _P, _L, _P, _P, _P]
         _l = L
     return _l
 
@@ -169,4 +170,15 @@ def seed_nearest_leaf(nodes, levels, x, y, z):
     su, sv = np.zeros(n), np.zeros(n)
     lib().or_seed_nearest_leaf(nodes.ctypes.data, levels, x.ctypes.data, y.ctypes.data, z.ctypes.data, n,
                                leaf.ctypes.data, su.ctypes.data, sv.ctypes.data)
+    return leaf, su, sv
+
+
+def seed_greedy_leaf(nodes, levels, x, y, z):
+    nodes = _a(nodes)
+    x, y, z = (_a(a) for a in (x, y, z))
+    n = x.size
+    leaf = np.zeros(n, dtype=np.int32)
+    su, sv = np.zeros(n), np.zeros(n)
+    lib().or_seed_greedy_leaf(nodes.ctypes.data, levels, x.ctypes.data, y.ctypes.data, z.ctypes.data, n,
+                              leaf.ctypes.data, su.ctypes.data, sv.ctypes.data)
     return leaf, su, sv

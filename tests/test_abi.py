@@ -43,7 +43,7 @@ def test_library_exports_every_declared_symbol():
 def test_version_and_defaults_without_gpu():
     assert b"gfx950" in G.lib().gn_version()
     o = G.default_opts()
-    assert o.max_iter == 20 and o.precision == G.GN_FP64 and o.seed_mode == G.SEED_NEAREST_LEAF
+    assert o.max_iter == 20 and o.precision == G.GN_FP64 and o.seed_mode == G.SEED_GREEDY_LEAF
     assert o.fp64_tail == 3 and o.newton == 0 and abs(o.tol_step - 1e-10) < 1e-25
 
 

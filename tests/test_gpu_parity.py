@@ -173,7 +173,7 @@ def test_seed_nearest_leaf_matches_brute_force():
     rng = np.random.default_rng(4)
     n = 3000
     P = np.concatenate([rng.uniform(-0.3, 1.3, (3, n // 2)), os_.eval(*synth.uniform_uv(n - n // 2, 8), 0)], axis=1)
-    leaf, su, sv = (_np(t) for t in gs.Seed(P[0], P[1], P[2]))
+    leaf, su, sv = (_np(t) for t in gs.Seed(P[0], P[1], P[2], exact=True))
     ref_leaf, ref_su, ref_sv = O.seed_nearest_leaf(nodes, 3, P[0], P[1], P[2])
     C = nodes[:, 10:13]
     d_got = np.sum((C[leaf] - P.T) ** 2, axis=1)
@@ -181,6 +181,22 @@ def test_seed_nearest_leaf_matches_brute_force():
     same = leaf == ref_leaf
     # a different leaf is only allowed on a tie up to rounding
     assert np.all(same | (np.abs(d_got - d_ref) <= 1e-12 * np.maximum(d_ref, 1e-30)))
+    assert same.mean() > 0.999
+    assert np.array_equal(su[same], ref_su[same]) and np.array_equal(sv[same], ref_sv[same])
+
+
+def test_seed_greedy_leaf_matches_oracle():
+    sd = synth.surface_G(32, 32, 3, 3, seed=1002)
+    gs, os_ = _pair(sd)
+    gs.BuildTree(3)
+    nodes, _, _ = gs.GetTree()
+    u0, v0 = synth.uniform_uv(20000, 31)
+    b = blocks(os_.eval(u0, v0, 1))
+    P, _ = synth.offset_queries(b["S"], b["Su"], b["Sv"], 1e-2, 32)
+    leaf, su, sv = (_np(t) for t in gs.Seed(P[0], P[1], P[2], exact=False))
+    ref_leaf, ref_su, ref_sv = O.seed_greedy_leaf(nodes, 3, P[0], P[1], P[2])
+    same = leaf == ref_leaf
+    # identical except where FP32 box distances of the descent tie within rounding
     assert same.mean() > 0.999
     assert np.array_equal(su[same], ref_su[same]) and np.array_equal(sv[same], ref_sv[same])
 
@@ -217,9 +233,9 @@ def test_invert_mixed_reaches_1e9():
     u, v, st = (_np(t) for t in gs.NewInverse(P[0], P[1], P[2], opts))
     err = np.maximum(np.abs(u - u0), np.abs(v - v0))
     assert np.mean(err <= 1e-9) > 0.9
-    # against the FP64 oracle (exact Hessian) from the same nearest-leaf seeds, where both converge
+    # against the FP64 oracle (exact Hessian) from the same greedy-leaf seeds, where both converge
     nodes, _, _ = gs.GetTree()
-    _, ssu, ssv = O.seed_nearest_leaf(nodes, 3, P[0], P[1], P[2])
+    _, ssu, ssv = O.seed_greedy_leaf(nodes, 3, P[0], P[1], P[2])
     ru, rv, rst, _ = os_.gauss_newton(P[0], P[1], P[2], ssu, ssv, tol_step=1e-12, max_iter=40, newton=1)
     both = (st == G.PT_CONVERGED) & (rst == O.CONV_STEP)
     agree = np.maximum(np.abs(u - ru), np.abs(v - rv)) <= 1e-9
@@ -287,3 +303,18 @@ def test_errors_are_reported_not_fatal():
         G.Surface.readNu("/nonexistent.nu")
     # empty batches are no-ops
     assert gs.Evaluate(np.zeros(0), np.zeros(0), 2).shape == (18, 0)
+
+
+@pytest.mark.parametrize("precision,newton", [(G.GN_FP64, 0), (G.GN_MIXED, G.NEWTON_FP32 | G.NEWTON_FP64)])
+def test_result_independent_of_batch_and_order(precision, newton):
+    # a large batch is processed in spatially sorted order with lane refill; a small one in index
+    # order: every point's result must be bit-identical either way
+    sd, gs, os_, u0, v0, P = _projection_case(30000, 1e-2)
+    gs.BuildTree(3)
+    opts = G.default_opts(precision=precision, newton=newton)
+    big = [_np(t) for t in gs.NewInverse(P[0], P[1], P[2], opts)]
+    sel = np.random.default_rng(12).choice(30000, 3000, replace=False)
+    small = [_np(t) for t in gs.NewInverse(P[0][sel], P[1][sel], P[2][sel], G.default_opts(precision=precision,
+                                                                                           newton=newton))]
+    for a, b in zip(big, small):
+        assert np.array_equal(a[sel], b)

@@ -350,3 +350,26 @@ def test_first_hit_on_reference_model(golden_dir):
     assert st[0] == O.CONV_DIST
     S = os_.eval(u, v, 0)[:, 0]
     assert np.linalg.norm(S - [0.0, 0.200049, 0.616]) < 0.01
+
+
+def test_seed_greedy_leaf_properties():
+    # greedy descent + 3x3 refinement: a query exactly at a leaf centre gets that leaf, the result
+    # is never farther than the plain descent leaf, and it agrees with the exact nearest leaf for
+    # the large majority of near-surface queries
+    sd, os_ = _surf(32, 32, 3, 3, seed=1002)
+    nodes, valid, divided = os_.build_tree(3)
+    base = O.tree_base(2)
+    C = nodes[base:base + 4096, 10:13].T
+    leaf, su, sv = O.seed_greedy_leaf(nodes, 3, C[0], C[1], C[2])
+    assert np.mean(leaf == np.arange(base, base + 4096)) > 0.999
+    u0, v0 = synth.uniform_uv(3000, 21)
+    b = blocks(os_.eval(u0, v0, 1))
+    P, _ = synth.offset_queries(b["S"], b["Su"], b["Sv"], 1e-2, 22)
+    g_leaf, g_su, g_sv = O.seed_greedy_leaf(nodes, 3, P[0], P[1], P[2])
+    e_leaf, _, _ = O.seed_nearest_leaf(nodes, 3, P[0], P[1], P[2])
+    assert np.mean(g_leaf == e_leaf) > 0.95
+    dg = np.sum((nodes[g_leaf, 10:13] - P.T) ** 2, axis=1)
+    de = np.sum((nodes[e_leaf, 10:13] - P.T) ** 2, axis=1)
+    assert np.all(dg >= de)  # the exact search is never beaten
+    nd = nodes[g_leaf]
+    assert np.array_equal(g_su, (nd[:, 0] + nd[:, 2]) / 2) and np.array_equal(g_sv, (nd[:, 1] + nd[:, 3]) / 2)
