@@ -51,7 +51,7 @@ template <int K1, int K2, bool EXACT, typename T, bool NEWTON>
 __device__ __forceinline__ int gn_step(const GnSurfDev &d, const double *__restrict__ ku,
                                        const double *__restrict__ kv, T Px, T Py, T Pz,
                                        double &cu, double &cv, double tol_dist, double tol_step,
-                                       int clamp, SpanCache &sc) {
+                                       double tol_quad, int clamp, SpanCache &sc) {
     Cart<T> o;
     eval_point_cached<K1, K2, NEWTON ? 2 : 1, EXACT, T>(d, ku, kv, cu, cv, sc, o);
     const T r0 = o.S[0] - Px, r1 = o.S[1] - Py, r2 = o.S[2] - Pz;
@@ -83,6 +83,8 @@ __device__ __forceinline__ int gn_step(const GnSurfDev &d, const double *__restr
     }
     if (!(cu == cu) || !(cv == cv)) return GN_PT_NONFINITE;
     if (tol_step > 0 && fabs(du) < T(tol_step) && fabs(dv) < T(tol_step)) return GN_PT_CONVERGED;
+    if (NEWTON && tol_quad > 0 && fabs(du) < T(tol_quad) && fabs(dv) < T(tol_quad))
+        return GN_PT_CONVERGED;
     return kStepContinue;
 }
 
@@ -91,12 +93,13 @@ template <int K1, int K2, bool EXACT, typename T, bool NEWTON>
 __device__ __forceinline__ int gn_loop(const GnSurfDev &d, const double *__restrict__ ku,
                                        const double *__restrict__ kv, const double P[3],
                                        double &cu, double &cv, int max_iter, double tol_dist,
-                                       double tol_step, int clamp, int &iters) {
+                                       double tol_step, int clamp, int &iters,
+                                       double tol_quad = 0.0) {
     int it = 0, st = GN_PT_MAXITER;
     SpanCache sc;
     while (it < max_iter) {
         const int s = gn_step<K1, K2, EXACT, T, NEWTON>(d, ku, kv, T(P[0]), T(P[1]), T(P[2]), cu,
-                                                        cv, tol_dist, tol_step, clamp, sc);
+                                                        cv, tol_dist, tol_step, tol_quad, clamp, sc);
         if (s == kStepContinue || s == GN_PT_CONVERGED) it++;
         if (s != kStepContinue) { st = s; break; }
     }
@@ -117,15 +120,20 @@ __device__ __forceinline__ unsigned long long bcast64(unsigned long long x, int 
 // one address serialise across the chip (about 12 ns each), so spreading the claims over up to 64
 // counters keeps them off the critical path.  Each pass of the loop either advances every active
 // point by one step (at most max_iter each) or hands out new points (at most n), so it ends.
-//   BULK = true : FP32 bulk of the mixed mode; writes the bulk state (u, v, status, it32); a
-//                 breakdown (singular / non-finite) writes the seed back so the FP64 pass
+//   MODE kPlain : one-pass FP64 (or FP32) iteration; writes u, v, status, it64.
+//   MODE kBulk  : FP32 bulk of the mixed mode; writes the bulk state (u, v, status, it32); a
+//                 breakdown (singular / non-finite) writes the seed back so the closing pass
 //                 restarts from it.
-//   BULK = false: final (plain) mode; writes u, v, status, it64.
-template <int K1, int K2, bool EXACT, typename T, bool NEWTON, bool BULK>
+//   MODE kClose : FP64 closing pass of the mixed mode; a point gets max_iter steps from the bulk
+//                 result, or restart_iter steps when its bulk status is a breakdown.
+enum { kPlain = 0, kBulk = 1, kClose = 2 };
+
+template <int K1, int K2, bool EXACT, typename T, bool NEWTON, int MODE>
 __global__ __launch_bounds__(kInvBlock) void k_gn_refill(GnSurfDev d, InvArgs a,
                                                          unsigned long long *counters, int slices,
-                                                         int max_iter, double tol_dist,
-                                                         double tol_step) {
+                                                         int max_iter, int restart_iter,
+                                                         double tol_dist, double tol_step,
+                                                         double tol_quad) {
     __shared__ double sk[kLdsKnotsInv];
     const double *ku, *kv;
     stage_knots_inv(d, sk, ku, kv);
@@ -141,7 +149,7 @@ __global__ __launch_bounds__(kInvBlock) void k_gn_refill(GnSurfDev d, InvArgs a,
     long long idx = -1;
     double cu = 0, cv = 0, su = 0, sv = 0;
     T Px = 0, Py = 0, Pz = 0;
-    int it = 0;
+    int it = 0, lim = max_iter;
     SpanCache sc;
     while (true) {
         while (!exhausted) {
@@ -166,6 +174,11 @@ __global__ __launch_bounds__(kInvBlock) void k_gn_refill(GnSurfDev d, InvArgs a,
                     su = cu = a.u[idx];
                     sv = cv = a.v[idx];
                     it = 0;
+                    lim = max_iter;
+                    if (MODE == kClose) {
+                        const int sb = a.status[idx];
+                        if (sb == GN_PT_SINGULAR || sb == GN_PT_NONFINITE) lim = restart_iter;
+                    }
                     sc = SpanCache();
                 }
             }
@@ -175,55 +188,31 @@ __global__ __launch_bounds__(kInvBlock) void k_gn_refill(GnSurfDev d, InvArgs a,
         if (__ballot(idx >= 0) == 0ull) break;
         if (idx >= 0) {
             int st;
-            if (it >= max_iter) {
+            if (it >= lim) {
                 st = GN_PT_MAXITER;
             } else {
                 st = gn_step<K1, K2, EXACT, T, NEWTON>(d, ku, kv, Px, Py, Pz, cu, cv, tol_dist,
-                                                       tol_step, a.o.clamp, sc);
+                                                       tol_step, tol_quad, a.o.clamp, sc);
                 if (st == kStepContinue || st == GN_PT_CONVERGED) it++;
-                if (st == kStepContinue && it >= max_iter) st = GN_PT_MAXITER;
+                if (st == kStepContinue && it >= lim) st = GN_PT_MAXITER;
             }
             if (st != kStepContinue) {
-                if (BULK && (st == GN_PT_SINGULAR || st == GN_PT_NONFINITE)) {
+                if (MODE == kBulk && (st == GN_PT_SINGULAR || st == GN_PT_NONFINITE)) {
                     cu = su;
                     cv = sv;
                 }
                 a.u[idx] = cu;
                 a.v[idx] = cv;
                 a.status[idx] = (uint8_t)st;
-                if (BULK) {
+                if (MODE == kBulk) {
                     if (a.it32) a.it32[idx] = sat8(it);
                 } else {
-                    if (a.it32) a.it32[idx] = 0;
+                    if (MODE == kPlain && a.it32) a.it32[idx] = 0;
                     if (a.it64) a.it64[idx] = sat8(it);
                 }
                 idx = -1;
             }
         }
-    }
-}
-
-// Closing FP64 iterations of the mixed mode, one lane per point: up to fp64_tail iterations from
-// the bulk result, or up to max_iter from the seed when the FP32 bulk broke down.
-template <int K1, int K2, bool EXACT, bool NEWTON>
-__global__ __launch_bounds__(kInvBlock) void k_gn_tail(GnSurfDev d, InvArgs a) {
-    __shared__ double sk[kLdsKnotsInv];
-    const double *ku, *kv;
-    stage_knots_inv(d, sk, ku, kv);
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < a.n;
-         i += (size_t)gridDim.x * blockDim.x) {
-        const double P[3] = {a.x[i], a.y[i], a.z[i]};
-        double cu = a.u[i], cv = a.v[i];
-        const int sb = a.status[i];
-        const bool restart = sb == GN_PT_SINGULAR || sb == GN_PT_NONFINITE;
-        int it;
-        const int st = gn_loop<K1, K2, EXACT, double, NEWTON>(
-            d, ku, kv, P, cu, cv, restart ? a.o.max_iter : a.o.fp64_tail, a.o.tol_dist,
-            a.o.tol_step, a.o.clamp, it);
-        a.u[i] = cu;
-        a.v[i] = cv;
-        a.status[i] = (uint8_t)st;
-        if (a.it64) a.it64[i] = sat8(it);
     }
 }
 

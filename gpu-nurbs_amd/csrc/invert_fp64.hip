@@ -16,15 +16,15 @@ struct PlainLaunch {
         unsigned grid;
         int slices;
         if (a.o.newton & GN_NEWTON_FP64) {
-            auto k = k_gn_refill<K1, K2, EXACT, double, true, false>;
+            auto k = k_gn_refill<K1, K2, EXACT, double, true, kPlain>;
             refill_shape((const void *)k, a.n, grid, slices);
-            k<<<grid, kInvBlock, 0, st>>>(d, a, ctr, slices, a.o.max_iter, a.o.tol_dist,
-                                          a.o.tol_step);
+            k<<<grid, kInvBlock, 0, st>>>(d, a, ctr, slices, a.o.max_iter, 0, a.o.tol_dist,
+                                          a.o.tol_step, a.o.tol_quad);
         } else {
-            auto k = k_gn_refill<K1, K2, EXACT, double, false, false>;
+            auto k = k_gn_refill<K1, K2, EXACT, double, false, kPlain>;
             refill_shape((const void *)k, a.n, grid, slices);
-            k<<<grid, kInvBlock, 0, st>>>(d, a, ctr, slices, a.o.max_iter, a.o.tol_dist,
-                                          a.o.tol_step);
+            k<<<grid, kInvBlock, 0, st>>>(d, a, ctr, slices, a.o.max_iter, 0, a.o.tol_dist,
+                                          a.o.tol_step, 0.0);
         }
         GN_HIP(hipGetLastError());
         return GN_OK;

@@ -1,6 +1,6 @@
 // invert_mixed.hip -- mixed-precision inversion: FP32 bulk iterations against the FP32 span
-// table (persistent lane-refill kernel), then FP64 closing iterations whose residual, rational
-// weight division and 2x2 solve are FP64 (one lane per point).
+// table, then FP64 closing iterations whose residual, rational weight division and 2x2 solve are
+// FP64; both passes are persistent lane-refill kernels.
 #include "invert_common.hpp"
 
 namespace gn {
@@ -17,19 +17,28 @@ struct MixedLaunch {
         unsigned grid;
         int slices;
         if (a.o.newton & GN_NEWTON_FP32) {
-            auto k = k_gn_refill<K1, K2, EXACT, float, true, true>;
+            auto k = k_gn_refill<K1, K2, EXACT, float, true, kBulk>;
             refill_shape((const void *)k, a.n, grid, slices);
-            k<<<grid, kInvBlock, 0, st>>>(d, a, ctr, slices, a.o.fp32_iters, 0.0, a.o.tol_step32);
+            k<<<grid, kInvBlock, 0, st>>>(d, a, ctr, slices, a.o.fp32_iters, 0, 0.0,
+                                          a.o.tol_step32, 0.0);
         } else {
-            auto k = k_gn_refill<K1, K2, EXACT, float, false, true>;
+            auto k = k_gn_refill<K1, K2, EXACT, float, false, kBulk>;
             refill_shape((const void *)k, a.n, grid, slices);
-            k<<<grid, kInvBlock, 0, st>>>(d, a, ctr, slices, a.o.fp32_iters, 0.0, a.o.tol_step32);
+            k<<<grid, kInvBlock, 0, st>>>(d, a, ctr, slices, a.o.fp32_iters, 0, 0.0,
+                                          a.o.tol_step32, 0.0);
         }
-        const unsigned tgrid = grid_for(a.n, kInvBlock, (unsigned)cu_count() * 16);
-        if (a.o.newton & GN_NEWTON_FP64)
-            k_gn_tail<K1, K2, EXACT, true><<<tgrid, kInvBlock, 0, st>>>(d, a);
-        else
-            k_gn_tail<K1, K2, EXACT, false><<<tgrid, kInvBlock, 0, st>>>(d, a);
+        if ((rc = work_counters(st, &ctr))) return rc;
+        if (a.o.newton & GN_NEWTON_FP64) {
+            auto k = k_gn_refill<K1, K2, EXACT, double, true, kClose>;
+            refill_shape((const void *)k, a.n, grid, slices);
+            k<<<grid, kInvBlock, 0, st>>>(d, a, ctr, slices, a.o.fp64_tail, a.o.max_iter,
+                                          a.o.tol_dist, a.o.tol_step, a.o.tol_quad);
+        } else {
+            auto k = k_gn_refill<K1, K2, EXACT, double, false, kClose>;
+            refill_shape((const void *)k, a.n, grid, slices);
+            k<<<grid, kInvBlock, 0, st>>>(d, a, ctr, slices, a.o.fp64_tail, a.o.max_iter,
+                                          a.o.tol_dist, a.o.tol_step, 0.0);
+        }
         GN_HIP(hipGetLastError());
         return GN_OK;
     }

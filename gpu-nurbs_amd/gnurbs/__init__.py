@@ -48,6 +48,7 @@ class InvertOpts(ctypes.Structure):
         ("tol_step32", _c_double),
         ("seed_mode", _c_int),
         ("newton", _c_int),
+        ("tol_quad", _c_double),
     ]
 
 

@@ -80,6 +80,9 @@ typedef struct {
     int newton;         /* bitmask of GN_NEWTON_*: iterations that add the residual-curvature  */
                         /* term sum_k r_k d2S_k to J^T J (exact Hessian of |S-P|^2/2, falling  */
                         /* back to J^T J where that is not positive definite)                  */
+    double tol_quad;    /* FP64 exact-Hessian steps also stop (converged) once the step is     */
+                        /* below tol_quad: the error left after such a step is O(step^2);      */
+                        /* <= 0 off                                                            */
 } gn_invert_opts;
 
 enum { GN_NEWTON_FP32 = 1, /* FP32 bulk iterations of GN_MIXED                                */
