@@ -45,7 +45,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-points", type=int, default=1 << 18)
     ap.add_argument("--copy-calibration", action="store_true",
-                    help="also run one 256 MiB device-to-device copy (reference for HBM counters)")
+                    help="also run one 512 MiB read + 512 MiB write scale kernel (reference for HBM "
+                         "counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (inversion is "
@@ -61,11 +62,15 @@ def main():
     torch.cuda.set_device(0)
     dev = "cuda"
     if args.copy_calibration:
-        src = torch.ones(256 << 18, dtype=torch.float32, device=dev)  # 256 MiB
+        # known-size reference for the HBM counters: read 512 MiB, write 512 MiB (after a cache
+        # flush-sized fill of another buffer, so the read comes from HBM)
+        src = torch.ones(512 << 18, dtype=torch.float32, device=dev)
+        flush = torch.zeros(1024 << 18, dtype=torch.float32, device=dev)
         dst = torch.empty_like(src)
-        dst.copy_(src)
         torch.cuda.synchronize()
-        del src, dst
+        torch.mul(src, 2.0, out=dst)
+        torch.cuda.synchronize()
+        del src, dst, flush
     N = args.points
     sd = synth.surface_G(32, 32, 3, 3, seed=1002)
     gs = G.Surface(sd.p, sd.q, sd.ku, sd.kv, sd.cp, sd.w)
@@ -162,8 +167,16 @@ def main():
     t = gn_ms / 1e3
     ideal = f32 / PEAK_F32 + f64 / PEAK_F64
     peak_eff = (f32 + f64) / ideal
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "inversion_projection_1M_traffic.json")
+    if N == 1 << 20 and os.path.exists(tpath):
+        # HBM bytes of the two Gauss-Newton launches of one step, from the committed rocprofv3
+        # FETCH_SIZE / WRITE_SIZE passes (scaled by the copy-kernel calibration)
+        with open(tpath) as f:
+            traffic = json.load(f)["gn_step_traffic_bytes"]
     roof = {"bound": "mfma", "achieved": (f32 + f64) / t / 1e12, "peak": peak_eff / 1e12, "unit": "TFLOP/s",
-            "frac": ideal / t, "traffic": None,
+            "frac": ideal / t, "traffic": traffic,
+            "algorithmic_bytes": N * (40 + 17) + N * (41 + 17),
             "kernel": "k_gn_refill FP32 bulk + FP64 closing pass (Gauss-Newton from given seeds)",
             "roof_note": "compute roof: FP32 flops at 157.3 TFLOP/s plus FP64 flops at 78.6 TFLOP/s "
                          "(published MI355X peaks); VALU arithmetic",
