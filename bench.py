@@ -36,6 +36,61 @@ def flops_per_iter(k1, k2, newton):
     return f
 
 
+def bench_eval(G, synth, torch, steps):
+    """Evaluation at scale: bicubic 64x64 surface (generator G, seed 1001), 16,777,216 uniform
+    (u, v), point + derivatives to order 2.  FP64 per-point (160 B/pt), FP32 per-point (80 B/pt),
+    and the FP32 MFMA span-GEMM form on a 4096 x 4096 parameter grid (72 B/pt written).  FP32
+    error is measured against the FP64 evaluation (itself held to the FP64 oracle by the tests)."""
+    import numpy as np
+
+    n = 1 << 24
+    sd = synth.surface_G(64, 64, 3, 3, seed=1001)
+    gs = G.Surface(sd.p, sd.q, sd.ku, sd.kv, sd.cp, sd.w)
+    g = torch.Generator(device="cuda").manual_seed(1001)
+    u = torch.rand(n, dtype=torch.float64, device="cuda", generator=g)
+    v = torch.rand(n, dtype=torch.float64, device="cuda", generator=g)
+    u32, v32 = u.float(), v.float()
+    out64 = torch.empty((18, n), dtype=torch.float64, device="cuda")
+    out32 = torch.empty((18, n), dtype=torch.float32, device="cuda")
+    stream = torch.cuda.current_stream()
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        for _ in range(steps):
+            fn()
+        b.record(stream)
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) / steps
+
+    res = {"workload": "bicubic 64x64 net (G, seed 1001), 16,777,216 points, order 2 (18 outputs)"}
+    ms = timed(lambda: gs.Evaluate(u, v, 2, out=out64))
+    res["fp64"] = {"ms": ms, "points_per_s": n / (ms / 1e3), "GB_per_s": n * 160 / (ms / 1e3) / 1e9,
+                   "hbm_frac": n * 160 / (ms / 1e3) / PEAK_HBM}
+    ms = timed(lambda: gs.Evaluate(u32, v32, 2, precision=G.GN_FP32, out=out32))
+    ref = gs.Evaluate(u32.double(), v32.double(), 2)
+    err = []
+    for blk in range(6):
+        r = ref[3 * blk:3 * blk + 3]
+        e = (out32[3 * blk:3 * blk + 3].double() - r).abs()
+        scale = max(r.abs().max().item(), 1e-2 * ref[0:3].abs().max().item())
+        err.append(e.max().item() / scale)
+    res["fp32"] = {"ms": ms, "points_per_s": n / (ms / 1e3), "GB_per_s": n * 80 / (ms / 1e3) / 1e9,
+                   "hbm_frac": n * 80 / (ms / 1e3) / PEAK_HBM, "max_rel_err_vs_fp64": max(err)}
+    del ref
+    side = 4096
+    us = torch.linspace(0, 1, side, dtype=torch.float32, device="cuda")
+    ms = timed(lambda: gs.EvaluateGrid(us, us, 2, out=out32))
+    res["fp32_grid_mfma"] = {"ms": ms, "points_per_s": side * side / (ms / 1e3),
+                             "GB_per_s": side * side * 72 / (ms / 1e3) / 1e9,
+                             "hbm_frac": side * side * 72 / (ms / 1e3) / PEAK_HBM}
+    del out64, out32, u, v
+    torch.cuda.empty_cache()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -43,6 +98,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--points", type=int, default=1 << 20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-eval", action="store_true", help="skip the evaluation-at-scale measurement")
     ap.add_argument("--cpu-points", type=int, default=1 << 18)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 512 MiB read + 512 MiB write scale kernel (reference for HBM "
@@ -203,6 +259,8 @@ def main():
                          f"greedy-leaf seeds (seed search not timed); {dt:.2f} s wall",
                "ok_frac": cok / nc}
 
+    eval_res = None if args.no_eval else bench_eval(G, synth, torch, max(3, args.steps))
+
     mixed = res["mixed"]
     out = {
         "metric": "point inversions/s at <=1e-9 (u,v) err, mixed vs plain FP64; p50/p99 latency",
@@ -228,6 +286,7 @@ def main():
         "fp64_newton": res["fp64_newton"],
         "speedup_vs_plain_fp64": mixed["points_per_s"] / max(res["plain_fp64"]["points_per_s"], 1e-30),
         "seed_ms": seed_ms,
+        "eval_16M": eval_res,
     }
     print(json.dumps(out))
 

@@ -3,9 +3,10 @@
 //
 // One lane per query point, grid-stride over the batch.  Parameters stream in and results stream
 // out as SoA arrays, so every global load and store of the batch is a fully coalesced 256-byte
-// (FP32) or 512-byte (FP64) wave access.  Knots are staged in LDS for the span search; span
-// patches are gathered from the L2-resident span table.  The kernel is bound by HBM traffic
-// (8 or 16 bytes in, 12..144 bytes out per point).
+// (FP32) or 512-byte (FP64) wave access.  Knots are staged in LDS for the span search.  Span
+// patches come from the L2-resident span table: staged through LDS by cooperative contiguous
+// loads for patches up to 512 bytes (k_eval_lds), gathered per lane otherwise (k_eval).  The
+// target roof is HBM traffic (8 or 16 bytes in, 12..144 bytes out per point).
 #include "gn_device.hpp"
 
 namespace gn {
@@ -59,6 +60,96 @@ __global__ __launch_bounds__(kEvalBlock) void k_eval(GnSurfDev d, const TP *__re
     }
 }
 
+// LDS-staged form for small patches (at most 512 bytes: up to bicubic in FP64).  A wave takes 64
+// points at a time; it first gathers their 64 span patches cooperatively -- lane l loads 16-byte
+// vector l of the wave's patch list per instruction, so each load instruction covers whole
+// contiguous patches instead of 64 scattered lines -- into its own LDS area (patch stride padded
+// by 16 bytes against bank conflicts), then every lane contracts its own patch out of LDS.
+constexpr int kLdsBlock = 128;
+constexpr int kLdsWaves = kLdsBlock / 64;
+constexpr int kLdsKnotsSmall = 512;
+
+template <int K1, int K2, int ORDER, typename T, typename TP>
+__global__ __launch_bounds__(kLdsBlock) void k_eval_lds(GnSurfDev d, const TP *__restrict__ u,
+                                                         const TP *__restrict__ v, size_t n,
+                                                         T *__restrict__ out) {
+    constexpr int kVec = K1 * K2 * 4 * (int)sizeof(T) / 16;  // 16-byte vectors per patch
+    constexpr int kStride = kVec + 1;                         // padded patch stride (vectors)
+    static_assert(kVec * 16 <= 512, "patch too large for LDS staging");
+    __shared__ double sk[kLdsKnotsSmall];
+    __shared__ float4 stage[kLdsWaves * 64 * kStride];
+    const double *ku = d.ku, *kv = d.kv;
+    {
+        const int nku = d.nu + d.p + 1, nkv = d.nv + d.q + 1;
+        if (nku + nkv <= kLdsKnotsSmall) {
+            for (int i = threadIdx.x; i < nku; i += blockDim.x) sk[i] = d.ku[i];
+            for (int i = threadIdx.x; i < nkv; i += blockDim.x) sk[nku + i] = d.kv[i];
+            __syncthreads();
+            ku = sk;
+            kv = sk + nku;
+        }
+    }
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    float4 *my = stage + (size_t)wv * 64 * kStride;
+    const float4 *tab = reinterpret_cast<const float4 *>(sizeof(T) == 8 ? (const void *)d.tab64
+                                                                        : (const void *)d.tab32);
+    const size_t waves = (size_t)gridDim.x * kLdsWaves;
+    size_t base = ((size_t)blockIdx.x * kLdsWaves + wv) * 64;
+    // parameters of the next tile are loaded one tile ahead (software pipelining of the stream)
+    TP pu = 0, pv = 0;
+    if (base + lane < n) { pu = u[base + lane]; pv = v[base + lane]; }
+    for (; base < n; base += waves * 64) {
+        const size_t i = base + lane;
+        const bool live = i < n;
+        const double uu = live ? (double)pu : 0.0, vv = live ? (double)pv : 0.0;
+        const size_t inext = base + waves * 64 + lane;
+        if (inext < n) { pu = u[inext]; pv = v[inext]; }
+        const int m = find_span(ku, d.p, d.nu, uu);
+        const int nn = find_span(kv, d.q, d.nv, vv);
+        const int im = m - d.p, in = nn - d.q;
+        const int slot = im * d.sv + in;
+        // cooperative gather of the 64 patches
+#pragma unroll 4
+        for (int j = 0; j < kVec; j++) {
+            const int vec = j * 64 + lane;
+            const int pp = vec / kVec, o = vec - pp * kVec;
+            const int sp = __shfl(slot, pp, 64);
+            my[pp * kStride + o] = tab[(size_t)sp * kVec + o];
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const double ihu = d.ihu[im], ihv = d.ihv[in];
+        const T t = T((uu - ku[m]) * ihu), s = T((vv - kv[nn]) * ihv);
+        HomDer<T> h;
+        contract<K1, K2, ORDER, true, T>(reinterpret_cast<const T *>(my + lane * kStride), K1, K2, t,
+                                         s, h);
+        Cart<T> o;
+        quotient<ORDER, T>(h, T(ihu), T(ihv), o);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        if (live) {
+#pragma unroll
+            for (int c = 0; c < 3; c++) __builtin_nontemporal_store(o.S[c], out + c * n + i);
+            if (ORDER >= 1) {
+#pragma unroll
+                for (int c = 0; c < 3; c++) {
+                    __builtin_nontemporal_store(o.Su[c], out + (3 + c) * n + i);
+                    __builtin_nontemporal_store(o.Sv[c], out + (6 + c) * n + i);
+                }
+            }
+            if (ORDER >= 2) {
+#pragma unroll
+                for (int c = 0; c < 3; c++) {
+                    __builtin_nontemporal_store(o.Suu[c], out + (9 + c) * n + i);
+                    __builtin_nontemporal_store(o.Suv[c], out + (12 + c) * n + i);
+                    __builtin_nontemporal_store(o.Svv[c], out + (15 + c) * n + i);
+                }
+            }
+        }
+    }
+}
+
 template <typename T, typename TP>
 struct EvalLaunch {
     const GnSurfDev &d;
@@ -70,6 +161,16 @@ struct EvalLaunch {
     template <int K1, int K2, bool EXACT>
     int operator()() {
         const unsigned grid = grid_for(n, kEvalBlock, 256 * 16);
+        if constexpr (EXACT && K1 * K2 * 4 * sizeof(T) <= 512) {
+            const unsigned g2 = grid_for(n, kLdsBlock, 256 * 32);
+            switch (order) {
+                case 0: k_eval_lds<K1, K2, 0, T, TP><<<g2, kLdsBlock, 0, st>>>(d, u, v, n, out); break;
+                case 1: k_eval_lds<K1, K2, 1, T, TP><<<g2, kLdsBlock, 0, st>>>(d, u, v, n, out); break;
+                default: k_eval_lds<K1, K2, 2, T, TP><<<g2, kLdsBlock, 0, st>>>(d, u, v, n, out); break;
+            }
+            GN_HIP(hipGetLastError());
+            return GN_OK;
+        }
         switch (order) {
             case 0: k_eval<K1, K2, EXACT, 0, T, TP><<<grid, kEvalBlock, 0, st>>>(d, u, v, n, out); break;
             case 1: k_eval<K1, K2, EXACT, 1, T, TP><<<grid, kEvalBlock, 0, st>>>(d, u, v, n, out); break;

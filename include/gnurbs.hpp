@@ -1,0 +1,128 @@
+// gnurbs.hpp -- C++ host interface over the C ABI of libgnurbs (include/gnurbs.h).
+//
+// The reference passes a surface around as a loose tuple (degree_u, degree_v, cp_u, cp_v,
+// knots_u, knots_v, control_points, weights) and exposes CUDA kernels launched from
+// src/main.cu.  NurbsSurface wraps that tuple in a class, owns its device-resident span tables
+// and tree, and keeps the reference's entry-point names as methods:
+//   readNu        src/main.cu:45-113
+//   findSpan      src/NURBS/Evaluate.cu:504-533
+//   GetMatrixN    src/NURBS/Evaluate.cu:3-88       (span table; host copy of one span matrix)
+//   EvaluateRandom src/NURBS/Evaluate.cu:1023-1038 (plus the derivatives of 535-930)
+//   BuildTree     src/kernel/kernel.cu:375-431, 544-621, 707-733 (launch sequence main.cu:280-309)
+//   NewInverse    src/kernel/kernel.cu:433-469 with GaussNewton 151-249
+// Errors throw gnurbs::Error carrying the gn_status code and message (the reference exits the
+// process in HANDLE_ERROR, src/main.cu:34-43).  Query and result buffers are device pointers
+// owned by the caller, as in the reference.
+#ifndef GNURBS_HPP
+#define GNURBS_HPP
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "gnurbs.h"
+
+namespace gnurbs {
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+inline void check(int rc) {
+    if (rc != GN_OK) throw Error(rc, gn_last_error());
+}
+
+inline gn_invert_opts default_opts() {
+    gn_invert_opts o;
+    gn_invert_opts_default(&o);
+    return o;
+}
+
+class NurbsSurface {
+  public:
+    NurbsSurface(int degree_u, int degree_v, int cp_u, int cp_v, const std::vector<double> &knots_u,
+                 const std::vector<double> &knots_v, const std::vector<double> &control_points_xyz,
+                 const std::vector<double> &weights, gn_stream stream = nullptr) {
+        if ((long)knots_u.size() != cp_u + degree_u + 1 || (long)knots_v.size() != cp_v + degree_v + 1 ||
+            (long)control_points_xyz.size() != 3L * cp_u * cp_v || (long)weights.size() != (long)cp_u * cp_v)
+            throw Error(GN_EINVAL, "surface arrays do not match the degrees and net size");
+        check(gn_surface_create(degree_u, degree_v, cp_u, cp_v, knots_u.data(), knots_v.data(),
+                                control_points_xyz.data(), weights.data(), stream, &s_));
+        check(gn_surface_info(s_, &p_, &q_, &nu_, &nv_));
+    }
+    static NurbsSurface readNu(const std::string &path, gn_stream stream = nullptr) {
+        gn_surface *s = nullptr;
+        check(gn_surface_from_nu(path.c_str(), stream, &s));
+        return NurbsSurface(s);
+    }
+    NurbsSurface(const NurbsSurface &) = delete;
+    NurbsSurface &operator=(const NurbsSurface &) = delete;
+    NurbsSurface(NurbsSurface &&o) noexcept { *this = std::move(o); }
+    NurbsSurface &operator=(NurbsSurface &&o) noexcept {
+        if (this != &o) {
+            reset();
+            s_ = o.s_;
+            p_ = o.p_; q_ = o.q_; nu_ = o.nu_; nv_ = o.nv_; levels_ = o.levels_;
+            o.s_ = nullptr;
+        }
+        return *this;
+    }
+    ~NurbsSurface() { reset(); }
+
+    int degree_u() const { return p_; }
+    int degree_v() const { return q_; }
+    int cp_u() const { return nu_; }
+    int cp_v() const { return nv_; }
+    gn_surface *handle() const { return s_; }
+
+    // findSpan for n device parameters (dir 0 = u, 1 = v) into device int32 spans
+    void findSpan(int dir, const double *u, size_t n, int32_t *span, gn_stream st = nullptr) const {
+        check(gn_find_span(s_, dir, u, n, span, st));
+    }
+    // span matrix (i, j) of MatrixN, [c][r][s], 4*(p+1)*(q+1) doubles
+    std::vector<double> GetMatrixN(int i, int j) const {
+        std::vector<double> m(4 * (p_ + 1) * (q_ + 1));
+        check(gn_get_span_matrix(s_, i, j, m.data()));
+        return m;
+    }
+    // points (order 0) and derivatives up to order 2, SoA blocks of n on the device
+    void EvaluateRandom(const double *u, const double *v, size_t n, int order, double *out,
+                        gn_stream st = nullptr) const {
+        check(gn_evaluate(s_, u, v, n, order, out, st));
+    }
+    void EvaluateRandomF32(const float *u, const float *v, size_t n, int order, float *out,
+                           gn_stream st = nullptr) const {
+        check(gn_evaluate_f32(s_, u, v, n, order, out, st));
+    }
+    void EvaluateGrid(const float *us, int nu, const float *vs, int nv, int order, float *out,
+                      gn_stream st = nullptr) const {
+        check(gn_evaluate_grid_f32(s_, us, nu, vs, nv, order, out, st));
+    }
+    // tree of `levels` 8x8 levels (the reference's `level` loop of src/main.cu:283)
+    void BuildTree(int levels, int flags = 0, gn_stream st = nullptr) {
+        check(gn_build_tree(s_, levels, flags, st));
+        levels_ = levels;
+    }
+    // inversion of device points (x, y, z) into device (u, v, status)
+    void NewInverse(const double *x, const double *y, const double *z, size_t n, double *u,
+                    double *v, uint8_t *status, const gn_invert_opts &o = default_opts(),
+                    gn_stream st = nullptr) const {
+        check(gn_invert(s_, x, y, z, n, &o, u, v, status, st));
+    }
+
+  private:
+    explicit NurbsSurface(gn_surface *s) : s_(s) { check(gn_surface_info(s_, &p_, &q_, &nu_, &nv_)); }
+    void reset() {
+        if (s_) gn_surface_destroy(s_);
+        s_ = nullptr;
+    }
+    gn_surface *s_ = nullptr;
+    int p_ = 0, q_ = 0, nu_ = 0, nv_ = 0, levels_ = 0;
+};
+
+}  // namespace gnurbs
+
+#endif  // GNURBS_HPP
