@@ -1,0 +1,392 @@
+// batch_api.cpp -- C ABI of the batched inversion: packing many surfaces into one device
+// allocation, the tile list, the tile-shape autotuner and hipGraph capture of a batch launch.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <fstream>
+#include <mutex>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "gn_internal.hpp"
+
+using gn::fail;
+
+struct gn_batch {
+    int count = 0;
+    std::vector<int> cls;
+    GnBatchObj *d_objs = nullptr;
+    char *d_pack = nullptr;
+    size_t bytes = 0;
+    int tile = 1024;  // points per tile
+    int bpc = 0;      // workgroups per CU cap (0: occupancy limit)
+};
+
+struct gn_graph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    GnTile *d_tiles = nullptr;
+};
+
+namespace {
+
+size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+int degree_class(int p, int q) {
+    if (p >= 2 && p <= 5 && q >= 2 && q <= 5) return (p - 2) * 4 + (q - 2);
+    return 16;
+}
+
+int check_segments(const gn_batch *b, const gn_segment *segs, int nseg, size_t n) {
+    if (nseg < 0 || (nseg > 0 && !segs)) return fail(GN_EINVAL, "bad segment list");
+    for (int i = 0; i < nseg; i++) {
+        const gn_segment &s = segs[i];
+        if (s.obj < 0 || s.obj >= b->count) return fail(GN_EINVAL, "segment object out of range");
+        if (s.count < 0 || s.start < 0 || (size_t)s.start + (size_t)s.count > n)
+            return fail(GN_EINVAL, "segment outside the point arrays");
+    }
+    return GN_OK;
+}
+
+std::vector<GnTile> build_tiles(const gn_batch *b, const gn_segment *segs, int nseg, int tile) {
+    std::vector<GnTile> t;
+    for (int i = 0; i < nseg; i++) {
+        for (int64_t off = 0; off < segs[i].count; off += tile) {
+            const int64_t c = std::min<int64_t>(tile, segs[i].count - off);
+            t.push_back(GnTile{segs[i].obj, (int32_t)c, segs[i].start + off});
+        }
+    }
+    std::stable_sort(t.begin(), t.end(), [b](const GnTile &x, const GnTile &y) {
+        return b->cls[x.obj] < b->cls[y.obj];
+    });
+    return t;
+}
+
+void free_tiles_host(void *p) { delete static_cast<std::vector<GnTile> *>(p); }
+
+// Tiles go to a stream-ordered device buffer; the host copy is released by a host callback in
+// stream order, after the upload has consumed it.
+int run_batch(gn_batch *b, const gn_segment *segs, int nseg, const double *x, const double *y,
+              const double *z, size_t n, const gn_invert_opts &o, double *u, double *v,
+              uint8_t *status, uint8_t *it32, uint8_t *it64, int tile, int bpc, hipStream_t st) {
+    auto *tiles = new std::vector<GnTile>(build_tiles(b, segs, nseg, tile));
+    if (tiles->empty()) {
+        delete tiles;
+        return GN_OK;
+    }
+    const size_t nt = tiles->size();
+    GnTile *d_tiles = nullptr;
+    hipError_t e = hipMallocAsync((void **)&d_tiles, sizeof(GnTile) * nt, st);
+    if (e != hipSuccess) {
+        delete tiles;
+        return gn::hip_fail(e, "hipMallocAsync(tiles)");
+    }
+    e = hipMemcpyAsync(d_tiles, tiles->data(), sizeof(GnTile) * nt, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipLaunchHostFunc(st, free_tiles_host, tiles);
+    if (e != hipSuccess) {
+        (void)hipStreamSynchronize(st);
+        delete tiles;
+        (void)hipFreeAsync(d_tiles, st);
+        return gn::hip_fail(e, "tile upload");
+    }
+    int rc = gn::launch_batch_invert(b->d_objs, d_tiles, (int)nt, x, y, z, n, o, u, v, status,
+                                     it32, it64, bpc, st);
+    GN_HIP(hipFreeAsync(d_tiles, st));
+    return rc;
+}
+
+int check_opts(const gn_invert_opts &o) {
+    if (o.precision != GN_FP64 && o.precision != GN_MIXED)
+        return fail(GN_EINVAL, "precision must be GN_FP64 or GN_MIXED");
+    if (o.seed_mode == GN_SEED_REF_FIRST_HIT)
+        return fail(GN_EINVAL, "first-hit seeding is a single-surface reference mode");
+    if (o.seed_mode < GN_SEED_NEAREST_LEAF || o.seed_mode > GN_SEED_GREEDY_LEAF)
+        return fail(GN_EINVAL, "bad seed_mode");
+    if (o.max_iter < 0 || o.fp32_iters < 0 || o.fp64_tail < 0 || o.max_iter > 10000 ||
+        o.fp32_iters > 10000 || o.fp64_tail > 10000)
+        return fail(GN_EINVAL, "iteration counts must be 0..10000");
+    return GN_OK;
+}
+
+std::string signature(const gn_batch *b, const gn_segment *segs, int nseg, size_t n) {
+    std::vector<double> per(17, 0.0);
+    for (int i = 0; i < nseg; i++) per[b->cls[segs[i].obj]] += (double)segs[i].count;
+    std::ostringstream os;
+    os << "objs=" << b->count << ",n2=" << (n ? (int)std::log2((double)n) : 0);
+    for (int c = 0; c < 17; c++) os << "," << (per[c] > 0 ? (int)std::log2(per[c]) : -1);
+    return os.str();
+}
+
+}  // namespace
+
+extern "C" {
+
+int gn_batch_create(gn_surface *const *surfaces, int count, gn_stream stream, gn_batch **out) {
+    if (!out) return fail(GN_EINVAL, "out is null");
+    *out = nullptr;
+    if (count <= 0 || !surfaces) return fail(GN_EINVAL, "empty surface list");
+    hipStream_t st = (hipStream_t)stream;
+    // layout of the pack
+    struct Part {
+        const void *src;
+        size_t bytes;
+        size_t off;
+    };
+    std::vector<std::vector<Part>> parts(count);
+    size_t total = 0;
+    for (int i = 0; i < count; i++) {
+        const gn_surface *s = surfaces[i];
+        if (!s) return fail(GN_EINVAL, "null surface in the list");
+        if (!s->tree_levels) return fail(GN_ESTATE, "every surface needs its tree (gn_build_tree)");
+        const int D = 3 * (s->tree_levels - 1);
+        const size_t qnodes = (((size_t)1 << (2 * (D + 1))) - 1) / 3, leaves = (size_t)1 << (2 * D);
+        const Part ps[] = {
+            {s->d_ku, sizeof(double) * s->ku.size(), 0},
+            {s->d_kv, sizeof(double) * s->kv.size(), 0},
+            {s->d_ihu, sizeof(double) * (s->nu - s->p), 0},
+            {s->d_ihv, sizeof(double) * (s->nv - s->q), 0},
+            {s->d_tab64, sizeof(double) * s->tab_entries, 0},
+            {s->d_tab32, sizeof(float) * s->tab_entries, 0},
+            {s->d_sbox32, sizeof(float) * qnodes * 6, 0},
+            {s->d_scentre, sizeof(double) * leaves * 3, 0},
+            {s->d_sseed, sizeof(double) * leaves * 2, 0},
+            {s->d_sref, sizeof(int32_t) * leaves, 0},
+        };
+        for (Part p : ps) {
+            p.off = total;
+            total += align256(p.bytes);
+            parts[i].push_back(p);
+        }
+    }
+    gn_batch *b = new gn_batch();
+    b->count = count;
+    b->cls.resize(count);
+    hipError_t e = hipMalloc((void **)&b->d_pack, total);
+    if (e != hipSuccess) {
+        delete b;
+        return gn::hip_fail(e, "hipMalloc(batch pack)");
+    }
+    e = hipMalloc((void **)&b->d_objs, sizeof(GnBatchObj) * count);
+    if (e != hipSuccess) {
+        (void)hipFree(b->d_pack);
+        delete b;
+        return gn::hip_fail(e, "hipMalloc(batch objects)");
+    }
+    b->bytes = total + sizeof(GnBatchObj) * count;
+    std::vector<GnBatchObj> objs(count);
+    for (int i = 0; i < count && e == hipSuccess; i++) {
+        const gn_surface *s = surfaces[i];
+        for (const Part &p : parts[i])
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(b->d_pack + p.off, p.src, p.bytes, hipMemcpyDeviceToDevice, st);
+        char *base = b->d_pack;
+        GnBatchObj &ob = objs[i];
+        ob.s = s->dev();
+        ob.s.ku = (const double *)(base + parts[i][0].off);
+        ob.s.kv = (const double *)(base + parts[i][1].off);
+        ob.s.ihu = (const double *)(base + parts[i][2].off);
+        ob.s.ihv = (const double *)(base + parts[i][3].off);
+        ob.s.tab64 = (const double *)(base + parts[i][4].off);
+        ob.s.tab32 = (const float *)(base + parts[i][5].off);
+        ob.sd = s->search();
+        ob.sd.box = nullptr;
+        ob.sd.box32 = (const float *)(base + parts[i][6].off);
+        ob.sd.centre = (const double *)(base + parts[i][7].off);
+        ob.sd.seed_uv = (const double *)(base + parts[i][8].off);
+        ob.sd.ref_index = (const int32_t *)(base + parts[i][9].off);
+        ob.cls = degree_class(s->p, s->q);
+        ob.pad_ = 0;
+        b->cls[i] = ob.cls;
+    }
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(b->d_objs, objs.data(), sizeof(GnBatchObj) * count,
+                           hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) {
+        (void)hipFree(b->d_pack);
+        (void)hipFree(b->d_objs);
+        delete b;
+        return gn::hip_fail(e, "batch packing copies");
+    }
+    *out = b;
+    return GN_OK;
+}
+
+int gn_batch_destroy(gn_batch *b) {
+    if (!b) return GN_OK;
+    (void)hipDeviceSynchronize();
+    (void)hipFree(b->d_pack);
+    (void)hipFree(b->d_objs);
+    delete b;
+    return GN_OK;
+}
+
+size_t gn_batch_device_bytes(const gn_batch *b) { return b ? b->bytes : 0; }
+
+int gn_batch_set_shape(gn_batch *b, int tile_points, int blocks_per_cu) {
+    if (!b) return fail(GN_EINVAL, "null batch");
+    if (tile_points < 1 || tile_points > (1 << 24) || blocks_per_cu < 0 || blocks_per_cu > 64)
+        return fail(GN_EINVAL, "tile_points 1..2^24, blocks_per_cu 0..64");
+    b->tile = tile_points;
+    b->bpc = blocks_per_cu;
+    return GN_OK;
+}
+
+int gn_batch_get_shape(const gn_batch *b, int *tile_points, int *blocks_per_cu) {
+    if (!b) return fail(GN_EINVAL, "null batch");
+    if (tile_points) *tile_points = b->tile;
+    if (blocks_per_cu) *blocks_per_cu = b->bpc;
+    return GN_OK;
+}
+
+int gn_batch_invert(gn_batch *b, const gn_segment *segs, int nseg, const double *x,
+                    const double *y, const double *z, size_t n, const gn_invert_opts *opts,
+                    double *u, double *v, uint8_t *status, gn_stream stream) {
+    if (!b) return fail(GN_EINVAL, "null batch");
+    gn_invert_opts o;
+    if (opts) o = *opts;
+    else gn_invert_opts_default(&o);
+    int rc = check_opts(o);
+    if (rc) return rc;
+    if ((rc = check_segments(b, segs, nseg, n))) return rc;
+    if (n && (!x || !y || !z || !u || !v || !status)) return fail(GN_EINVAL, "null buffer");
+    return run_batch(b, segs, nseg, x, y, z, n, o, u, v, status, nullptr, nullptr, b->tile, b->bpc,
+                     (hipStream_t)stream);
+}
+
+int gn_batch_tune(gn_batch *b, const gn_segment *segs, int nseg, const double *x, const double *y,
+                  const double *z, size_t n, const gn_invert_opts *opts, double *u, double *v,
+                  uint8_t *status, const char *cache_path, gn_stream stream) {
+    if (!b) return fail(GN_EINVAL, "null batch");
+    gn_invert_opts o;
+    if (opts) o = *opts;
+    else gn_invert_opts_default(&o);
+    int rc = check_opts(o);
+    if (rc) return rc;
+    if ((rc = check_segments(b, segs, nseg, n))) return rc;
+    if (n && (!x || !y || !z || !u || !v || !status)) return fail(GN_EINVAL, "null buffer");
+    const std::string sig = signature(b, segs, nseg, n);
+    if (cache_path) {  // cached shape for this signature?
+        std::ifstream in(cache_path);
+        std::string line;
+        while (std::getline(in, line)) {
+            std::istringstream ls(line);
+            std::string s;
+            int t, c;
+            if (ls >> s >> t >> c && s == sig && t > 0 && c >= 0) {
+                b->tile = t;
+                b->bpc = c;
+                return GN_OK;
+            }
+        }
+    }
+    hipStream_t st = (hipStream_t)stream;
+    hipEvent_t e0, e1;
+    GN_HIP(hipEventCreate(&e0));
+    GN_HIP(hipEventCreate(&e1));
+    const int tiles[] = {256, 512, 1024, 2048, 4096};
+    const int caps[] = {0, 1};
+    float best = 1e30f;
+    int bt = b->tile, bc = b->bpc;
+    for (int t : tiles) {
+        for (int c : caps) {
+            float sum = 0;
+            for (int r = 0; r < 3 && !rc; r++) {  // first run warms up, not counted
+                if (o.seed_mode == GN_SEED_GIVEN) break;  // seeds would be overwritten
+                GN_HIP(hipEventRecord(e0, st));
+                rc = run_batch(b, segs, nseg, x, y, z, n, o, u, v, status, nullptr, nullptr, t, c, st);
+                GN_HIP(hipEventRecord(e1, st));
+                GN_HIP(hipEventSynchronize(e1));
+                float ms = 0;
+                GN_HIP(hipEventElapsedTime(&ms, e0, e1));
+                if (r) sum += ms;
+            }
+            if (rc) break;
+            if (sum > 0 && sum < best) {
+                best = sum;
+                bt = t;
+                bc = c;
+            }
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    b->tile = bt;
+    b->bpc = bc;
+    if (cache_path) {
+        std::ofstream outf(cache_path, std::ios::app);
+        outf << sig << " " << bt << " " << bc << "\n";
+    }
+    return GN_OK;
+}
+
+int gn_batch_graph_create(gn_batch *b, const gn_segment *segs, int nseg, const double *x,
+                          const double *y, const double *z, size_t n, const gn_invert_opts *opts,
+                          double *u, double *v, uint8_t *status, gn_stream stream,
+                          gn_graph **out) {
+    if (!b || !out) return fail(GN_EINVAL, "null argument");
+    *out = nullptr;
+    if (!stream) return fail(GN_EINVAL, "graph capture needs a created (non-null) stream");
+    gn_invert_opts o;
+    if (opts) o = *opts;
+    else gn_invert_opts_default(&o);
+    int rc = check_opts(o);
+    if (rc) return rc;
+    if ((rc = check_segments(b, segs, nseg, n))) return rc;
+    if (n && (!x || !y || !z || !u || !v || !status)) return fail(GN_EINVAL, "null buffer");
+    if ((rc = gn::ensure_runtime())) return rc;
+    std::vector<GnTile> tiles = build_tiles(b, segs, nseg, b->tile);
+    gn_graph *g = new gn_graph();
+    hipStream_t st = (hipStream_t)stream;
+    if (!tiles.empty()) {
+        hipError_t e = hipMalloc((void **)&g->d_tiles, sizeof(GnTile) * tiles.size());
+        if (e == hipSuccess)
+            e = hipMemcpy(g->d_tiles, tiles.data(), sizeof(GnTile) * tiles.size(),
+                          hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            (void)hipFree(g->d_tiles);
+            delete g;
+            return gn::hip_fail(e, "graph tile list");
+        }
+    }
+    hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) {
+        (void)hipFree(g->d_tiles);
+        delete g;
+        return gn::hip_fail(e, "hipStreamBeginCapture");
+    }
+    rc = gn::launch_batch_invert(b->d_objs, g->d_tiles, (int)tiles.size(), x, y, z, n, o, u, v,
+                                 status, nullptr, nullptr, b->bpc, st);
+    e = hipStreamEndCapture(st, &g->graph);
+    if (!rc && e == hipSuccess) e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+    if (rc || e != hipSuccess) {
+        if (g->graph) (void)hipGraphDestroy(g->graph);
+        (void)hipFree(g->d_tiles);
+        delete g;
+        return rc ? rc : gn::hip_fail(e, "graph capture / instantiate");
+    }
+    *out = g;
+    return GN_OK;
+}
+
+int gn_graph_launch(gn_graph *g, gn_stream stream) {
+    if (!g) return fail(GN_EINVAL, "null graph");
+    GN_HIP(hipGraphLaunch(g->exec, (hipStream_t)stream));
+    return GN_OK;
+}
+
+int gn_graph_destroy(gn_graph *g) {
+    if (!g) return GN_OK;
+    (void)hipDeviceSynchronize();
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    (void)hipFree(g->d_tiles);
+    delete g;
+    return GN_OK;
+}
+
+}  // extern "C"

@@ -41,6 +41,22 @@ struct GnSearchDev {
     const int32_t *ref_index; // per leaf (Morton order) reference tree index
 };
 
+// One object of a batch: the surface and its seed tree (pointers into the batch's pack) and its
+// degree class: (p-2)*4 + (q-2) for p, q in 2..5, 16 otherwise.
+struct GnBatchObj {
+    GnSurfDev s;
+    GnSearchDev sd;
+    int cls;
+    int pad_;
+};
+
+// A tile of work: `count` points of object `obj` starting at point `start`.
+struct GnTile {
+    int32_t obj;
+    int32_t count;
+    int64_t start;
+};
+
 // ---------------------------------------------------------------------------------------------
 // error handling
 // ---------------------------------------------------------------------------------------------
@@ -117,6 +133,11 @@ int launch_search_build(const GnSurfDev &d, int levels, const GnTreeNode *tree, 
 int launch_seed(const GnSearchDev &sd, int mode, const double *x, const double *y,
                 const double *z, size_t n, int32_t *leaf, double *su, double *sv,
                 hipStream_t st);
+int launch_batch_invert(const GnBatchObj *objs, const GnTile *tiles, int ntiles, const double *x,
+                        const double *y, const double *z, size_t n, const gn_invert_opts &o,
+                        double *u, double *v, uint8_t *status, uint8_t *it32, uint8_t *it64,
+                        int bpc_cap, hipStream_t st);
+int ensure_runtime();  // allocates the per-device claim counters (before any graph capture)
 int launch_invert(const GnSurfDev &d, const GnSearchDev &sd, const GnTreeNode *tree,
                   const uint8_t *valid, int tree_levels, const double *x, const double *y,
                   const double *z, size_t n, const gn_invert_opts &o, double *u, double *v,

@@ -1,0 +1,133 @@
+// gn_search.hpp -- device functions of the seed search over the quadtree of tree leaves
+// (exact nearest leaf centre, and greedy descent + 3 x 3 neighbourhood), shared by the
+// single-surface seed kernel and the batched inversion kernels.
+#pragma once
+#include "gn_device.hpp"
+
+namespace gn {
+
+__host__ __device__ inline long qoff(int l) { return ((1L << (2 * l)) - 1) / 3; }
+
+__device__ inline uint32_t morton2(uint32_t x, uint32_t y) {  // x bits odd, y bits even
+    uint32_t c = 0;
+    for (int b = 0; b < 16; b++) c |= ((x >> b) & 1u) << (2 * b + 1) | ((y >> b) & 1u) << (2 * b);
+    return c;
+}
+
+// Squared distance from p to an FP32 box (0 inside); a lower bound of the FP64 distance from P to
+// the FP64 box, because the FP32 box contains it with a margin.
+__device__ __forceinline__ float box_d2(const float *__restrict__ b, const float p[3]) {
+    float s = 0.f;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const float e = fmaxf(fmaxf(b[k] - p[k], p[k] - b[3 + k]), 0.f);
+        s = fmaf(e, e, s);
+    }
+    return s;
+}
+
+// Exact nearest leaf centre (ties to the lower reference index).  Returns the Morton index.
+// Pruning uses the FP32 boxes (a conservative bound); leaf distances are FP64.
+__device__ inline uint32_t nearest_leaf(const GnSearchDev &sd, const double P[3]) {
+    const int D = sd.depth;
+    if (D == 0) return 0;
+    const float p[3] = {(float)P[0], (float)P[1], (float)P[2]};
+    // greedy descent for an initial bound
+    uint32_t c = 0;
+    for (int l = 0; l < D; l++) {
+        float bd = INFINITY;
+        uint32_t bc = 4 * c;
+        const float *b = sd.box32 + (qoff(l + 1) + 4 * c) * 6;
+#pragma unroll
+        for (int a = 0; a < 4; a++) {
+            const float dd = box_d2(b + a * 6, p);
+            if (dd < bd) { bd = dd; bc = 4 * c + a; }
+        }
+        c = bc;
+    }
+    auto cdist = [&](uint32_t leaf) {
+        const double *q = sd.centre + leaf * 3;
+        const double dx = P[0] - q[0], dy = P[1] - q[1], dz = P[2] - q[2];
+        return fma(dz, dz, fma(dy, dy, dx * dx));
+    };
+    uint32_t best_c = c;
+    double best = cdist(c);
+    // stackless depth-first walk with pruning
+    int l = 1;
+    c = 0;
+    while (true) {
+        const double lb = (double)box_d2(sd.box32 + (qoff(l) + c) * 6, p);
+        bool descend = false;
+        if (lb <= best) {
+            if (l == D) {
+                const double dd = cdist(c);
+                if (dd < best || (dd == best && sd.ref_index[c] < sd.ref_index[best_c])) {
+                    best = dd;
+                    best_c = c;
+                }
+            } else {
+                descend = true;
+            }
+        }
+        if (descend) {
+            c <<= 2;
+            l++;
+            continue;
+        }
+        while (l > 1 && (c & 3u) == 3u) {
+            c >>= 2;
+            l--;
+        }
+        if (l == 1 && (c & 3u) == 3u) break;
+        c++;
+    }
+    return best_c;
+}
+
+__device__ inline uint32_t compact2(uint32_t c) {  // inverse of the interleave, even bits
+    uint32_t r = 0;
+    for (int b = 0; b < 16; b++) r |= ((c >> (2 * b)) & 1u) << b;
+    return r;
+}
+
+// Fast seed: greedy descent by FP32 box distance (ties to the lower child), then the nearest
+// leaf centre (FP64, ties to the lower reference index) among the 3 x 3 leaves around the
+// descent's leaf in the uv grid.  Fixed work per point: 4*D box tests and 9 distances.
+__device__ inline uint32_t greedy_leaf(const GnSearchDev &sd, const double P[3]) {
+    const int D = sd.depth;
+    if (D == 0) return 0;
+    const float p[3] = {(float)P[0], (float)P[1], (float)P[2]};
+    uint32_t c = 0;
+    for (int l = 0; l < D; l++) {
+        float bd = INFINITY;
+        uint32_t bc = 4 * c;
+        const float *b = sd.box32 + (qoff(l + 1) + 4 * c) * 6;
+#pragma unroll
+        for (int a = 0; a < 4; a++) {
+            const float dd = box_d2(b + a * 6, p);
+            if (dd < bd) { bd = dd; bc = 4 * c + a; }
+        }
+        c = bc;
+    }
+    const int side = 1 << D;
+    const int gx = (int)compact2(c >> 1), gy = (int)compact2(c);
+    double best = INFINITY;
+    uint32_t best_c = c;
+    for (int dx = -1; dx <= 1; dx++) {
+        for (int dy = -1; dy <= 1; dy++) {
+            const int x = gx + dx, y = gy + dy;
+            if (x < 0 || y < 0 || x >= side || y >= side) continue;
+            const uint32_t m = morton2((uint32_t)x, (uint32_t)y);
+            const double *q = sd.centre + m * 3;
+            const double ex = P[0] - q[0], ey = P[1] - q[1], ez = P[2] - q[2];
+            const double dd = fma(ez, ez, fma(ey, ey, ex * ex));
+            if (dd < best || (dd == best && sd.ref_index[m] < sd.ref_index[best_c])) {
+                best = dd;
+                best_c = m;
+            }
+        }
+    }
+    return best_c;
+}
+
+}  // namespace gn

@@ -28,20 +28,26 @@ int g_cus[kMaxDevices] = {};
 std::atomic<unsigned> g_next{0};
 }  // namespace
 
-int work_counters(hipStream_t st, unsigned long long **out) {
+int ensure_runtime() {
     int dev = 0;
     GN_HIP(hipGetDevice(&dev));
     if (dev < 0 || dev >= kMaxDevices) return fail(GN_EDEVICE, "device index out of range");
-    {
-        std::lock_guard<std::mutex> lk(g_mu);
-        if (!g_counters[dev]) {
-            void *p = nullptr;
-            GN_HIP(hipMalloc(&p, sizeof(unsigned long long) * kCounterSlots * kSlotWords));
-            GN_HIP(hipMemset(p, 0, sizeof(unsigned long long) * kCounterSlots * kSlotWords));
-            GN_HIP(hipDeviceSynchronize());
-            g_counters[dev] = (unsigned long long *)p;
-        }
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!g_counters[dev]) {
+        void *p = nullptr;
+        GN_HIP(hipMalloc(&p, sizeof(unsigned long long) * kCounterSlots * kSlotWords));
+        GN_HIP(hipMemset(p, 0, sizeof(unsigned long long) * kCounterSlots * kSlotWords));
+        GN_HIP(hipDeviceSynchronize());
+        g_counters[dev] = (unsigned long long *)p;
     }
+    return GN_OK;
+}
+
+int work_counters(hipStream_t st, unsigned long long **out) {
+    int rc = ensure_runtime();
+    if (rc) return rc;
+    int dev = 0;
+    GN_HIP(hipGetDevice(&dev));
     const unsigned slot = g_next.fetch_add(1, std::memory_order_relaxed) % kCounterSlots;
     unsigned long long *c = g_counters[dev] + (size_t)slot * kSlotWords;
     k_zero_slot<<<1, 64, 0, st>>>(c);

@@ -81,7 +81,19 @@ _SIGS = [
     ("gn_seed", _c_int, [_c_void, _c_int, _c_void, _c_void, _c_void, _c_size, _c_void, _c_void, _c_void, _c_void]),
     ("gn_invert", _c_int, [_c_void, _c_void, _c_void, _c_void, _c_size, ctypes.POINTER(InvertOpts), _c_void, _c_void, _c_void, _c_void]),
     ("gn_invert_iters", _c_int, [_c_void, _c_void, _c_void, _c_void, _c_size, ctypes.POINTER(InvertOpts), _c_void, _c_void, _c_void, _c_void, _c_void, _c_void]),
+    ("gn_batch_create", _c_int, [_c_void, _c_int, _c_void, ctypes.POINTER(_c_void)]),
+    ("gn_batch_destroy", _c_int, [_c_void]),
+    ("gn_batch_device_bytes", _c_size, [_c_void]),
+    ("gn_batch_set_shape", _c_int, [_c_void, _c_int, _c_int]),
+    ("gn_batch_get_shape", _c_int, [_c_void, ctypes.POINTER(_c_int), ctypes.POINTER(_c_int)]),
+    ("gn_batch_invert", _c_int, [_c_void, _c_void, _c_int, _c_void, _c_void, _c_void, _c_size, ctypes.POINTER(InvertOpts), _c_void, _c_void, _c_void, _c_void]),
+    ("gn_batch_tune", _c_int, [_c_void, _c_void, _c_int, _c_void, _c_void, _c_void, _c_size, ctypes.POINTER(InvertOpts), _c_void, _c_void, _c_void, ctypes.c_char_p, _c_void]),
+    ("gn_batch_graph_create", _c_int, [_c_void, _c_void, _c_int, _c_void, _c_void, _c_void, _c_size, ctypes.POINTER(InvertOpts), _c_void, _c_void, _c_void, _c_void, ctypes.POINTER(_c_void)]),
+    ("gn_graph_launch", _c_int, [_c_void, _c_void]),
+    ("gn_graph_destroy", _c_int, [_c_void]),
 ]
+
+SEGMENT = np.dtype([("obj", np.int32), ("count", np.int32), ("start", np.int64)])
 
 
 def lib():
@@ -312,3 +324,109 @@ class Surface:
         return u, v, st, it32, it64
 
     Invert = NewInverse
+
+
+def segments(objs, counts, starts=None) -> np.ndarray:
+    """gn_segment array: points starts[i] .. starts[i]+counts[i]-1 belong to object objs[i]
+    (starts default to the running sum of counts)."""
+    counts = np.asarray(counts, dtype=np.int64)
+    if starts is None:
+        starts = np.concatenate([[0], np.cumsum(counts)[:-1]])
+    s = np.zeros(len(counts), dtype=SEGMENT)
+    s["obj"], s["count"], s["start"] = objs, counts, starts
+    return s
+
+
+class Graph:
+    """An instantiated hipGraph of one batched inversion (gn_graph)."""
+
+    def __init__(self, handle, keep):
+        self._h, self._keep = handle, keep
+
+    def launch(self, stream=None):
+        _check(lib().gn_graph_launch(self._h, _c_void(stream_handle(stream))))
+
+    def close(self):
+        if self._h:
+            lib().gn_graph_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Many surfaces packed for one persistent inversion launch (gn_batch).  Every surface must
+    have its tree built."""
+
+    def __init__(self, surfaces, stream=None):
+        arr = (_c_void * len(surfaces))(*[s.handle for s in surfaces])
+        h = _c_void()
+        _check(lib().gn_batch_create(arr, len(surfaces), _c_void(stream_handle(stream)), ctypes.byref(h)))
+        self._h = h
+        self.count = len(surfaces)
+        _live.add(self)
+
+    def close(self):
+        if self._h:
+            lib().gn_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def device_bytes(self) -> int:
+        return int(lib().gn_batch_device_bytes(self._h))
+
+    def set_shape(self, tile_points, blocks_per_cu=0):
+        _check(lib().gn_batch_set_shape(self._h, tile_points, blocks_per_cu))
+
+    def shape(self):
+        t, b = _c_int(), _c_int()
+        _check(lib().gn_batch_get_shape(self._h, ctypes.byref(t), ctypes.byref(b)))
+        return t.value, b.value
+
+    def _bufs(self, x, y, z, out):
+        torch = _torch()
+        xd, yd, zd = (to_device(a, torch.float64) for a in (x, y, z))
+        n = xd.numel()
+        if out is None:
+            out = (torch.empty(n, dtype=torch.float64, device="cuda"), torch.empty(n, dtype=torch.float64, device="cuda"),
+                   torch.empty(n, dtype=torch.uint8, device="cuda"))
+        return xd, yd, zd, n, out
+
+    def invert(self, segs, x, y, z, opts: Optional[InvertOpts] = None, out=None, stream=None):
+        segs = np.ascontiguousarray(segs, dtype=SEGMENT)
+        xd, yd, zd, n, (u, v, st) = self._bufs(x, y, z, out)
+        opts = opts if opts is not None else default_opts()
+        _check(lib().gn_batch_invert(self._h, segs.ctypes.data, len(segs), xd.data_ptr(), yd.data_ptr(), zd.data_ptr(),
+                                     n, ctypes.byref(opts), u.data_ptr(), v.data_ptr(), st.data_ptr(),
+                                     _c_void(stream_handle(stream))))
+        return u, v, st
+
+    def tune(self, segs, x, y, z, opts: Optional[InvertOpts] = None, cache_path: Optional[str] = None, out=None,
+             stream=None):
+        segs = np.ascontiguousarray(segs, dtype=SEGMENT)
+        xd, yd, zd, n, (u, v, st) = self._bufs(x, y, z, out)
+        opts = opts if opts is not None else default_opts()
+        _check(lib().gn_batch_tune(self._h, segs.ctypes.data, len(segs), xd.data_ptr(), yd.data_ptr(), zd.data_ptr(),
+                                   n, ctypes.byref(opts), u.data_ptr(), v.data_ptr(), st.data_ptr(),
+                                   cache_path.encode() if cache_path else None, _c_void(stream_handle(stream))))
+        return self.shape()
+
+    def graph(self, segs, x, y, z, u, v, status, opts: Optional[InvertOpts] = None, stream=None) -> Graph:
+        """Captures one batched inversion on these device tensors; stream must be a created
+        stream (torch.cuda.Stream)."""
+        segs = np.ascontiguousarray(segs, dtype=SEGMENT)
+        opts = opts if opts is not None else default_opts()
+        h = _c_void()
+        _check(lib().gn_batch_graph_create(self._h, segs.ctypes.data, len(segs), x.data_ptr(), y.data_ptr(),
+                                           z.data_ptr(), x.numel(), ctypes.byref(opts), u.data_ptr(), v.data_ptr(),
+                                           status.data_ptr(), _c_void(stream_handle(stream)), ctypes.byref(h)))
+        return Graph(h, (x, y, z, u, v, status))

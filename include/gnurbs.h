@@ -165,6 +165,48 @@ int gn_invert_iters(const gn_surface *s, const double *x, const double *y, const
                     size_t n, const gn_invert_opts *opts, double *u_out, double *v_out,
                     uint8_t *status, uint8_t *iters_fp32, uint8_t *iters_fp64, gn_stream stream);
 
+/* ---- batched inversion: many surfaces in one persistent launch ---------------------------- */
+
+/* A run of points that belong to one object of a batch: points start .. start+count-1 of the
+ * query arrays are inverted on object `obj` (index into the batch's surface list). */
+typedef struct {
+    int32_t obj;
+    int32_t count;
+    int64_t start;
+} gn_segment;
+
+typedef struct gn_graph gn_graph; /* an instantiated hipGraph of one batched inversion */
+
+/* Packs copies of the surfaces' span tables, knots and seed trees into one device allocation.
+ * Every surface needs its tree (gn_build_tree).  The surfaces may be destroyed afterwards. */
+int gn_batch_create(gn_surface *const *surfaces, int count, gn_stream stream, gn_batch **out);
+int gn_batch_destroy(gn_batch *b);
+size_t gn_batch_device_bytes(const gn_batch *b);
+/* Tile shape: points per tile (work unit claimed by one wave) and a cap on resident workgroups
+ * per CU (0 = occupancy limit). */
+int gn_batch_set_shape(gn_batch *b, int tile_points, int blocks_per_cu);
+int gn_batch_get_shape(const gn_batch *b, int *tile_points, int *blocks_per_cu);
+/* Inverts the points of all segments (device arrays x, y, z, u, v, status of length n) with one
+ * persistent launch per precision phase; seeds: GN_SEED_GREEDY_LEAF, GN_SEED_NEAREST_LEAF (each
+ * point on its own surface's tree) or GN_SEED_GIVEN. */
+int gn_batch_invert(gn_batch *b, const gn_segment *segs, int nseg, const double *x,
+                    const double *y, const double *z, size_t n, const gn_invert_opts *opts,
+                    double *u, double *v, uint8_t *status, gn_stream stream);
+/* Autotuner: times the candidate tile shapes on this work (outputs are overwritten) and keeps
+ * the fastest; with cache_path, a shape stored for the same work signature is reused and a newly
+ * tuned one is appended to the file. */
+int gn_batch_tune(gn_batch *b, const gn_segment *segs, int nseg, const double *x, const double *y,
+                  const double *z, size_t n, const gn_invert_opts *opts, double *u, double *v,
+                  uint8_t *status, const char *cache_path, gn_stream stream);
+/* Captures gn_batch_invert on these buffers into a hipGraph (stream must not be NULL); launching
+ * it re-runs the inversion on whatever the buffers hold.  One launch of a graph at a time. */
+int gn_batch_graph_create(gn_batch *b, const gn_segment *segs, int nseg, const double *x,
+                          const double *y, const double *z, size_t n, const gn_invert_opts *opts,
+                          double *u, double *v, uint8_t *status, gn_stream stream,
+                          gn_graph **out);
+int gn_graph_launch(gn_graph *g, gn_stream stream);
+int gn_graph_destroy(gn_graph *g);
+
 #ifdef __cplusplus
 }
 #endif
