@@ -91,6 +91,80 @@ def bench_eval(G, synth, torch, steps):
     return res
 
 
+def bench_batch(G, synth, torch, steps, count=1024):
+    """Uneven batch: `count` rational surfaces, degrees U{2..5} per direction, nets 4x4..48x48,
+    jittered clamped knots, weights U[0.5, 2]; points per surface from a power law on
+    [256, 65536]; queries offset N(0, 1e-2) along the normal.  One persistent launch per
+    precision phase over all surfaces, tile shape from the autotuner, captured in a hipGraph."""
+    import numpy as np
+
+    defs, counts = synth.uneven_batch(count, seed=1003)
+    gen = torch.Generator(device="cuda").manual_seed(1003)
+    surfs, xs, u0s, v0s = [], [], [], []
+    for sd, c in zip(defs, counts):
+        s = G.Surface(sd.p, sd.q, sd.ku, sd.kv, sd.cp, sd.w)
+        s.BuildTree(3)
+        surfs.append(s)
+        u0 = torch.rand(int(c), dtype=torch.float64, device="cuda", generator=gen)
+        v0 = torch.rand(int(c), dtype=torch.float64, device="cuda", generator=gen)
+        ev = s.Evaluate(u0, v0, 1)
+        nrm = torch.linalg.cross(ev[3:6].T, ev[6:9].T).T
+        nrm = nrm / torch.linalg.norm(nrm, dim=0, keepdim=True)
+        delta = torch.randn(int(c), dtype=torch.float64, device="cuda", generator=gen) * 1e-2
+        xs.append(ev[0:3] + delta * nrm)
+        u0s.append(u0)
+        v0s.append(v0)
+    P = torch.cat(xs, dim=1)
+    x, y, z = P[0].contiguous(), P[1].contiguous(), P[2].contiguous()
+    u0, v0 = torch.cat(u0s), torch.cat(v0s)
+    del xs, u0s, v0s, P
+    n = x.numel()
+    b = G.Batch(surfs)
+    segs = G.segments(np.arange(count), counts)
+    u = torch.empty(n, dtype=torch.float64, device="cuda")
+    v = torch.empty_like(u)
+    st = torch.empty(n, dtype=torch.uint8, device="cuda")
+    mixed = dict(precision=G.GN_MIXED, newton=G.NEWTON_FP32 | G.NEWTON_FP64, fp32_iters=20, fp64_tail=3,
+                 tol_step32=1e-4, tol_step=1e-10, tol_quad=1e-6)
+    shape = b.tune(segs, x, y, z, G.default_opts(**mixed), out=(u, v, st))
+    stream = torch.cuda.Stream()
+    res = {"workload": f"{count} surfaces (degrees 2..5, nets 4..48, jittered knots), {n} points "
+                       "(power law 256..65536 per surface), offsets N(0,1e-2)",
+           "points": n, "tile_points": shape[0], "blocks_per_cu": shape[1],
+           "batch_device_bytes": b.device_bytes()}
+
+    def ok():
+        return int(((u - u0).abs().maximum((v - v0).abs()) <= 1e-9).sum().item())
+
+    def timed(fn):
+        with torch.cuda.stream(stream):
+            fn()
+            stream.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            for _ in range(steps):
+                fn()
+            e1.record(stream)
+            stream.synchronize()
+        return e0.elapsed_time(e1) / steps
+
+    for name, opts in (("mixed", mixed), ("plain_fp64", dict(precision=G.GN_FP64, max_iter=20, tol_step=1e-10))):
+        ms = timed(lambda: b.invert(segs, x, y, z, G.default_opts(**opts), out=(u, v, st), stream=stream))
+        k = ok()
+        res[name] = {"ms": ms, "ok_frac": k / n, "points_per_s": k / (ms / 1e3)}
+    g = b.graph(segs, x, y, z, u, v, st, G.default_opts(**mixed), stream=stream)
+    ms = timed(lambda: g.launch(stream=stream))
+    k = ok()
+    res["mixed_graph"] = {"ms": ms, "ok_frac": k / n, "points_per_s": k / (ms / 1e3)}
+    res["speedup_vs_plain_fp64"] = res["mixed_graph"]["points_per_s"] / res["plain_fp64"]["points_per_s"]
+    g.close()
+    b.close()
+    for s in surfs:
+        s.close()
+    torch.cuda.empty_cache()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -99,6 +173,7 @@ def main():
     ap.add_argument("--points", type=int, default=1 << 20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-eval", action="store_true", help="skip the evaluation-at-scale measurement")
+    ap.add_argument("--no-batch", action="store_true", help="skip the uneven-batch measurement")
     ap.add_argument("--cpu-points", type=int, default=1 << 18)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 512 MiB read + 512 MiB write scale kernel (reference for HBM "
@@ -260,6 +335,7 @@ def main():
                "ok_frac": cok / nc}
 
     eval_res = None if args.no_eval else bench_eval(G, synth, torch, max(3, args.steps))
+    batch_res = None if args.no_batch else bench_batch(G, synth, torch, max(3, args.steps))
 
     mixed = res["mixed"]
     out = {
@@ -287,6 +363,7 @@ def main():
         "speedup_vs_plain_fp64": mixed["points_per_s"] / max(res["plain_fp64"]["points_per_s"], 1e-30),
         "seed_ms": seed_ms,
         "eval_16M": eval_res,
+        "uneven_batch": batch_res,
     }
     print(json.dumps(out))
 
