@@ -202,7 +202,7 @@ int gn_surface_from_nu(const char *path, gn_stream stream, gn_surface **out) {
 
 int gn_surface_destroy(gn_surface *s) {
     if (!s) return GN_OK;
-    hipDeviceSynchronize();
+    (void)hipDeviceSynchronize();
     free_surface(s);
     return GN_OK;
 }

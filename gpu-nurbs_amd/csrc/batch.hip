@@ -10,7 +10,9 @@
 // without divergence.  When the next tile is of another class, the wave drains its current points
 // before switching.  Seeds are found inline when a lane takes a point (greedy leaf of the
 // point's own surface).  The FP32 bulk and the FP64 closing pass are two launches of the same
-// kernel template, as in the single-surface mixed mode.
+// kernel template, as in the single-surface mixed mode.  Point arrays are addressed through a
+// request table (GnReq): a tile names its request, so one launch can serve the points of many
+// independent requests (the serving loop) or of one call (a one-entry table).
 #include "gn_search.hpp"
 #include "invert_common.hpp"
 
@@ -20,10 +22,7 @@ struct BatchArgs {
     const GnBatchObj *objs;
     const GnTile *tiles;
     int ntiles;
-    const double *x, *y, *z;
-    double *u, *v;
-    uint8_t *status;
-    uint8_t *it32, *it64;
+    const GnReq *reqs;
     gn_invert_opts o;
 };
 
@@ -62,13 +61,13 @@ __global__ __launch_bounds__(kInvBlock) void k_batch(BatchArgs a, unsigned long 
     unsigned long long *counter = counters + (size_t)slice * kCounterStride;
     // wave-uniform tile state
     long long q_lo = 0, q_hi = 0;
-    int cur_obj = -1, cur_cls = -1;
+    int cur_obj = -1, cur_cls = -1, cur_req = 0;
     bool exhausted = false, pend = false;
     GnTile pt{0, 0, 0};
     int pcls = -1;
     // lane state
     long long idx = -1;
-    int obj = 0, it = 0, lim = max_iter;
+    int obj = 0, rq = 0, it = 0, lim = max_iter;
     GnSurfDev d{};
     double cu = 0, cv = 0, su = 0, sv = 0;
     T Px = 0, Py = 0, Pz = 0;
@@ -94,6 +93,7 @@ __global__ __launch_bounds__(kInvBlock) void k_batch(BatchArgs a, unsigned long 
                 if (pcls != cur_cls && idle != all) break;  // drain the current class first
                 cur_obj = pt.obj;
                 cur_cls = pcls;
+                cur_req = pt.req;
                 q_lo = pt.start;
                 q_hi = pt.start + pt.count;
                 pend = false;
@@ -105,21 +105,23 @@ __global__ __launch_bounds__(kInvBlock) void k_batch(BatchArgs a, unsigned long 
                 if (rank < avail) {
                     idx = q_lo + (long long)rank;
                     obj = cur_obj;
+                    rq = cur_req;
                     d = a.objs[obj].s;
-                    const double P[3] = {a.x[idx], a.y[idx], a.z[idx]};
+                    const GnReq &R = a.reqs[cur_req];
+                    const double P[3] = {R.x[idx], R.y[idx], R.z[idx]};
                     Px = T(P[0]);
                     Py = T(P[1]);
                     Pz = T(P[2]);
                     it = 0;
                     lim = max_iter;
                     if (MODE == kClose) {
-                        cu = a.u[idx];
-                        cv = a.v[idx];
-                        const int sb = a.status[idx];
+                        cu = R.u[idx];
+                        cv = R.v[idx];
+                        const int sb = R.status[idx];
                         if (sb == GN_PT_SINGULAR || sb == GN_PT_NONFINITE) lim = restart_iter;
                     } else if (a.o.seed_mode == GN_SEED_GIVEN) {
-                        cu = a.u[idx];
-                        cv = a.v[idx];
+                        cu = R.u[idx];
+                        cv = R.v[idx];
                     } else {
                         const GnSearchDev &sd = a.objs[obj].sd;
                         const uint32_t c = a.o.seed_mode == GN_SEED_NEAREST_LEAF
@@ -152,14 +154,15 @@ __global__ __launch_bounds__(kInvBlock) void k_batch(BatchArgs a, unsigned long 
                     cu = su;
                     cv = sv;
                 }
-                a.u[idx] = cu;
-                a.v[idx] = cv;
-                a.status[idx] = (uint8_t)st;
+                const GnReq &R = a.reqs[rq];
+                R.u[idx] = cu;
+                R.v[idx] = cv;
+                R.status[idx] = (uint8_t)st;
                 if (MODE == kBulk) {
-                    if (a.it32) a.it32[idx] = sat8(it);
+                    if (R.it32) R.it32[idx] = sat8(it);
                 } else {
-                    if (MODE == kPlain && a.it32) a.it32[idx] = 0;
-                    if (a.it64) a.it64[idx] = sat8(it);
+                    if (MODE == kPlain && R.it32) R.it32[idx] = 0;
+                    if (R.it64) R.it64[idx] = sat8(it);
                 }
                 idx = -1;
             }
@@ -197,12 +200,10 @@ static int launch_one(const BatchArgs &a, size_t n, int bpc_cap, int max_iter, i
     return GN_OK;
 }
 
-int launch_batch_invert(const GnBatchObj *objs, const GnTile *tiles, int ntiles, const double *x,
-                        const double *y, const double *z, size_t n, const gn_invert_opts &o,
-                        double *u, double *v, uint8_t *status, uint8_t *it32, uint8_t *it64,
-                        int bpc_cap, hipStream_t st) {
+int launch_batch_invert(const GnBatchObj *objs, const GnTile *tiles, int ntiles, const GnReq *reqs,
+                        size_t n, const gn_invert_opts &o, int bpc_cap, hipStream_t st) {
     if (n == 0 || ntiles == 0) return GN_OK;
-    BatchArgs a{objs, tiles, ntiles, x, y, z, u, v, status, it32, it64, o};
+    BatchArgs a{objs, tiles, ntiles, reqs, o};
     const bool n32 = o.newton & GN_NEWTON_FP32, n64 = o.newton & GN_NEWTON_FP64;
     int rc;
     if (o.precision == GN_MIXED) {

@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstring>
 #include <fstream>
 #include <mutex>
 #include <sstream>
@@ -28,7 +29,7 @@ struct gn_batch {
 struct gn_graph {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    GnTile *d_tiles = nullptr;
+    char *d_buf = nullptr;  // tile list + request table of the captured call
 };
 
 namespace {
@@ -56,7 +57,7 @@ std::vector<GnTile> build_tiles(const gn_batch *b, const gn_segment *segs, int n
     for (int i = 0; i < nseg; i++) {
         for (int64_t off = 0; off < segs[i].count; off += tile) {
             const int64_t c = std::min<int64_t>(tile, segs[i].count - off);
-            t.push_back(GnTile{segs[i].obj, (int32_t)c, segs[i].start + off});
+            t.push_back(GnTile{segs[i].obj, (int32_t)c, segs[i].start + off, 0, 0});
         }
     }
     std::stable_sort(t.begin(), t.end(), [b](const GnTile &x, const GnTile &y) {
@@ -65,36 +66,55 @@ std::vector<GnTile> build_tiles(const gn_batch *b, const gn_segment *segs, int n
     return t;
 }
 
-void free_tiles_host(void *p) { delete static_cast<std::vector<GnTile> *>(p); }
+// Device image of one batched call: the tile list, then (256-byte aligned) a one-entry request
+// table holding the call's point arrays.
+struct CallImage {
+    std::vector<char> bytes;
+    size_t ntiles = 0, req_off = 0;
+};
 
-// Tiles go to a stream-ordered device buffer; the host copy is released by a host callback in
-// stream order, after the upload has consumed it.
+CallImage call_image(const std::vector<GnTile> &t, const GnReq &r) {
+    CallImage c;
+    c.ntiles = t.size();
+    c.req_off = align256(sizeof(GnTile) * t.size());
+    c.bytes.resize(c.req_off + sizeof(GnReq));
+    if (!t.empty()) std::memcpy(c.bytes.data(), t.data(), sizeof(GnTile) * t.size());
+    std::memcpy(c.bytes.data() + c.req_off, &r, sizeof(GnReq));
+    return c;
+}
+
+void free_image_host(void *p) { delete static_cast<CallImage *>(p); }
+
+// The call image goes to a stream-ordered device buffer; the host copy is released by a host
+// callback in stream order, after the upload has consumed it.
 int run_batch(gn_batch *b, const gn_segment *segs, int nseg, const double *x, const double *y,
               const double *z, size_t n, const gn_invert_opts &o, double *u, double *v,
               uint8_t *status, uint8_t *it32, uint8_t *it64, int tile, int bpc, hipStream_t st) {
-    auto *tiles = new std::vector<GnTile>(build_tiles(b, segs, nseg, tile));
-    if (tiles->empty()) {
-        delete tiles;
+    const GnReq req{x, y, z, u, v, status, it32, it64};
+    auto *img = new CallImage(call_image(build_tiles(b, segs, nseg, tile), req));
+    if (img->ntiles == 0) {
+        delete img;
         return GN_OK;
     }
-    const size_t nt = tiles->size();
-    GnTile *d_tiles = nullptr;
-    hipError_t e = hipMallocAsync((void **)&d_tiles, sizeof(GnTile) * nt, st);
+    char *d = nullptr;
+    hipError_t e = hipMallocAsync((void **)&d, img->bytes.size(), st);
     if (e != hipSuccess) {
-        delete tiles;
+        delete img;
         return gn::hip_fail(e, "hipMallocAsync(tiles)");
     }
-    e = hipMemcpyAsync(d_tiles, tiles->data(), sizeof(GnTile) * nt, hipMemcpyHostToDevice, st);
-    if (e == hipSuccess) e = hipLaunchHostFunc(st, free_tiles_host, tiles);
+    const int nt = (int)img->ntiles;
+    const size_t req_off = img->req_off;
+    e = hipMemcpyAsync(d, img->bytes.data(), img->bytes.size(), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipLaunchHostFunc(st, free_image_host, img);
     if (e != hipSuccess) {
         (void)hipStreamSynchronize(st);
-        delete tiles;
-        (void)hipFreeAsync(d_tiles, st);
+        delete img;
+        (void)hipFreeAsync(d, st);
         return gn::hip_fail(e, "tile upload");
     }
-    int rc = gn::launch_batch_invert(b->d_objs, d_tiles, (int)nt, x, y, z, n, o, u, v, status,
-                                     it32, it64, bpc, st);
-    GN_HIP(hipFreeAsync(d_tiles, st));
+    int rc = gn::launch_batch_invert(b->d_objs, (const GnTile *)d, nt, (const GnReq *)(d + req_off),
+                                     n, o, bpc, st);
+    GN_HIP(hipFreeAsync(d, st));
     return rc;
 }
 
@@ -121,6 +141,21 @@ std::string signature(const gn_batch *b, const gn_segment *segs, int nseg, size_
 }
 
 }  // namespace
+
+namespace gn {
+// Read-only view of a batch for the inversion server.
+int batch_view(const gn_batch *b, const GnBatchObj **objs, int *count, std::vector<int> *cls,
+               int *tile, int *bpc, size_t *bytes) {
+    if (!b) return fail(GN_EINVAL, "null batch");
+    *objs = b->d_objs;
+    *count = b->count;
+    *cls = b->cls;
+    *tile = b->tile;
+    *bpc = b->bpc;
+    *bytes = b->bytes;
+    return GN_OK;
+}
+}  // namespace gn
 
 extern "C" {
 
@@ -339,33 +374,31 @@ int gn_batch_graph_create(gn_batch *b, const gn_segment *segs, int nseg, const d
     if ((rc = check_segments(b, segs, nseg, n))) return rc;
     if (n && (!x || !y || !z || !u || !v || !status)) return fail(GN_EINVAL, "null buffer");
     if ((rc = gn::ensure_runtime())) return rc;
-    std::vector<GnTile> tiles = build_tiles(b, segs, nseg, b->tile);
+    const GnReq req{x, y, z, u, v, status, nullptr, nullptr};
+    const CallImage img = call_image(build_tiles(b, segs, nseg, b->tile), req);
     gn_graph *g = new gn_graph();
     hipStream_t st = (hipStream_t)stream;
-    if (!tiles.empty()) {
-        hipError_t e = hipMalloc((void **)&g->d_tiles, sizeof(GnTile) * tiles.size());
-        if (e == hipSuccess)
-            e = hipMemcpy(g->d_tiles, tiles.data(), sizeof(GnTile) * tiles.size(),
-                          hipMemcpyHostToDevice);
-        if (e != hipSuccess) {
-            (void)hipFree(g->d_tiles);
-            delete g;
-            return gn::hip_fail(e, "graph tile list");
-        }
-    }
-    hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+    hipError_t e = hipMalloc((void **)&g->d_buf, img.bytes.size());
+    if (e == hipSuccess)
+        e = hipMemcpy(g->d_buf, img.bytes.data(), img.bytes.size(), hipMemcpyHostToDevice);
     if (e != hipSuccess) {
-        (void)hipFree(g->d_tiles);
+        (void)hipFree(g->d_buf);
+        delete g;
+        return gn::hip_fail(e, "graph tile list");
+    }
+    e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) {
+        (void)hipFree(g->d_buf);
         delete g;
         return gn::hip_fail(e, "hipStreamBeginCapture");
     }
-    rc = gn::launch_batch_invert(b->d_objs, g->d_tiles, (int)tiles.size(), x, y, z, n, o, u, v,
-                                 status, nullptr, nullptr, b->bpc, st);
+    rc = gn::launch_batch_invert(b->d_objs, (const GnTile *)g->d_buf, (int)img.ntiles,
+                                 (const GnReq *)(g->d_buf + img.req_off), n, o, b->bpc, st);
     e = hipStreamEndCapture(st, &g->graph);
     if (!rc && e == hipSuccess) e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
     if (rc || e != hipSuccess) {
         if (g->graph) (void)hipGraphDestroy(g->graph);
-        (void)hipFree(g->d_tiles);
+        (void)hipFree(g->d_buf);
         delete g;
         return rc ? rc : gn::hip_fail(e, "graph capture / instantiate");
     }
@@ -384,7 +417,7 @@ int gn_graph_destroy(gn_graph *g) {
     (void)hipDeviceSynchronize();
     if (g->exec) (void)hipGraphExecDestroy(g->exec);
     if (g->graph) (void)hipGraphDestroy(g->graph);
-    (void)hipFree(g->d_tiles);
+    (void)hipFree(g->d_buf);
     delete g;
     return GN_OK;
 }

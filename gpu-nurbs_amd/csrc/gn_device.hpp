@@ -231,6 +231,48 @@ __device__ __forceinline__ void eval_point_cached(const GnSurfDev &d, const doub
     quotient<ORDER, T>(h, T(sc.ihu), T(sc.ihv), o);
 }
 
+// M[r*k + a] = coefficient of t^r in N_{span-deg+a, deg}(k[span] + t*(k[span+1]-k[span])).
+// Same matrix as the Nrc recursion of src/NURBS/Evaluate.cu:90-103; 0/0 is taken as 0.
+__device__ inline void basis_matrix(int deg, const double *__restrict__ kn, int span, double *M) {
+    double N[GN_MAXK][GN_MAXK], T[GN_MAXK][GN_MAXK];
+    for (int a = 0; a < GN_MAXK; a++)
+        for (int e = 0; e < GN_MAXK; e++) N[a][e] = 0.0;
+    N[0][0] = 1.0;
+    const double h = kn[span + 1] - kn[span];
+    for (int d = 1; d <= deg; d++) {
+        for (int a = 0; a <= d; a++)
+            for (int e = 0; e <= d; e++) T[a][e] = 0.0;
+        for (int a = 0; a <= d; a++) {
+            const int i = span - d + a;
+            if (a >= 1) {
+                double den = kn[i + d] - kn[i];
+                if (den != 0.0) {
+                    double c0 = (kn[span] - kn[i]) / den, c1 = h / den;
+                    for (int e = 0; e < d; e++) {
+                        T[a][e] += c0 * N[a - 1][e];
+                        T[a][e + 1] += c1 * N[a - 1][e];
+                    }
+                }
+            }
+            if (a <= d - 1) {
+                double den = kn[i + d + 1] - kn[i + 1];
+                if (den != 0.0) {
+                    double c0 = (kn[i + d + 1] - kn[span]) / den, c1 = -h / den;
+                    for (int e = 0; e < d; e++) {
+                        T[a][e] += c0 * N[a][e];
+                        T[a][e + 1] += c1 * N[a][e];
+                    }
+                }
+            }
+        }
+        for (int a = 0; a <= d; a++)
+            for (int e = 0; e <= d; e++) N[a][e] = T[a][e];
+    }
+    const int k = deg + 1;
+    for (int r = 0; r < k; r++)
+        for (int a = 0; a < k; a++) M[r * k + a] = N[a][r];
+}
+
 // Dispatch helper: calls F.template operator()<K1, K2, EXACT>() for the degree pair of a
 // surface.  Degrees 2..5 in each direction get exact instantiations; everything else uses the
 // padded <8, 8> form with runtime sizes.

@@ -50,11 +50,31 @@ struct GnBatchObj {
     int pad_;
 };
 
-// A tile of work: `count` points of object `obj` starting at point `start`.
+// Device view of one curve: span slot m - p holds patch[r][c] = sum_a M[r][a] P_w[m-p+a][c].
+struct GnCurveDev {
+    int p, n, ns;          // degree, control points, span slots (n - p)
+    const double *k, *ih;  // knots, 1/h per slot
+    const double *tab64;   // [slot][r][4]
+    const float *tab32;
+    const float *samp;     // [ns * 8][3] seed sample points (FP32)
+    const double *samp_t;  // their parameters
+};
+
+// The point arrays of one request (or of one whole batched call): inputs x, y, z and outputs
+// u, v (curves: t in u, v unused), status and optional iteration counts, all indexed from 0.
+struct GnReq {
+    const double *x, *y, *z;
+    double *u, *v;
+    uint8_t *status, *it32, *it64;
+};
+
+// A tile of work: `count` points of object `obj`, points start .. start+count-1 of request `req`.
 struct GnTile {
     int32_t obj;
     int32_t count;
     int64_t start;
+    int32_t req;
+    int32_t pad_;
 };
 
 // ---------------------------------------------------------------------------------------------
@@ -110,10 +130,28 @@ struct gn_surface {
     }
 };
 
+// host-side curve object
+struct gn_curve {
+    int p, n;
+    std::vector<double> k, xyz, w;
+    double *d_k = nullptr, *d_ih = nullptr, *d_xyz = nullptr, *d_w = nullptr, *d_tab64 = nullptr,
+           *d_samp_t = nullptr;
+    float *d_tab32 = nullptr, *d_samp = nullptr;
+    size_t bytes = 0;
+    GnCurveDev dev() const {
+        GnCurveDev c;
+        c.p = p; c.n = n; c.ns = n - p;
+        c.k = d_k; c.ih = d_ih; c.tab64 = d_tab64; c.tab32 = d_tab32;
+        c.samp = d_samp; c.samp_t = d_samp_t;
+        return c;
+    }
+};
+
 // ---------------------------------------------------------------------------------------------
 // kernel launchers (defined in the .hip files)
 // ---------------------------------------------------------------------------------------------
 namespace gn {
+constexpr int kCurveSamples = 8;  // curve seed samples per span
 int launch_span_table(const gn_surface *s, const double *d_xyz, const double *d_w,
                       hipStream_t st);
 int launch_find_span(const GnSurfDev &d, int dir, const double *u, size_t n, int32_t *span,
@@ -133,10 +171,12 @@ int launch_search_build(const GnSurfDev &d, int levels, const GnTreeNode *tree, 
 int launch_seed(const GnSearchDev &sd, int mode, const double *x, const double *y,
                 const double *z, size_t n, int32_t *leaf, double *su, double *sv,
                 hipStream_t st);
-int launch_batch_invert(const GnBatchObj *objs, const GnTile *tiles, int ntiles, const double *x,
-                        const double *y, const double *z, size_t n, const gn_invert_opts &o,
-                        double *u, double *v, uint8_t *status, uint8_t *it32, uint8_t *it64,
-                        int bpc_cap, hipStream_t st);
+// n: total points of the tiles (sizes the persistent grid)
+int launch_batch_invert(const GnBatchObj *objs, const GnTile *tiles, int ntiles, const GnReq *reqs,
+                        size_t n, const gn_invert_opts &o, int bpc_cap, hipStream_t st);
+// tiles of curve objects (obj indexes `curves`); max_samples: largest seed sample count
+int launch_curve_batch(const GnCurveDev *curves, const GnTile *tiles, int ntiles, const GnReq *reqs,
+                       int max_samples, const gn_invert_opts &o, hipStream_t st);
 int ensure_runtime();  // allocates the per-device claim counters (before any graph capture)
 int launch_invert(const GnSurfDev &d, const GnSearchDev &sd, const GnTreeNode *tree,
                   const uint8_t *valid, int tree_levels, const double *x, const double *y,

@@ -1,0 +1,479 @@
+// server.cpp -- continuous-batching inversion server (gn_server_*).
+//
+// One worker thread per server.  Submitted requests wait in arrival order until their result
+// buffers fit the HBM arena (admission); every admitted request that is not yet running joins
+// the next step.  A step is one upload of its request table and tile lists (pinned staging ->
+// device, one copy) followed by one batched launch per object kind on the step slot's stream and
+// an event; up to `streams` steps are in flight at once, so requests arriving while a step runs
+// are gathered into the next one as soon as a slot frees.  Completion is found by polling the
+// slots' events.  Every point is computed by the same per-point iteration whatever else shares
+// its step, so a request's results are the same alone and in a mix.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include <time.h>
+
+#include "arena.hpp"
+#include "gn_internal.hpp"
+
+using gn::fail;
+
+namespace {
+
+constexpr int kMaxStepReqs = 2048;     // requests per step
+constexpr int kMaxStepTiles = 32768;   // tiles per step (surfaces + curves)
+constexpr int kCurveTile = 1024;       // points per curve tile (one workgroup)
+
+enum ReqState { kWaiting, kAdmitted, kRunning, kDone, kFailed };
+
+struct Req {
+    int64_t id = 0;
+    int obj = 0;
+    bool curve = false;
+    size_t n = 0;
+    const double *x = nullptr, *y = nullptr, *z = nullptr;
+    size_t off = 0, bytes = 0;  // arena range
+    double *u = nullptr, *v = nullptr;
+    uint8_t *status = nullptr;
+    int64_t submit_ns = 0, admit_ns = 0, launch_ns = 0, done_ns = 0;
+    ReqState state = kWaiting;
+};
+
+size_t a256(size_t b) { return gn::Arena::round(b); }
+
+size_t result_bytes(bool curve, size_t n) {
+    return curve ? a256(8 * n) + a256(n) : 2 * a256(8 * n) + a256(n);
+}
+
+struct Slot {
+    hipStream_t st = nullptr;
+    hipEvent_t ev = nullptr;
+    char *h_stage = nullptr;  // pinned
+    char *d_stage = nullptr;
+    std::vector<Req *> reqs;
+    bool busy = false;
+};
+
+constexpr size_t kStageReqs = sizeof(GnReq) * kMaxStepReqs;
+constexpr size_t kStageBytes = kStageReqs + sizeof(GnTile) * kMaxStepTiles + 256;
+
+}  // namespace
+
+int64_t gn_now_ns_impl() {
+    timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return (int64_t)ts.tv_sec * 1000000000ll + ts.tv_nsec;
+}
+
+struct gn_server {
+    gn_batch *batch = nullptr;
+    int nsurf = 0, ncurve = 0;
+    std::vector<int> cls;          // degree class per surface
+    GnBatchObj *d_objs = nullptr;  // the batch's object table
+    int tile = 1024, bpc = 0;
+    GnCurveDev *d_curves = nullptr;
+    int max_samples = 0;
+    gn_invert_opts o{};
+    int device = 0;
+    size_t budget = 0, resident = 0;
+    char *d_arena = nullptr;
+    gn::Arena arena;
+    std::vector<Slot> slots;
+
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<Req *> waiting, admitted;
+    std::unordered_map<int64_t, Req *> reqs;
+    int64_t next_id = 1;
+    bool stop = false, dirty = false;
+    gn_server_stats stats{};
+    std::thread worker;
+
+    void admit_locked();
+    void run();
+    int launch_step(Slot &sl);
+    void finish(Slot &sl, bool ok);
+};
+
+// Helpers exported by batch_api.cpp for the server (batch internals).
+namespace gn {
+int batch_view(const gn_batch *b, const GnBatchObj **objs, int *count, std::vector<int> *cls,
+               int *tile, int *bpc, size_t *bytes);
+}
+
+void gn_server::admit_locked() {
+    while (!waiting.empty()) {
+        Req *r = waiting.front();
+        size_t off = 0;
+        if (!arena.alloc(r->bytes, &off)) break;  // arrival order: wait for releases
+        r->off = off;
+        char *base = d_arena + off;
+        r->u = (double *)base;
+        if (r->curve) {
+            r->v = nullptr;
+            r->status = (uint8_t *)(base + a256(8 * r->n));
+        } else {
+            r->v = (double *)(base + a256(8 * r->n));
+            r->status = (uint8_t *)(base + 2 * a256(8 * r->n));
+        }
+        r->state = kAdmitted;
+        r->admit_ns = gn_now_ns_impl();
+        waiting.pop_front();
+        admitted.push_back(r);
+    }
+    if (arena.peak() > stats.arena_peak) stats.arena_peak = arena.peak();
+}
+
+// Builds the step image in the slot's pinned staging and launches it.  Called without the lock;
+// sl.reqs were taken from `admitted` by the worker.
+int gn_server::launch_step(Slot &sl) {
+    const int nreq = (int)sl.reqs.size();
+    size_t spts = 0, cpts = 0;
+    for (Req *r : sl.reqs) (r->curve ? cpts : spts) += r->n;
+    // tile size that keeps the step within kMaxStepTiles (sum of ceil(n/T) <= pts/T + nreq)
+    const size_t room = (size_t)(kMaxStepTiles - 2 * nreq);
+    size_t ts = (size_t)tile;
+    if (spts / ts > room / 2) ts = (spts + room / 2 - 1) / (room / 2);
+    size_t tc = kCurveTile;
+    if (cpts / tc > room / 2) tc = (cpts + room / 2 - 1) / (room / 2);
+    GnReq *rq = (GnReq *)sl.h_stage;
+    std::vector<GnTile> st_tiles, ct_tiles;
+    for (int k = 0; k < nreq; k++) {
+        Req *r = sl.reqs[k];
+        rq[k] = GnReq{r->x, r->y, r->z, r->u, r->v, r->status, nullptr, nullptr};
+        const size_t T = r->curve ? tc : ts;
+        for (size_t off = 0; off < r->n; off += T) {
+            const size_t c = std::min(T, r->n - off);
+            GnTile t{r->curve ? r->obj - nsurf : r->obj, (int32_t)c, (int64_t)off, k, 0};
+            (r->curve ? ct_tiles : st_tiles).push_back(t);
+        }
+    }
+    std::stable_sort(st_tiles.begin(), st_tiles.end(),
+                     [this](const GnTile &a, const GnTile &b) { return cls[a.obj] < cls[b.obj]; });
+    const size_t ns = st_tiles.size(), nc = ct_tiles.size();
+    if (ns + nc > (size_t)kMaxStepTiles) return fail(GN_EDEVICE, "step tile list overflow");
+    GnTile *tl = (GnTile *)(sl.h_stage + kStageReqs);
+    if (ns) std::memcpy(tl, st_tiles.data(), sizeof(GnTile) * ns);
+    if (nc) std::memcpy(tl + ns, ct_tiles.data(), sizeof(GnTile) * nc);
+    const size_t bytes = kStageReqs + sizeof(GnTile) * (ns + nc);
+    GN_HIP(hipMemcpyAsync(sl.d_stage, sl.h_stage, bytes, hipMemcpyHostToDevice, sl.st));
+    const GnReq *d_rq = (const GnReq *)sl.d_stage;
+    const GnTile *d_tl = (const GnTile *)(sl.d_stage + kStageReqs);
+    int rc = GN_OK;
+    if (ns) rc = gn::launch_batch_invert(d_objs, d_tl, (int)ns, d_rq, spts, o, bpc, sl.st);
+    if (!rc && nc) rc = gn::launch_curve_batch(d_curves, d_tl + ns, (int)nc, d_rq, max_samples, o, sl.st);
+    if (rc) return rc;
+    GN_HIP(hipEventRecord(sl.ev, sl.st));
+    return GN_OK;
+}
+
+void gn_server::finish(Slot &sl, bool ok) {
+    const int64_t now = gn_now_ns_impl();
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        for (Req *r : sl.reqs) {
+            r->done_ns = now;
+            r->state = ok ? kDone : kFailed;
+            if (ok) {
+                stats.completed++;
+                stats.points += (int64_t)r->n;
+            }
+        }
+        sl.reqs.clear();
+        sl.busy = false;
+    }
+    cv_done.notify_all();
+}
+
+void gn_server::run() {
+    (void)hipSetDevice(device);
+    while (true) {
+        bool progressed = false, any_busy = false;
+        for (Slot &sl : slots) {
+            if (!sl.busy) continue;
+            const hipError_t q = hipEventQuery(sl.ev);
+            if (q == hipErrorNotReady) {
+                any_busy = true;
+                continue;
+            }
+            finish(sl, q == hipSuccess);
+            progressed = true;
+        }
+        Slot *free_slot = nullptr;
+        for (Slot &sl : slots)
+            if (!sl.busy) {
+                free_slot = &sl;
+                break;
+            }
+        {
+            std::unique_lock<std::mutex> lk(mu);
+            dirty = false;
+            admit_locked();
+            if (stop) {
+                free_slot = nullptr;
+                bool busy = false;
+                for (Slot &sl : slots) busy |= sl.busy;
+                if (!busy) {  // nothing in flight: fail what never ran, then exit
+                    for (auto *dq : {&waiting, &admitted})
+                        for (Req *r : *dq) {
+                            r->state = kFailed;
+                            r->done_ns = gn_now_ns_impl();
+                        }
+                    waiting.clear();
+                    admitted.clear();
+                    lk.unlock();
+                    cv_done.notify_all();
+                    return;
+                }
+            } else if (free_slot && !admitted.empty()) {
+                const int64_t now = gn_now_ns_impl();
+                while (!admitted.empty() && (int)free_slot->reqs.size() < kMaxStepReqs) {
+                    Req *r = admitted.front();
+                    r->state = kRunning;
+                    r->launch_ns = now;
+                    free_slot->reqs.push_back(r);
+                    admitted.pop_front();
+                }
+                free_slot->busy = true;
+                stats.steps++;
+                stats.max_step_requests =
+                    std::max<int64_t>(stats.max_step_requests, (int64_t)free_slot->reqs.size());
+            } else {
+                free_slot = nullptr;
+            }
+            if (!free_slot && !progressed) {
+                if (!any_busy) {
+                    cv_work.wait_for(lk, std::chrono::milliseconds(2), [this] { return dirty || stop; });
+                    continue;
+                }
+            }
+        }
+        if (free_slot) {
+            if (launch_step(*free_slot) != GN_OK) {
+                (void)hipStreamSynchronize(free_slot->st);
+                finish(*free_slot, false);
+            }
+            continue;
+        }
+        if (!progressed && any_busy) std::this_thread::yield();
+    }
+}
+
+extern "C" {
+
+int64_t gn_now_ns(void) { return gn_now_ns_impl(); }
+
+int gn_server_create(gn_batch *batch, gn_curve *const *curves, int ncurves,
+                     const gn_invert_opts *opts, size_t hbm_budget, int streams,
+                     gn_server **out) {
+    if (!out) return fail(GN_EINVAL, "out is null");
+    *out = nullptr;
+    if (ncurves < 0 || (ncurves > 0 && !curves)) return fail(GN_EINVAL, "bad curve list");
+    if (!batch && ncurves == 0) return fail(GN_EINVAL, "a server needs objects");
+    if (streams < 1 || streams > 8) return fail(GN_EINVAL, "streams must be 1..8");
+    gn_invert_opts o;
+    if (opts) o = *opts;
+    else gn_invert_opts_default(&o);
+    if (o.precision != GN_FP64 && o.precision != GN_MIXED)
+        return fail(GN_EINVAL, "precision must be GN_FP64 or GN_MIXED");
+    if (o.seed_mode != GN_SEED_GREEDY_LEAF && o.seed_mode != GN_SEED_NEAREST_LEAF)
+        return fail(GN_EINVAL, "server seeds: GN_SEED_GREEDY_LEAF or GN_SEED_NEAREST_LEAF");
+    if (o.max_iter < 0 || o.fp32_iters < 0 || o.fp64_tail < 0 || o.max_iter > 10000 ||
+        o.fp32_iters > 10000 || o.fp64_tail > 10000)
+        return fail(GN_EINVAL, "iteration counts must be 0..10000");
+    for (int i = 0; i < ncurves; i++)
+        if (!curves[i]) return fail(GN_EINVAL, "null curve in the list");
+    int rc = gn::ensure_runtime();
+    if (rc) return rc;
+    auto *s = new gn_server();
+    s->o = o;
+    s->budget = hbm_budget;
+    GN_HIP(hipGetDevice(&s->device));
+    size_t batch_bytes = 0;
+    if (batch) {
+        const GnBatchObj *objs = nullptr;
+        if ((rc = gn::batch_view(batch, &objs, &s->nsurf, &s->cls, &s->tile, &s->bpc, &batch_bytes))) {
+            delete s;
+            return rc;
+        }
+        s->batch = batch;
+        s->d_objs = const_cast<GnBatchObj *>(objs);
+    }
+    s->ncurve = ncurves;
+    size_t curve_bytes = 0;
+    std::vector<GnCurveDev> cd(ncurves);
+    for (int i = 0; i < ncurves; i++) {
+        cd[i] = curves[i]->dev();
+        curve_bytes += curves[i]->bytes;
+        s->max_samples = std::max(s->max_samples, cd[i].ns * gn::kCurveSamples);
+    }
+    s->resident = batch_bytes + curve_bytes + sizeof(GnCurveDev) * ncurves +
+                  (size_t)streams * kStageBytes;
+    if (hbm_budget < s->resident + (1u << 20)) {
+        delete s;
+        return fail(GN_EBUDGET, "objects leave less than 1 MiB of the HBM budget for results");
+    }
+    const size_t arena = (hbm_budget - s->resident) & ~(size_t)255;
+    hipError_t e = hipMalloc((void **)&s->d_arena, arena);
+    if (e == hipSuccess && ncurves) {
+        e = hipMalloc((void **)&s->d_curves, sizeof(GnCurveDev) * ncurves);
+        if (e == hipSuccess)
+            e = hipMemcpy(s->d_curves, cd.data(), sizeof(GnCurveDev) * ncurves, hipMemcpyHostToDevice);
+    }
+    s->arena.reset(arena);
+    s->stats.arena_bytes = arena;
+    s->slots.resize(streams);
+    for (Slot &sl : s->slots) {
+        if (e == hipSuccess) e = hipStreamCreateWithFlags(&sl.st, hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&sl.ev, hipEventDisableTiming);
+        if (e == hipSuccess) e = hipHostMalloc((void **)&sl.h_stage, kStageBytes, hipHostMallocDefault);
+        if (e == hipSuccess) e = hipMalloc((void **)&sl.d_stage, kStageBytes);
+    }
+    if (e != hipSuccess) {
+        for (Slot &sl : s->slots) {
+            if (sl.st) (void)hipStreamDestroy(sl.st);
+            if (sl.ev) (void)hipEventDestroy(sl.ev);
+            if (sl.h_stage) (void)hipHostFree(sl.h_stage);
+            if (sl.d_stage) (void)hipFree(sl.d_stage);
+        }
+        (void)hipFree(s->d_arena);
+        (void)hipFree(s->d_curves);
+        delete s;
+        return gn::hip_fail(e, "server set-up");
+    }
+    s->worker = std::thread([s] { s->run(); });
+    *out = s;
+    return GN_OK;
+}
+
+int gn_server_submit(gn_server *s, int obj, const double *x, const double *y, const double *z,
+                     size_t n, int64_t *id) {
+    if (!s || !id) return fail(GN_EINVAL, "null argument");
+    if (obj < 0 || obj >= s->nsurf + s->ncurve) return fail(GN_ENOTFOUND, "unknown object id");
+    if (n && (!x || !y || !z)) return fail(GN_EINVAL, "null point array");
+    if (n > ((size_t)1 << 31)) return fail(GN_EINVAL, "at most 2^31 points per request");
+    auto *r = new Req();
+    r->obj = obj;
+    r->curve = obj >= s->nsurf;
+    r->n = n;
+    r->x = x;
+    r->y = y;
+    r->z = z;
+    r->bytes = result_bytes(r->curve, n);
+    if (r->bytes > s->arena.capacity()) {
+        delete r;
+        return fail(GN_EBUDGET, "request results exceed the server's HBM arena");
+    }
+    r->submit_ns = gn_now_ns_impl();
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (s->stop) {
+            delete r;
+            return fail(GN_ESTATE, "server is shutting down");
+        }
+        r->id = s->next_id++;
+        s->reqs[r->id] = r;
+        s->waiting.push_back(r);
+        s->stats.submitted++;
+        s->dirty = true;
+        *id = r->id;
+    }
+    s->cv_work.notify_one();
+    return GN_OK;
+}
+
+static int report(const Req *r, gn_request_info *info) {
+    if (r->state == kFailed) return fail(GN_EDEVICE, "request failed (step launch error or server shut down)");
+    if (r->state != kDone) return GN_EAGAIN;
+    if (info) {
+        info->submit_ns = r->submit_ns;
+        info->admit_ns = r->admit_ns;
+        info->launch_ns = r->launch_ns;
+        info->done_ns = r->done_ns;
+        info->u = r->u;
+        info->v = r->v;
+        info->status = r->status;
+        info->n = r->n;
+        info->obj = r->obj;
+    }
+    return GN_OK;
+}
+
+int gn_server_poll(gn_server *s, int64_t id, gn_request_info *info) {
+    if (!s) return fail(GN_EINVAL, "null server");
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(id);
+    if (it == s->reqs.end()) return fail(GN_ENOTFOUND, "unknown request id");
+    return report(it->second, info);
+}
+
+int gn_server_wait(gn_server *s, int64_t id, int timeout_ms, gn_request_info *info) {
+    if (!s) return fail(GN_EINVAL, "null server");
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(id);
+    if (it == s->reqs.end()) return fail(GN_ENOTFOUND, "unknown request id");
+    Req *r = it->second;
+    auto ready = [r] { return r->state == kDone || r->state == kFailed; };
+    if (timeout_ms < 0) s->cv_done.wait(lk, ready);
+    else s->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), ready);
+    return report(r, info);
+}
+
+int gn_server_release(gn_server *s, int64_t id) {
+    if (!s) return fail(GN_EINVAL, "null server");
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        auto it = s->reqs.find(id);
+        if (it == s->reqs.end()) return fail(GN_ENOTFOUND, "unknown request id");
+        Req *r = it->second;
+        if (r->state != kDone && r->state != kFailed)
+            return fail(GN_ESTATE, "request is still queued or running");
+        if (r->u) s->arena.release(r->off, r->bytes);
+        s->reqs.erase(it);
+        delete r;
+        s->dirty = true;
+    }
+    s->cv_work.notify_one();
+    return GN_OK;
+}
+
+int gn_server_stats_get(gn_server *s, gn_server_stats *stats) {
+    if (!s || !stats) return fail(GN_EINVAL, "null argument");
+    std::lock_guard<std::mutex> lk(s->mu);
+    *stats = s->stats;
+    return GN_OK;
+}
+
+int gn_server_destroy(gn_server *s) {
+    if (!s) return GN_OK;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+    }
+    s->cv_work.notify_all();
+    if (s->worker.joinable()) s->worker.join();
+    for (Slot &sl : s->slots) {
+        (void)hipStreamSynchronize(sl.st);
+        (void)hipStreamDestroy(sl.st);
+        (void)hipEventDestroy(sl.ev);
+        (void)hipHostFree(sl.h_stage);
+        (void)hipFree(sl.d_stage);
+    }
+    (void)hipFree(s->d_arena);
+    (void)hipFree(s->d_curves);
+    for (auto &kv : s->reqs) delete kv.second;
+    delete s;
+    return GN_OK;
+}
+
+}  // extern "C"

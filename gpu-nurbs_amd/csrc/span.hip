@@ -10,48 +10,6 @@
 
 namespace gn {
 
-// M[r*k + a] = coefficient of t^r in N_{span-deg+a, deg}(k[span] + t*(k[span+1]-k[span])).
-// Same matrix as the Nrc recursion of src/NURBS/Evaluate.cu:90-103; 0/0 is taken as 0.
-__device__ void basis_matrix(int deg, const double *__restrict__ kn, int span, double *M) {
-    double N[GN_MAXK][GN_MAXK], T[GN_MAXK][GN_MAXK];
-    for (int a = 0; a < GN_MAXK; a++)
-        for (int e = 0; e < GN_MAXK; e++) N[a][e] = 0.0;
-    N[0][0] = 1.0;
-    const double h = kn[span + 1] - kn[span];
-    for (int d = 1; d <= deg; d++) {
-        for (int a = 0; a <= d; a++)
-            for (int e = 0; e <= d; e++) T[a][e] = 0.0;
-        for (int a = 0; a <= d; a++) {
-            const int i = span - d + a;
-            if (a >= 1) {
-                double den = kn[i + d] - kn[i];
-                if (den != 0.0) {
-                    double c0 = (kn[span] - kn[i]) / den, c1 = h / den;
-                    for (int e = 0; e < d; e++) {
-                        T[a][e] += c0 * N[a - 1][e];
-                        T[a][e + 1] += c1 * N[a - 1][e];
-                    }
-                }
-            }
-            if (a <= d - 1) {
-                double den = kn[i + d + 1] - kn[i + 1];
-                if (den != 0.0) {
-                    double c0 = (kn[i + d + 1] - kn[span]) / den, c1 = -h / den;
-                    for (int e = 0; e < d; e++) {
-                        T[a][e] += c0 * N[a][e];
-                        T[a][e + 1] += c1 * N[a][e];
-                    }
-                }
-            }
-        }
-        for (int a = 0; a <= d; a++)
-            for (int e = 0; e <= d; e++) N[a][e] = T[a][e];
-    }
-    const int k = deg + 1;
-    for (int r = 0; r < k; r++)
-        for (int a = 0; a < k; a++) M[r * k + a] = N[a][r];
-}
-
 // One thread per span slot in one direction: basis matrix (k*k doubles) and 1/h.
 __global__ void k_basis(int deg, int ncp, const double *__restrict__ kn, double *__restrict__ Mout,
                         double *__restrict__ ih) {

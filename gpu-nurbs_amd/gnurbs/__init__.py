@@ -91,6 +91,20 @@ _SIGS = [
     ("gn_batch_graph_create", _c_int, [_c_void, _c_void, _c_int, _c_void, _c_void, _c_void, _c_size, ctypes.POINTER(InvertOpts), _c_void, _c_void, _c_void, _c_void, ctypes.POINTER(_c_void)]),
     ("gn_graph_launch", _c_int, [_c_void, _c_void]),
     ("gn_graph_destroy", _c_int, [_c_void]),
+    ("gn_curve_create", _c_int, [_c_int, _c_int, _c_void, _c_void, _c_void, _c_void, ctypes.POINTER(_c_void)]),
+    ("gn_curve_destroy", _c_int, [_c_void]),
+    ("gn_curve_info", _c_int, [_c_void, ctypes.POINTER(_c_int), ctypes.POINTER(_c_int)]),
+    ("gn_curve_device_bytes", _c_size, [_c_void]),
+    ("gn_curve_evaluate", _c_int, [_c_void, _c_void, _c_size, _c_int, _c_void, _c_void]),
+    ("gn_curve_invert", _c_int, [_c_void, _c_void, _c_void, _c_void, _c_size, ctypes.POINTER(InvertOpts), _c_void, _c_void, _c_void]),
+    ("gn_server_create", _c_int, [_c_void, _c_void, _c_int, ctypes.POINTER(InvertOpts), _c_size, _c_int, ctypes.POINTER(_c_void)]),
+    ("gn_server_submit", _c_int, [_c_void, _c_int, _c_void, _c_void, _c_void, _c_size, ctypes.POINTER(ctypes.c_int64)]),
+    ("gn_server_poll", _c_int, [_c_void, ctypes.c_int64, _c_void]),
+    ("gn_server_wait", _c_int, [_c_void, ctypes.c_int64, _c_int, _c_void]),
+    ("gn_server_release", _c_int, [_c_void, ctypes.c_int64]),
+    ("gn_server_stats_get", _c_int, [_c_void, _c_void]),
+    ("gn_server_destroy", _c_int, [_c_void]),
+    ("gn_now_ns", ctypes.c_int64, []),
 ]
 
 SEGMENT = np.dtype([("obj", np.int32), ("count", np.int32), ("start", np.int64)])
@@ -326,6 +340,178 @@ class Surface:
     Invert = NewInverse
 
 
+class Curve:
+    """A rational B-spline curve resident in HBM (gn_curve); no counterpart in the reference.
+
+    Construction: (degree, knots, control_points (n, 3), weights (n,)).
+    """
+
+    def __init__(self, degree, knots, control_points, weights, stream=None):
+        self._h = None
+        cp = np.ascontiguousarray(control_points, dtype=np.float64)
+        w = np.ascontiguousarray(weights, dtype=np.float64)
+        k = np.ascontiguousarray(knots, dtype=np.float64)
+        h = _c_void()
+        _check(lib().gn_curve_create(int(degree), int(w.size), k.ctypes.data, cp.ctypes.data, w.ctypes.data,
+                                     _c_void(stream_handle(stream)), ctypes.byref(h)))
+        self._h = h
+        p, n = _c_int(), _c_int()
+        _check(lib().gn_curve_info(self._h, ctypes.byref(p), ctypes.byref(n)))
+        self.p, self.n = p.value, n.value
+        _live.add(self)
+
+    @classmethod
+    def from_def(cls, cd, stream=None) -> "Curve":
+        return cls(cd.p, cd.k, cd.cp, cd.w, stream)
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if self._h:
+            lib().gn_curve_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def device_bytes(self) -> int:
+        return int(lib().gn_curve_device_bytes(self._h))
+
+    def Evaluate(self, t, order: int = 0, out=None, stream=None):
+        """C (and C', C'') at t; returns a device tensor of shape (3*(order+1), n)."""
+        torch = _torch()
+        td = to_device(t, torch.float64)
+        n = td.numel()
+        nblk = 3 * (order + 1) if 0 <= order <= 2 else 3
+        out = out if out is not None else torch.empty((nblk, n), dtype=torch.float64, device="cuda")
+        _check(lib().gn_curve_evaluate(self._h, td.data_ptr(), n, order, out.data_ptr(), _c_void(stream_handle(stream))))
+        return out
+
+    def Invert(self, x, y, z, opts: Optional[InvertOpts] = None, seeds=None, out=None, stream=None):
+        """Point inversion onto the curve; returns (t, status) device tensors."""
+        torch = _torch()
+        xd, yd, zd = (to_device(a, torch.float64) for a in (x, y, z))
+        n = xd.numel()
+        opts = opts if opts is not None else default_opts()
+        if out is not None:
+            t, st = out
+        else:
+            t = torch.empty(n, dtype=torch.float64, device="cuda")
+            st = torch.empty(n, dtype=torch.uint8, device="cuda")
+        if seeds is not None:
+            t.copy_(to_device(seeds, torch.float64))
+            opts.seed_mode = SEED_GIVEN
+        _check(lib().gn_curve_invert(self._h, xd.data_ptr(), yd.data_ptr(), zd.data_ptr(), n, ctypes.byref(opts),
+                                     t.data_ptr(), st.data_ptr(), _c_void(stream_handle(stream))))
+        return t, st
+
+
+class RequestInfo(ctypes.Structure):
+    _fields_ = [("submit_ns", ctypes.c_int64), ("admit_ns", ctypes.c_int64), ("launch_ns", ctypes.c_int64),
+                ("done_ns", ctypes.c_int64), ("u", _c_void), ("v", _c_void), ("status", _c_void),
+                ("n", _c_size), ("obj", _c_int)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("submitted", ctypes.c_int64), ("completed", ctypes.c_int64), ("steps", ctypes.c_int64),
+                ("points", ctypes.c_int64), ("max_step_requests", ctypes.c_int64), ("arena_bytes", _c_size),
+                ("arena_peak", _c_size)]
+
+
+class _DeviceView:
+    """Zero-copy view of a device array for torch.as_tensor (CUDA array interface)."""
+
+    def __init__(self, ptr, n, typestr):
+        self.__cuda_array_interface__ = {"shape": (int(n),), "typestr": typestr, "data": (int(ptr or 0), False),
+                                         "version": 2, "strides": None}
+
+
+def now_ns() -> int:
+    """CLOCK_MONOTONIC in ns: the clock of the server's request timestamps."""
+    return int(lib().gn_now_ns())
+
+
+class Server:
+    """Continuous-batching inversion server (gn_server): objects 0..nsurf-1 are the batch's
+    surfaces, then the curves."""
+
+    def __init__(self, batch=None, curves=(), opts: Optional[InvertOpts] = None, hbm_budget: int = 1 << 30,
+                 streams: int = 2):
+        self._h = None
+        self._batch, self._curves = batch, list(curves)  # must outlive the server
+        arr = (_c_void * max(1, len(self._curves)))(*[c.handle for c in self._curves])
+        opts = opts if opts is not None else default_opts()
+        h = _c_void()
+        _check(lib().gn_server_create(batch.handle if batch is not None else None, arr, len(self._curves),
+                                      ctypes.byref(opts), int(hbm_budget), int(streams), ctypes.byref(h)))
+        self._h = h
+        self._keep = {}
+        _live.add(self)
+
+    def close(self):
+        if self._h:
+            lib().gn_server_destroy(self._h)
+            self._h = None
+            self._keep.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, obj: int, x, y, z) -> int:
+        """x, y, z: device float64 tensors, kept alive until release."""
+        torch = _torch()
+        xd, yd, zd = (to_device(a, torch.float64) for a in (x, y, z))
+        rid = ctypes.c_int64()
+        _check(lib().gn_server_submit(self._h, int(obj), xd.data_ptr(), yd.data_ptr(), zd.data_ptr(), xd.numel(),
+                                      ctypes.byref(rid)))
+        self._keep[rid.value] = (xd, yd, zd)
+        return rid.value
+
+    def poll(self, rid: int) -> Optional[RequestInfo]:
+        info = RequestInfo()
+        rc = lib().gn_server_poll(self._h, rid, ctypes.byref(info))
+        if rc == GN_EAGAIN:
+            return None
+        _check(rc)
+        return info
+
+    def wait(self, rid: int, timeout_ms: int = -1) -> Optional[RequestInfo]:
+        info = RequestInfo()
+        rc = lib().gn_server_wait(self._h, rid, int(timeout_ms), ctypes.byref(info))
+        if rc == GN_EAGAIN:
+            return None
+        _check(rc)
+        return info
+
+    @staticmethod
+    def results(info: RequestInfo):
+        """Copies of a finished request's results: (u, v, status) for surfaces, (t, None, status)
+        for curves, as device tensors."""
+        torch = _torch()
+        n = info.n
+        u = torch.as_tensor(_DeviceView(info.u, n, "<f8"), device="cuda").clone()
+        v = torch.as_tensor(_DeviceView(info.v, n, "<f8"), device="cuda").clone() if info.v else None
+        st = torch.as_tensor(_DeviceView(info.status, n, "|u1"), device="cuda").clone()
+        return u, v, st
+
+    def release(self, rid: int):
+        _check(lib().gn_server_release(self._h, rid))
+        self._keep.pop(rid, None)
+
+    def stats(self) -> ServerStats:
+        s = ServerStats()
+        _check(lib().gn_server_stats_get(self._h, ctypes.byref(s)))
+        return s
+
+
 def segments(objs, counts, starts=None) -> np.ndarray:
     """gn_segment array: points starts[i] .. starts[i]+counts[i]-1 belong to object objs[i]
     (starts default to the running sum of counts)."""
@@ -369,6 +555,10 @@ class Batch:
         self._h = h
         self.count = len(surfaces)
         _live.add(self)
+
+    @property
+    def handle(self):
+        return self._h
 
     def close(self):
         if self._h:

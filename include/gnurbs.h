@@ -207,6 +207,74 @@ int gn_batch_graph_create(gn_batch *b, const gn_segment *segs, int nseg, const d
 int gn_graph_launch(gn_graph *g, gn_stream stream);
 int gn_graph_destroy(gn_graph *g);
 
+/* ---- curves ------------------------------------------------------------------------------- */
+/* No counterpart in the reference: its README (README.md:10) names curves but src/ has no curve
+ * code.  Same matrix form in one parameter: per span, patch = M * P_w with M the span's basis
+ * matrix (as GetMatrixN, src/NURBS/Evaluate.cu:3-88, in one direction). */
+
+/* degree 1..7, cp control points (host arrays: knots cp+degree+1, xyz cp*3, w cp); copied. */
+int gn_curve_create(int degree, int cp, const double *knots, const double *xyz, const double *w,
+                    gn_stream stream, gn_curve **out);
+int gn_curve_destroy(gn_curve *c);
+int gn_curve_info(const gn_curve *c, int *degree, int *cp);
+size_t gn_curve_device_bytes(const gn_curve *c);
+/* out (device, SoA blocks of n doubles): C.x C.y C.z, then C' (max_order >= 1), then C''
+ * (max_order 2); derivatives with respect to t. */
+int gn_curve_evaluate(const gn_curve *c, const double *t, size_t n, int max_order, double *out,
+                      gn_stream stream);
+/* Point inversion onto the curve: 1-D Gauss-Newton (opts->newton adds the curvature term),
+ * seeds from 8 samples per span (or GN_SEED_GIVEN: seeds in t_out).  GN_FP64 or GN_MIXED. */
+int gn_curve_invert(const gn_curve *c, const double *x, const double *y, const double *z, size_t n,
+                    const gn_invert_opts *opts, double *t_out, uint8_t *status, gn_stream stream);
+
+/* ---- serving: continuous batching of inversion requests ------------------------------------ */
+/* No counterpart in the reference (its driver inverts one fixed batch, src/main.cu:277-327).
+ * A server owns a worker thread that gathers submitted requests into steps: each step is one
+ * batched launch over every admitted request (surfaces: the gn_batch kernel; curves: one
+ * workgroup per tile), launched as soon as one of its `streams` step slots is free, so requests
+ * that arrive while a step runs join the next one.  Admission: a request's results (u, v, status
+ * for surfaces; t in u and status for curves) live in a server-owned HBM arena of size
+ * hbm_budget minus the resident objects; requests are admitted in arrival order while their
+ * results fit, and stay admitted until gn_server_release.  Results do not depend on what else
+ * shares a step. */
+
+typedef struct {
+    int64_t submit_ns, admit_ns, launch_ns, done_ns; /* CLOCK_MONOTONIC (gn_now_ns)           */
+    const double *u, *v;     /* device results (curves: t in u, v NULL); valid until release */
+    const uint8_t *status;
+    size_t n;
+    int obj;
+} gn_request_info;
+
+typedef struct {
+    int64_t submitted, completed, steps, points; /* totals since creation                    */
+    int64_t max_step_requests;                   /* most requests in one step                */
+    size_t arena_bytes, arena_peak;              /* result arena capacity and peak use       */
+} gn_server_stats;
+
+/* Objects 0 .. nsurf-1 are the batch's surfaces (batch may be NULL), nsurf .. nsurf+ncurves-1
+ * the curves.  The batch and the curves must outlive the server.  opts: as gn_batch_invert
+ * (GN_FP64 or GN_MIXED; GIVEN seeds are not available).  streams: concurrent steps, 1..8.
+ * GN_EBUDGET if the objects alone exceed hbm_budget. */
+int gn_server_create(gn_batch *batch, gn_curve *const *curves, int ncurves,
+                     const gn_invert_opts *opts, size_t hbm_budget, int streams,
+                     gn_server **out);
+/* Queues n points (device arrays x, y, z, caller-owned, valid until the request is done) for
+ * object obj.  GN_EBUDGET if its results could never fit the arena. */
+int gn_server_submit(gn_server *s, int obj, const double *x, const double *y, const double *z,
+                     size_t n, int64_t *id);
+/* GN_OK and info when done; GN_EAGAIN while queued or running; GN_ENOTFOUND for an unknown id;
+ * GN_EDEVICE if its step failed. */
+int gn_server_poll(gn_server *s, int64_t id, gn_request_info *info);
+/* As poll, blocking up to timeout_ms (< 0: no limit). */
+int gn_server_wait(gn_server *s, int64_t id, int timeout_ms, gn_request_info *info);
+/* Frees a finished request's results (GN_ESTATE while it is still queued or running). */
+int gn_server_release(gn_server *s, int64_t id);
+int gn_server_stats_get(gn_server *s, gn_server_stats *stats);
+/* Finishes the steps in flight, fails the requests not yet launched, frees the arena. */
+int gn_server_destroy(gn_server *s);
+int64_t gn_now_ns(void);
+
 #ifdef __cplusplus
 }
 #endif

@@ -27,6 +27,9 @@
  *   or_gauss_newton     round trips on on-surface points, orthogonality of the residual
  *   or_build_tree       containment of dense samples, index formula, NeedDivide truth table
  *   or_seed_*           brute force
+ *   or_curve_*          (no reference counterpart) exact circle arcs, scipy BSpline with textbook
+ *                       derivative control points, finite differences, closed-form foot points
+ *                       on the circle, brute-force nearest sample (tests/test_oracle_curve.py)
  */
 #define _GNU_SOURCE
 #include <math.h>
@@ -729,4 +732,191 @@ void or_seed_greedy_leaf(const double *nodes, int levels, const double *x, const
     for (int l = 0; l <= D; l++) { free(bx[l]); free(b32[l]); }
     free(bx);
     free(b32);
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* Curves.  The reference has no curve code (README.md:10 names curves; src/ has none), so this  */
+/* part restates textbook algorithms instead of reference lines: basis functions and their       */
+/* derivatives by the Cox-de Boor triangle (Piegl & Tiller, The NURBS Book, A2.3), the rational  */
+/* quotient C = A/w, C' = (A' - w'C)/w, C'' = (A'' - 2w'C' - w''C)/w (eq. 4.8 ff.), and 1-D       */
+/* Gauss-Newton / Newton on f(t) = C'(t).(C(t) - P).  It shares no code with the matrix form of  */
+/* the GPU path.  Pinned by: exact circle arcs (w = cos(pi/4) quarter arcs), scipy BSpline for   */
+/* the non-rational value and derivatives, finite differences, round trips.                     */
+/* ------------------------------------------------------------------------------------------ */
+#define OR_CURVE_SAMPLES 8
+
+/* ders[k][j] = k-th derivative of N_{span-p+j,p} at u, k <= nd (A2.3) */
+static void ders_basis(int span, double u, int p, int nd, const double *U, double ders[3][OR_MAXK])
+{
+    double ndu[OR_MAXK][OR_MAXK], left[OR_MAXK], right[OR_MAXK], a[2][OR_MAXK];
+    ndu[0][0] = 1.0;
+    for (int j = 1; j <= p; j++) {
+        left[j] = u - U[span + 1 - j];
+        right[j] = U[span + j] - u;
+        double saved = 0.0;
+        for (int r = 0; r < j; r++) {
+            ndu[j][r] = right[r + 1] + left[j - r];
+            double temp = ndu[r][j - 1] / ndu[j][r];
+            ndu[r][j] = saved + right[r + 1] * temp;
+            saved = left[j - r] * temp;
+        }
+        ndu[j][j] = saved;
+    }
+    for (int k = 0; k <= 2; k++)
+        for (int j = 0; j <= p; j++) ders[k][j] = 0.0;
+    for (int j = 0; j <= p; j++) ders[0][j] = ndu[j][p];
+    for (int r = 0; r <= p; r++) {
+        int s1 = 0, s2 = 1;
+        a[0][0] = 1.0;
+        for (int k = 1; k <= nd && k <= p; k++) {
+            double d = 0.0;
+            int rk = r - k, pk = p - k;
+            if (r >= k) {
+                a[s2][0] = a[s1][0] / ndu[pk + 1][rk];
+                d = a[s2][0] * ndu[rk][pk];
+            }
+            int j1 = rk >= -1 ? 1 : -rk;
+            int j2 = (r - 1 <= pk) ? k - 1 : p - r;
+            for (int j = j1; j <= j2; j++) {
+                a[s2][j] = (a[s1][j] - a[s1][j - 1]) / ndu[pk + 1][rk + j];
+                d += a[s2][j] * ndu[rk + j][pk];
+            }
+            if (r <= pk) {
+                a[s2][k] = -a[s1][k - 1] / ndu[pk + 1][r];
+                d += a[s2][k] * ndu[r][pk];
+            }
+            ders[k][r] = d;
+            int t = s1; s1 = s2; s2 = t;
+        }
+    }
+    double f = p;
+    for (int k = 1; k <= nd && k <= p; k++) {
+        for (int j = 0; j <= p; j++) ders[k][j] *= f;
+        f *= (p - k);
+    }
+}
+
+typedef struct {
+    int p, n;
+    const double *k, *xyz, *w;
+} or_curve;
+
+/* C, C', C'' (xyz each) at t; order 0..2 */
+static void curve_eval_one(const or_curve *c, double t, int order, double out[9])
+{
+    int span = or_find_span(c->p, c->n, c->k, t);
+    double d[3][OR_MAXK];
+    ders_basis(span, t, c->p, order, c->k, d);
+    double A[3][4] = {{0}};
+    for (int j = 0; j <= c->p; j++) {
+        int g = span - c->p + j;
+        double w = c->w[g];
+        double hp[4] = {c->xyz[3 * g] * w, c->xyz[3 * g + 1] * w, c->xyz[3 * g + 2] * w, w};
+        for (int kk = 0; kk <= order; kk++)
+            for (int q = 0; q < 4; q++) A[kk][q] += d[kk][j] * hp[q];
+    }
+    double iw = 1.0 / A[0][3];
+    for (int q = 0; q < 3; q++) out[q] = A[0][q] * iw;
+    if (order >= 1)
+        for (int q = 0; q < 3; q++) out[3 + q] = (A[1][q] - A[1][3] * out[q]) * iw;
+    if (order >= 2)
+        for (int q = 0; q < 3; q++)
+            out[6 + q] = (A[2][q] - 2.0 * A[1][3] * out[3 + q] - A[2][3] * out[q]) * iw;
+}
+
+/* out: SoA blocks of n doubles -- C (3), then C' (3) if order >= 1, then C'' (3) if order 2 */
+void or_curve_eval(int p, int cp, const double *knots, const double *xyz, const double *w,
+                   const double *t, long n, int order, double *out, int nthreads)
+{
+    or_curve c = {p, cp, knots, xyz, w};
+#ifdef _OPENMP
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel for schedule(static) num_threads(nthreads)
+#else
+    (void)nthreads;
+#endif
+    for (long i = 0; i < n; i++) {
+        double o[9];
+        curve_eval_one(&c, t[i], order, o);
+        for (int q = 0; q < 3 * (order + 1); q++) out[q * n + i] = o[q];
+    }
+}
+
+/* Seeds: parameter of the nearest (FP32 squared distance, first index on ties) of 8 samples per
+ * domain span at t = k[m] + (j + 1/2) h / 8, samples rounded to FP32. */
+void or_curve_seed(int p, int cp, const double *knots, const double *xyz, const double *w,
+                   const double *x, const double *y, const double *z, long n, double *t_out)
+{
+    or_curve c = {p, cp, knots, xyz, w};
+    long ns = (long)(cp - p) * OR_CURVE_SAMPLES;
+    float *sp = (float *)malloc(sizeof(float) * 3 * ns);
+    double *st = (double *)malloc(sizeof(double) * ns);
+    for (long i = 0; i < ns; i++) {
+        int slot = (int)(i / OR_CURVE_SAMPLES), j = (int)(i % OR_CURVE_SAMPLES);
+        double lo = knots[slot + p], hi = knots[slot + p + 1];
+        double t = lo + (j + 0.5) * (hi - lo) / OR_CURVE_SAMPLES;
+        double o[9];
+        curve_eval_one(&c, t, 0, o);
+        for (int q = 0; q < 3; q++) sp[3 * i + q] = (float)o[q];
+        st[i] = t;
+    }
+    for (long i = 0; i < n; i++) {
+        float P[3] = {(float)x[i], (float)y[i], (float)z[i]};
+        float best = INFINITY;
+        long bj = 0;
+        for (long j = 0; j < ns; j++) {
+            float dx = sp[3 * j] - P[0], dy = sp[3 * j + 1] - P[1], dz = sp[3 * j + 2] - P[2];
+            float d2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+            if (d2 < best) { best = d2; bj = j; }
+        }
+        t_out[i] = st[bj];
+    }
+    free(sp);
+    free(st);
+}
+
+/* 1-D Gauss-Newton (newton: f' = C'.C' + C''.(C-P) when positive) from given seeds, with the
+ * stopping rules of the surface iteration: |C-P| < tol_dist before the update, |dt| < tol_step
+ * after it. */
+void or_curve_invert(int p, int cp, const double *knots, const double *xyz, const double *w,
+                     const double *x, const double *y, const double *z, long n, double tol_dist,
+                     double tol_step, int max_iter, int clamp, int newton, const double *seed,
+                     double *t_out, unsigned char *status, int *iters, int nthreads)
+{
+    or_curve c = {p, cp, knots, xyz, w};
+    double lo = knots[p], hi = knots[cp];
+#ifdef _OPENMP
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 256) num_threads(nthreads)
+#else
+    (void)nthreads;
+#endif
+    for (long i = 0; i < n; i++) {
+        double P[3] = {x[i], y[i], z[i]}, t = seed[i];
+        int st = OR_MAXITER, it;
+        for (it = 0; it < max_iter; it++) {
+            double o[9];
+            curve_eval_one(&c, t, newton ? 2 : 1, o);
+            double r[3] = {o[0] - P[0], o[1] - P[1], o[2] - P[2]};
+            double rr = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+            if (!(rr == rr)) { st = OR_NONFINITE; break; }
+            if (tol_dist > 0 && sqrt(rr) < tol_dist) { st = OR_CONV_DIST; break; }
+            double f = o[3] * r[0] + o[4] * r[1] + o[5] * r[2];
+            double fp = o[3] * o[3] + o[4] * o[4] + o[5] * o[5];
+            if (newton) {
+                double f2 = fp + o[6] * r[0] + o[7] * r[1] + o[8] * r[2];
+                if (f2 > 0) fp = f2;
+            }
+            if (!(fp == fp)) { st = OR_NONFINITE; break; }
+            if (fp == 0) { st = OR_SINGULAR; break; }
+            double dt = f / fp;
+            t -= dt;
+            if (clamp) t = t < lo ? lo : (t > hi ? hi : t);
+            if (!(t == t)) { st = OR_NONFINITE; it++; break; }
+            if (tol_step > 0 && fabs(dt) < tol_step) { st = OR_CONV_STEP; it++; break; }
+        }
+        t_out[i] = t;
+        if (status) status[i] = (unsigned char)st;
+        if (iters) iters[i] = it;
+    }
 }

@@ -51,6 +51,10 @@ def lib():
                                           _D, _D, _I, _I, _P, _P, _P, _P, _I]
         L.or_seed_nearest_leaf.argtypes = [_P, _I, _P, _P, _P, _L, _P, _P, _P]
         L.or_seed_greedy_leaf.argtypes = [_P, _I, _P, _P, _P, _L, _P, _P, _P]
+        L.or_curve_eval.argtypes = [_I, _I, _P, _P, _P, _P, _L, _I, _P, _I]
+        L.or_curve_seed.argtypes = [_I, _I, _P, _P, _P, _P, _P, _P, _L, _P]
+        L.or_curve_invert.argtypes = [_I, _I, _P, _P, _P, _P, _P, _P, _L, _D, _D, _I, _I, _I, _P, _P, _P,
+                                      _P, _I]
         _l = L
     return _l
 
@@ -182,3 +186,42 @@ def seed_greedy_leaf(nodes, levels, x, y, z):
     lib().or_seed_greedy_leaf(nodes.ctypes.data, levels, x.ctypes.data, y.ctypes.data, z.ctypes.data, n,
                               leaf.ctypes.data, su.ctypes.data, sv.ctypes.data)
     return leaf, su, sv
+
+
+class OCurve:
+    """Rational B-spline curve evaluated by the textbook de Boor form (no matrix form)."""
+
+    def __init__(self, p, k, cp, w):
+        self.p = int(p)
+        self.k, self.cp, self.w = _a(k), _a(cp), _a(w)
+        self.n = self.w.size
+
+    @classmethod
+    def from_def(cls, cd):
+        return cls(cd.p, cd.k, cd.cp, cd.w)
+
+    def _args(self):
+        return (self.p, self.n, self.k.ctypes.data, self.cp.ctypes.data, self.w.ctypes.data)
+
+    def eval(self, t, order=0, nthreads=0) -> np.ndarray:
+        t = _a(t)
+        out = np.zeros((3 * (order + 1), t.size))
+        lib().or_curve_eval(*self._args(), t.ctypes.data, t.size, order, out.ctypes.data, nthreads)
+        return out
+
+    def seed(self, x, y, z) -> np.ndarray:
+        x, y, z = _a(x), _a(y), _a(z)
+        t = np.zeros(x.size)
+        lib().or_curve_seed(*self._args(), x.ctypes.data, y.ctypes.data, z.ctypes.data, x.size, t.ctypes.data)
+        return t
+
+    def invert(self, x, y, z, seed, tol_dist=0.0, tol_step=1e-10, max_iter=20, clamp=1, newton=0, nthreads=0):
+        x, y, z, sd = (_a(a) for a in (x, y, z, seed))
+        n = x.size
+        t = np.zeros(n)
+        st = np.zeros(n, dtype=np.uint8)
+        it = np.zeros(n, dtype=np.int32)
+        lib().or_curve_invert(*self._args(), x.ctypes.data, y.ctypes.data, z.ctypes.data, n, tol_dist, tol_step,
+                              max_iter, clamp, newton, sd.ctypes.data, t.ctypes.data, st.ctypes.data,
+                              it.ctypes.data, nthreads)
+        return t, st, it

@@ -71,3 +71,45 @@ def test_argument_checks_fail_cleanly_without_gpu():
     assert L.gn_evaluate(None, None, None, 0, 0, None, None) == G.GN_EINVAL
     assert L.gn_invert(None, None, None, None, 0, None, None, None, None, None) == G.GN_EINVAL
     assert L.gn_surface_from_nu(b"/nonexistent/file.nu", None, ctypes.byref(h)) == G.GN_EIO
+
+
+def test_curve_argument_checks_fail_cleanly_without_gpu():
+    import numpy as np
+
+    L = G.lib()
+    h = ctypes.c_void_p()
+    k = np.array([0, 0, 0, 0, 1, 1, 1, 1.0])
+    cp, w = np.zeros(12), np.ones(4)
+    assert L.gn_curve_create(9, 4, k.ctypes.data, cp.ctypes.data, w.ctypes.data, None, ctypes.byref(h)) == G.GN_EINVAL
+    assert b"degree" in L.gn_last_error()
+    assert L.gn_curve_create(3, 3, k.ctypes.data, cp.ctypes.data, w.ctypes.data, None, ctypes.byref(h)) == G.GN_EINVAL
+    kd = np.array([0, 0, 0, 0, 1, 0.5, 1, 1.0])
+    assert L.gn_curve_create(3, 4, kd.ctypes.data, cp.ctypes.data, w.ctypes.data, None, ctypes.byref(h)) == G.GN_EINVAL
+    assert b"non-decreasing" in L.gn_last_error()
+    w0 = np.zeros(4)
+    assert L.gn_curve_create(3, 4, k.ctypes.data, cp.ctypes.data, w0.ctypes.data, None, ctypes.byref(h)) == G.GN_EINVAL
+    assert b"weights" in L.gn_last_error()
+    assert L.gn_curve_evaluate(None, None, 0, 0, None, None) == G.GN_EINVAL
+    assert L.gn_curve_invert(None, None, None, None, 0, None, None, None, None) == G.GN_EINVAL
+
+
+def test_server_argument_checks_fail_cleanly_without_gpu():
+    L = G.lib()
+    h = ctypes.c_void_p()
+    o = G.default_opts()
+    assert L.gn_server_create(None, None, 0, ctypes.byref(o), 1 << 30, 2, ctypes.byref(h)) == G.GN_EINVAL
+    assert b"objects" in L.gn_last_error()
+    fake = (ctypes.c_void_p * 1)(None)
+    assert L.gn_server_create(None, fake, 1, ctypes.byref(o), 1 << 30, 0, ctypes.byref(h)) == G.GN_EINVAL
+    assert b"streams" in L.gn_last_error()
+    o.seed_mode = G.SEED_GIVEN
+    assert L.gn_server_create(None, fake, 1, ctypes.byref(o), 1 << 30, 2, ctypes.byref(h)) == G.GN_EINVAL
+    o = G.default_opts()
+    assert L.gn_server_create(None, fake, 1, ctypes.byref(o), 1 << 30, 2, ctypes.byref(h)) == G.GN_EINVAL
+    assert b"null curve" in L.gn_last_error()
+    rid = ctypes.c_int64()
+    assert L.gn_server_submit(None, 0, None, None, None, 0, ctypes.byref(rid)) == G.GN_EINVAL
+    assert L.gn_server_poll(None, 1, None) == G.GN_EINVAL
+    assert L.gn_server_release(None, 1) == G.GN_EINVAL
+    assert L.gn_server_destroy(None) == G.GN_OK
+    assert L.gn_now_ns() > 0
