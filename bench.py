@@ -165,6 +165,141 @@ def bench_batch(G, synth, torch, steps, count=1024):
     return res
 
 
+def bench_serve(G, synth, torch, duration=1.0, loads=(0.5, 0.8), budget=32 << 30, streams=2,
+                modes=("mixed", "plain_fp64")):
+    """Serving: 224 uneven surfaces (the uneven-batch generator, seed 1004) and 32 cubic rational
+    helix curves (64 control points) resident; requests of log-uniform size on [64, 100000] points
+    for a uniformly drawn object, Poisson arrivals at 50% and 80% of the capacity measured by a
+    saturating burst; continuous batching (gn_server, 2 step slots, 32 GB HBM budget).  Latency is
+    from the scheduled arrival to the server's completion time."""
+    import ctypes
+    import sys
+
+    import numpy as np
+
+    L = G.lib()
+    pool = 100_000
+    defs, _ = synth.uneven_batch(224, seed=1004)
+    gen = torch.Generator(device="cuda").manual_seed(1004)
+    surfs, curves, pools = [], [], []
+    for sd in defs:
+        s = G.Surface(sd.p, sd.q, sd.ku, sd.kv, sd.cp, sd.w)
+        s.BuildTree(3)
+        surfs.append(s)
+        u0 = torch.rand(pool, dtype=torch.float64, device="cuda", generator=gen)
+        v0 = torch.rand(pool, dtype=torch.float64, device="cuda", generator=gen)
+        ev = s.Evaluate(u0, v0, 1)
+        nrm = torch.linalg.cross(ev[3:6].T, ev[6:9].T).T
+        nrm = nrm / torch.linalg.norm(nrm, dim=0, keepdim=True)
+        P = ev[0:3] + torch.randn(pool, dtype=torch.float64, device="cuda", generator=gen) * 1e-2 * nrm
+        pools.append([P[k].contiguous() for k in range(3)])
+    for i in range(32):
+        c = G.Curve.from_def(synth.helix_curve(64, 3, seed=1004 * 100 + i))
+        curves.append(c)
+        t0 = torch.rand(pool, dtype=torch.float64, device="cuda", generator=gen)
+        ev = c.Evaluate(t0, 1)
+        tan = ev[3:6] / torch.linalg.norm(ev[3:6], dim=0, keepdim=True)
+        a = torch.randn((3, pool), dtype=torch.float64, device="cuda", generator=gen)
+        nrm = a - tan * (a * tan).sum(0)
+        nrm = nrm / torch.linalg.norm(nrm, dim=0, keepdim=True)
+        P = ev[0:3] + torch.randn(pool, dtype=torch.float64, device="cuda", generator=gen) * 1e-2 * nrm
+        pools.append([P[k].contiguous() for k in range(3)])
+    torch.cuda.synchronize()
+    batch = G.Batch(surfs)
+    batch.set_shape(512, 0)
+    ptrs = np.array([[t.data_ptr() for t in p] for p in pools], dtype=np.uint64)
+    lo, hi = 64, 100_000
+    mean_size = (hi - lo) / np.log(hi / lo)
+    res = {"workload": "224 surfaces (degrees 2..5, nets 4..48) + 32 cubic rational curves (64 cps); "
+                       "requests log-uniform 64..100000 points, uniform object, Poisson arrivals",
+           "hbm_budget_bytes": budget, "step_slots": streams, "objects_device_bytes": batch.device_bytes() + sum(c.device_bytes() for c in curves)}
+    old_switch = sys.getswitchinterval()
+    sys.setswitchinterval(1e-4)
+
+    def trace(n_req, rate, seed):
+        rng = np.random.default_rng(seed)
+        sizes = np.exp(rng.uniform(np.log(lo), np.log(hi), n_req)).astype(np.int64)
+        objs = rng.integers(0, 256, n_req)
+        offs = (rng.random(n_req) * (pool - sizes + 1)).astype(np.int64)
+        arr = np.cumsum(rng.exponential(1.0 / rate, n_req)) if rate > 0 else np.zeros(n_req)
+        return sizes, objs, offs, (arr * 1e9).astype(np.int64)
+
+    def run(srv, sizes, objs, offs, arr_ns):
+        h = srv._h
+        n_req = sizes.size
+        ids = np.zeros(n_req, dtype=np.int64)
+        rid = ctypes.c_int64()
+        sub = L.gn_server_submit
+        now = L.gn_now_ns
+        t0 = now() + 2_000_000
+        for i in range(n_req):
+            due = t0 + int(arr_ns[i])
+            while now() < due:
+                pass
+            o, off = int(objs[i]), int(offs[i]) * 8
+            rc = sub(h, o, int(ptrs[o, 0]) + off, int(ptrs[o, 1]) + off, int(ptrs[o, 2]) + off, int(sizes[i]),
+                     ctypes.byref(rid))
+            if rc:
+                raise G.GnError(rc, L.gn_last_error().decode())
+            ids[i] = rid.value
+        info = G.RequestInfo()
+        done = np.zeros(n_req, dtype=np.int64)
+        subm = np.zeros(n_req, dtype=np.int64)
+        for i in range(n_req):
+            rc = L.gn_server_wait(h, int(ids[i]), 120000, ctypes.byref(info))
+            if rc:
+                raise G.GnError(rc, L.gn_last_error().decode())
+            done[i], subm[i] = info.done_ns, info.submit_ns
+            L.gn_server_release(h, int(ids[i]))
+        lat = (done - (t0 + arr_ns)) / 1e6
+        span = (done.max() - t0 - arr_ns.min()) / 1e9
+        return lat, (subm - (t0 + arr_ns)) / 1e6, span
+
+    mixed = dict(precision=G.GN_MIXED, newton=G.NEWTON_FP32 | G.NEWTON_FP64, fp32_iters=20, fp64_tail=3,
+                 tol_step32=1e-4, tol_step=1e-10, tol_quad=1e-6)
+    plain = dict(precision=G.GN_FP64, max_iter=20, tol_step=1e-10)
+    try:
+        for name, opts in (("mixed", mixed), ("plain_fp64", plain)):
+            if name not in modes:
+                continue
+            srv = G.Server(batch, curves, G.default_opts(**opts), hbm_budget=budget, streams=streams)
+            sizes, objs, offs, arr = trace(4000, 0, 77)
+            run(srv, sizes[:200], objs[:200], offs[:200], arr[:200])  # warm-up
+            lat, _, span = run(srv, sizes, objs, offs, arr)  # saturating burst: capacity
+            cap = float(sizes.sum()) / span
+            out = {"capacity_points_per_s": cap}
+            arena = srv.stats().arena_bytes
+            for load in loads:
+                rate = load * cap / mean_size
+                dur = min(duration, 0.6 * arena / (17.0 * load * cap))
+                n_req = max(200, int(rate * dur))
+                sizes, objs, offs, arr = trace(n_req, rate, int(1004 + 100 * load))
+                st0 = srv.stats()
+                lat, lag, span = run(srv, sizes, objs, offs, arr)
+                st1 = srv.stats()
+                mid = (sizes >= 5000) & (sizes <= 20000)
+                out[f"load_{int(load * 100)}"] = {
+                    "requests": int(n_req), "points": int(sizes.sum()),
+                    "offered_points_per_s": load * cap, "served_points_per_s": float(sizes.sum()) / span,
+                    "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+                    "p50_ms_10k": float(np.percentile(lat[mid], 50)) if mid.any() else None,
+                    "p99_ms_10k": float(np.percentile(lat[mid], 99)) if mid.any() else None,
+                    "max_ms": float(lat.max()), "submit_lag_p99_ms": float(np.percentile(lag, 99)),
+                    "steps": int(st1.steps - st0.steps), "max_step_requests": int(st1.max_step_requests),
+                }
+            out["arena_peak_bytes"] = int(srv.stats().arena_peak)
+            srv.close()
+            res[name] = out
+    finally:
+        sys.setswitchinterval(old_switch)
+        batch.close()
+        for o in surfs + curves:
+            o.close()
+        del pools
+        torch.cuda.empty_cache()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -174,6 +309,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-eval", action="store_true", help="skip the evaluation-at-scale measurement")
     ap.add_argument("--no-batch", action="store_true", help="skip the uneven-batch measurement")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving (latency) measurement")
     ap.add_argument("--cpu-points", type=int, default=1 << 18)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 512 MiB read + 512 MiB write scale kernel (reference for HBM "
@@ -336,6 +472,7 @@ def main():
 
     eval_res = None if args.no_eval else bench_eval(G, synth, torch, max(3, args.steps))
     batch_res = None if args.no_batch else bench_batch(G, synth, torch, max(3, args.steps))
+    serve_res = None if args.no_serve else bench_serve(G, synth, torch)
 
     mixed = res["mixed"]
     out = {
@@ -364,6 +501,7 @@ def main():
         "seed_ms": seed_ms,
         "eval_16M": eval_res,
         "uneven_batch": batch_res,
+        "serving": serve_res,
     }
     print(json.dumps(out))
 

@@ -168,6 +168,7 @@ __global__ __launch_bounds__(kInvBlock) void k_batch(BatchArgs a, unsigned long 
             }
         }
     }
+    if (lane == 0) retire_wave(counters, slices);
 }
 
 static void batch_shape(const void *kernel, int ntiles, size_t n, int bpc_cap, unsigned &grid,
@@ -198,6 +199,13 @@ static int launch_one(const BatchArgs &a, size_t n, int bpc_cap, int max_iter, i
                                   tol_quad);
     GN_HIP(hipGetLastError());
     return GN_OK;
+}
+
+int batch_waves(int bpc_cap) {
+    auto k = k_batch<float, true, kBulk>;
+    int bpc = blocks_per_cu((const void *)k, kInvBlock);
+    if (bpc_cap > 0 && bpc_cap < bpc) bpc = bpc_cap;
+    return cu_count() * bpc * (kInvBlock / 64);
 }
 
 int launch_batch_invert(const GnBatchObj *objs, const GnTile *tiles, int ntiles, const GnReq *reqs,

@@ -101,6 +101,22 @@ __global__ void k_curve_samples(GnCurveDev c, float *__restrict__ samp, double *
     samp_t[i] = t;
 }
 
+// Per span: the FP32 bounding box of its seed samples (lo xyz, hi xyz), stored after them.
+__global__ void k_curve_boxes(GnCurveDev c, float *__restrict__ box) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= c.ns) return;
+    for (int q = 0; q < 3; q++) {
+        float lo = INFINITY, hi = -INFINITY;
+        for (int j = 0; j < kCurveSamples; j++) {
+            const float v = c.samp[(s * kCurveSamples + j) * 3 + q];
+            lo = fminf(lo, v);
+            hi = fmaxf(hi, v);
+        }
+        box[s * 6 + q] = lo;
+        box[s * 6 + 3 + q] = hi;
+    }
+}
+
 template <int ORDER>
 __global__ void k_curve_eval(GnCurveDev c, const double *__restrict__ t, size_t n,
                              double *__restrict__ out) {
@@ -153,21 +169,55 @@ __device__ int curve_loop(const GnCurveDev &c, const double P[3], double &t, int
     return st;
 }
 
+__device__ __forceinline__ float d2_f32(const float *q, const float p[3]) {
+    const float dx = q[0] - p[0], dy = q[1] - p[1], dz = q[2] - p[2];
+    return fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+}
+
+// Lower bound of d2_f32 over every sample inside the box (same rounding and operation order, so
+// it never exceeds a sample's own distance).
+__device__ __forceinline__ float box_d2_f32(const float *b, const float p[3]) {
+    const float ex = fmaxf(fmaxf(b[0] - p[0], p[0] - b[3]), 0.f);
+    const float ey = fmaxf(fmaxf(b[1] - p[1], p[1] - b[4]), 0.f);
+    const float ez = fmaxf(fmaxf(b[2] - p[2], p[2] - b[5]), 0.f);
+    return fmaf(ez, ez, fmaf(ey, ey, ex * ex));
+}
+
+// Nearest seed sample (smallest FP32 squared distance, lowest index among equals -- the result
+// of a full scan): the samples of the span with the nearest box first, then every other span
+// whose box is not farther than the best sample so far.
+__device__ __forceinline__ int nearest_sample(const float *samp, const float *box, int ns,
+                                              const float p[3]) {
+    float bb = INFINITY;
+    int bs = 0;
+    for (int s = 0; s < ns; s++) {
+        const float b = box_d2_f32(box + 6 * s, p);
+        if (b < bb) { bb = b; bs = s; }
+    }
+    float best = INFINITY;
+    int bj = bs * kCurveSamples;
+    for (int j = bs * kCurveSamples; j < (bs + 1) * kCurveSamples; j++) {
+        const float d2 = d2_f32(samp + 3 * j, p);
+        if (d2 < best) { best = d2; bj = j; }
+    }
+    for (int s = 0; s < ns; s++) {
+        if (s == bs || box_d2_f32(box + 6 * s, p) > best) continue;
+        for (int j = s * kCurveSamples; j < (s + 1) * kCurveSamples; j++) {
+            const float d2 = d2_f32(samp + 3 * j, p);
+            if (d2 < best || (d2 == best && j < bj)) { best = d2; bj = j; }
+        }
+    }
+    return bj;
+}
+
 // Seed (nearest sample, or the given t) and the iterations of one point; returns the status.
-__device__ __forceinline__ int curve_invert_point(const GnCurveDev &c, const float *ssamp, int nsamp,
+// sdat: the curve's samples followed by its span boxes (LDS or global).
+__device__ __forceinline__ int curve_invert_point(const GnCurveDev &c, const float *sdat,
                                                   const double P[3], const gn_invert_opts &o,
                                                   double &t) {
     if (o.seed_mode != GN_SEED_GIVEN) {
         const float p[3] = {(float)P[0], (float)P[1], (float)P[2]};
-        float best = INFINITY;
-        int bj = 0;
-        for (int j = 0; j < nsamp; j++) {
-            const float dx = ssamp[3 * j] - p[0], dy = ssamp[3 * j + 1] - p[1],
-                        dz = ssamp[3 * j + 2] - p[2];
-            const float d2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-            if (d2 < best) { best = d2; bj = j; }
-        }
-        t = c.samp_t[bj];
+        t = c.samp_t[nearest_sample(sdat, sdat + 3 * kCurveSamples * c.ns, c.ns, p)];
     }
     const double ts = t;
     int it, st;
@@ -186,10 +236,14 @@ __device__ __forceinline__ int curve_invert_point(const GnCurveDev &c, const flo
                : curve_loop<double, false>(c, P, t, lim, o.tol_dist, o.tol_step, 0.0, o.clamp, it);
 }
 
-// Stages the curve's seed samples in LDS when use_lds is set; returns the array to read.
+// Floats of a curve's seed data: samples (3 per sample) then span boxes (6 per span).
+__host__ __device__ inline size_t seed_floats(int ns) { return (size_t)ns * (3 * kCurveSamples + 6); }
+
+// Stages the curve's seed data in LDS when use_lds is set; returns the array to read.
 __device__ __forceinline__ const float *stage_samples(const GnCurveDev &c, float *lds, int use_lds) {
     if (!use_lds) return c.samp;
-    for (int i = threadIdx.x; i < c.ns * kCurveSamples * 3; i += blockDim.x) lds[i] = c.samp[i];
+    const int nf = (int)seed_floats(c.ns);
+    for (int i = threadIdx.x; i < nf; i += blockDim.x) lds[i] = c.samp[i];
     __syncthreads();
     return lds;
 }
@@ -199,13 +253,13 @@ __global__ __launch_bounds__(256) void k_curve_invert(GnCurveDev c, const double
                                                       const double *__restrict__ z, size_t n,
                                                       gn_invert_opts o, double *__restrict__ t_out,
                                                       uint8_t *__restrict__ status, int use_lds) {
-    extern __shared__ float lds_samp[];  // [nsamp][3] when use_lds
+    extern __shared__ float lds_samp[];  // seed data when use_lds
     const float *ssamp = stage_samples(c, lds_samp, use_lds);
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (size_t)gridDim.x * blockDim.x) {
         const double P[3] = {x[i], y[i], z[i]};
         double t = o.seed_mode == GN_SEED_GIVEN ? t_out[i] : 0.0;
-        const int st = curve_invert_point(c, ssamp, c.ns * kCurveSamples, P, o, t);
+        const int st = curve_invert_point(c, ssamp, P, o, t);
         t_out[i] = t;
         status[i] = (uint8_t)st;
     }
@@ -228,7 +282,7 @@ __global__ __launch_bounds__(256) void k_curve_batch(const GnCurveDev *__restric
             const int64_t i = tl.start + j;
             const double P[3] = {r.x[i], r.y[i], r.z[i]};
             double t = o.seed_mode == GN_SEED_GIVEN ? r.u[i] : 0.0;
-            const int st = curve_invert_point(c, ssamp, c.ns * kCurveSamples, P, o, t);
+            const int st = curve_invert_point(c, ssamp, P, o, t);
             r.u[i] = t;
             r.status[i] = (uint8_t)st;
         }
@@ -238,7 +292,7 @@ __global__ __launch_bounds__(256) void k_curve_batch(const GnCurveDev *__restric
 int launch_curve_batch(const GnCurveDev *curves, const GnTile *tiles, int ntiles, const GnReq *reqs,
                        int max_samples, const gn_invert_opts &o, hipStream_t st) {
     if (ntiles <= 0) return GN_OK;
-    const size_t lds = sizeof(float) * 3 * (size_t)max_samples;
+    const size_t lds = sizeof(float) * seed_floats(max_samples / kCurveSamples);
     const int use_lds = lds <= 32 * 1024;
     const unsigned grid = (unsigned)(ntiles < 256 * 8 ? ntiles : 256 * 8);
     k_curve_batch<<<grid, 256, use_lds ? lds : 0, st>>>(curves, tiles, ntiles, reqs, o, use_lds);
@@ -304,7 +358,7 @@ int gn_curve_create(int degree, int cp, const double *knots, const double *xyz, 
     alloc((void **)&c->d_w, sizeof(double) * cp);
     alloc((void **)&c->d_tab64, sizeof(double) * tabn);
     alloc((void **)&c->d_tab32, sizeof(float) * tabn);
-    alloc((void **)&c->d_samp, sizeof(float) * 3 * nsamp);
+    alloc((void **)&c->d_samp, sizeof(float) * gn::seed_floats(ns));
     alloc((void **)&c->d_samp_t, sizeof(double) * nsamp);
     if (e == hipSuccess)
         e = hipMemcpyAsync(c->d_k, c->k.data(), sizeof(double) * c->k.size(), hipMemcpyHostToDevice, st);
@@ -317,6 +371,7 @@ int gn_curve_create(int degree, int cp, const double *knots, const double *xyz, 
         gn::k_curve_table<<<gn::grid_for(ns, 64, 1u << 20), 64, 0, st>>>(d, c->d_xyz, c->d_w, c->d_ih,
                                                                          c->d_tab64, c->d_tab32);
         gn::k_curve_samples<<<gn::grid_for(nsamp, 64, 1u << 20), 64, 0, st>>>(d, c->d_samp, c->d_samp_t);
+        gn::k_curve_boxes<<<gn::grid_for(ns, 64, 1u << 20), 64, 0, st>>>(d, c->d_samp + 3 * nsamp);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipStreamSynchronize(st);
@@ -374,8 +429,7 @@ int gn_curve_invert(const gn_curve *c, const double *x, const double *y, const d
         return fail(GN_EINVAL, "iteration counts must be 0..10000");
     if (n && (!x || !y || !z || !t_out || !status)) return fail(GN_EINVAL, "null buffer");
     if (!n) return GN_OK;
-    const size_t nsamp = (size_t)(c->n - c->p) * gn::kCurveSamples;
-    const size_t lds = sizeof(float) * 3 * nsamp;
+    const size_t lds = sizeof(float) * gn::seed_floats(c->n - c->p);
     const int use_lds = lds <= 32 * 1024;  // longer curves read their samples from L2
     const unsigned grid = gn::grid_for(n, 256, 256 * 8);
     gn::k_curve_invert<<<grid, 256, use_lds ? lds : 0, (hipStream_t)stream>>>(c->dev(), x, y, z, n, o,

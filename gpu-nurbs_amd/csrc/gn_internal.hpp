@@ -174,6 +174,8 @@ int launch_seed(const GnSearchDev &sd, int mode, const double *x, const double *
 // n: total points of the tiles (sizes the persistent grid)
 int launch_batch_invert(const GnBatchObj *objs, const GnTile *tiles, int ntiles, const GnReq *reqs,
                         size_t n, const gn_invert_opts &o, int bpc_cap, hipStream_t st);
+// resident waves of one persistent batch launch (sizes tiles for short serving steps)
+int batch_waves(int bpc_cap);
 // tiles of curve objects (obj indexes `curves`); max_samples: largest seed sample count
 int launch_curve_batch(const GnCurveDev *curves, const GnTile *tiles, int ntiles, const GnReq *reqs,
                        int max_samples, const gn_invert_opts &o, hipStream_t st);

@@ -12,8 +12,10 @@ constexpr int kStepContinue = -1;
 constexpr int kMaxSlices = 64;   // claim counters per launch
 constexpr int kCounterStride = 16;  // 16 x 8 bytes = one 128-byte line per counter
 
-// kMaxSlices zeroed counters (kCounterStride apart) for one launch on stream st
+// kMaxSlices zeroed counters (kCounterStride apart) for one launch on stream st.  The kernel
+// that uses them must end every wave with retire_wave, which leaves them zeroed again.
 int work_counters(hipStream_t st, unsigned long long **out);
+constexpr int kExitWord = 8;  // exit counter: word 8 of slice 0's line (slices use word 0)
 int cu_count();
 
 struct InvArgs {
@@ -24,6 +26,22 @@ struct InvArgs {
     uint8_t *it32, *it64;  // optional per-point iteration counts (FP32 bulk, FP64), or null
     gn_invert_opts o;
 };
+
+// Lane 0 of every wave calls this once, after the wave's last claim.  The exit count is a
+// release/acquire atomic, so the last wave of the grid to arrive sees every claim of the launch
+// done; it zeroes the claim counters and the exit count, which readies the slot for its next
+// launch without a separate clearing kernel (and keeps captured graphs replayable).
+__device__ __forceinline__ void retire_wave(unsigned long long *counters, int slices) {
+    const unsigned long long waves = (unsigned long long)gridDim.x * (blockDim.x / 64);
+    const unsigned long long prev = __hip_atomic_fetch_add(counters + kExitWord, 1ull, __ATOMIC_ACQ_REL,
+                                                           __HIP_MEMORY_SCOPE_AGENT);
+    if (prev + 1 == waves) {
+        for (int s = 0; s < slices; s++)
+            __hip_atomic_store(counters + (size_t)s * kCounterStride, 0ull, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(counters + kExitWord, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
 
 __device__ __forceinline__ uint8_t sat8(int x) { return (uint8_t)(x > 255 ? 255 : x); }
 
@@ -214,6 +232,7 @@ __global__ __launch_bounds__(kInvBlock) void k_gn_refill(GnSurfDev d, InvArgs a,
             }
         }
     }
+    if (lane == 0) retire_wave(counters, slices);
 }
 
 // Resident workgroups per CU of a kernel (runtime occupancy calculator, cached per kernel).

@@ -1,9 +1,10 @@
 // runtime.hip -- small per-device runtime services of libgnurbs:
 //   - claim counters for the persistent (lane-refill) kernels: a ring of slots per device, each
-//     slot kMaxSlices 64-bit counters on their own 128-byte lines; every launch takes the next
-//     slot and zeroes it with a small fill kernel on the caller's stream (a kernel, not a memset,
-//     so the sequence is also valid inside a captured hipGraph).  Two launches share a slot only
-//     if more than kCounterSlots launches are in flight at once.
+//     slot kMaxSlices 64-bit counters on their own 128-byte lines plus an exit counter; every
+//     launch takes the next slot.  Slots start zeroed and every kernel that uses one leaves it
+//     zeroed (the last wave to retire clears it), so a launch needs no clearing kernel or memset
+//     and a captured hipGraph can be replayed.  Two launches share a slot only if more than
+//     kCounterSlots launches are in flight at once.
 //   - the compute-unit count used to size persistent grids.
 #include <atomic>
 #include <mutex>
@@ -16,10 +17,6 @@ namespace gn {
 constexpr int kCounterSlots = 256;
 constexpr int kMaxDevices = 64;
 constexpr size_t kSlotWords = (size_t)kMaxSlices * kCounterStride;
-
-__global__ void k_zero_slot(unsigned long long *p) {
-    if (threadIdx.x < kMaxSlices) p[threadIdx.x * kCounterStride] = 0ull;
-}
 
 namespace {
 std::mutex g_mu;
@@ -49,10 +46,8 @@ int work_counters(hipStream_t st, unsigned long long **out) {
     int dev = 0;
     GN_HIP(hipGetDevice(&dev));
     const unsigned slot = g_next.fetch_add(1, std::memory_order_relaxed) % kCounterSlots;
-    unsigned long long *c = g_counters[dev] + (size_t)slot * kSlotWords;
-    k_zero_slot<<<1, 64, 0, st>>>(c);
-    GN_HIP(hipGetLastError());
-    *out = c;
+    (void)st;
+    *out = g_counters[dev] + (size_t)slot * kSlotWords;
     return GN_OK;
 }
 

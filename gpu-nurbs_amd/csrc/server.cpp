@@ -32,7 +32,7 @@ namespace {
 
 constexpr int kMaxStepReqs = 2048;     // requests per step
 constexpr int kMaxStepTiles = 32768;   // tiles per step (surfaces + curves)
-constexpr int kCurveTile = 1024;       // points per curve tile (one workgroup)
+constexpr int kCurveTile = 256;        // points per curve tile (one workgroup, a point per lane)
 
 enum ReqState { kWaiting, kAdmitted, kRunning, kDone, kFailed };
 
@@ -80,7 +80,7 @@ struct gn_server {
     int nsurf = 0, ncurve = 0;
     std::vector<int> cls;          // degree class per surface
     GnBatchObj *d_objs = nullptr;  // the batch's object table
-    int tile = 1024, bpc = 0;
+    int tile = 1024, bpc = 0, waves = 1;
     GnCurveDev *d_curves = nullptr;
     int max_samples = 0;
     gn_invert_opts o{};
@@ -140,9 +140,12 @@ int gn_server::launch_step(Slot &sl) {
     const int nreq = (int)sl.reqs.size();
     size_t spts = 0, cpts = 0;
     for (Req *r : sl.reqs) (r->curve ? cpts : spts) += r->n;
-    // tile size that keeps the step within kMaxStepTiles (sum of ceil(n/T) <= pts/T + nreq)
+    // Surface tile size: the batch's tile, shrunk (to a multiple of 64, at least 64) so that a
+    // short step still gives every resident wave about two tiles; grown if the step would exceed
+    // kMaxStepTiles (sum of ceil(n/T) <= pts/T + nreq).
     const size_t room = (size_t)(kMaxStepTiles - 2 * nreq);
-    size_t ts = (size_t)tile;
+    size_t ts = std::min<size_t>((size_t)tile, spts / (2 * (size_t)waves) / 64 * 64);
+    if (ts < 64) ts = 64;
     if (spts / ts > room / 2) ts = (spts + room / 2 - 1) / (room / 2);
     size_t tc = kCurveTile;
     if (cpts / tc > room / 2) tc = (cpts + room / 2 - 1) / (room / 2);
@@ -162,13 +165,15 @@ int gn_server::launch_step(Slot &sl) {
                      [this](const GnTile &a, const GnTile &b) { return cls[a.obj] < cls[b.obj]; });
     const size_t ns = st_tiles.size(), nc = ct_tiles.size();
     if (ns + nc > (size_t)kMaxStepTiles) return fail(GN_EDEVICE, "step tile list overflow");
-    GnTile *tl = (GnTile *)(sl.h_stage + kStageReqs);
+    // staging image: request table, then the tiles (256-byte aligned); one copy of what is used
+    const size_t toff = a256(sizeof(GnReq) * nreq);
+    GnTile *tl = (GnTile *)(sl.h_stage + toff);
     if (ns) std::memcpy(tl, st_tiles.data(), sizeof(GnTile) * ns);
     if (nc) std::memcpy(tl + ns, ct_tiles.data(), sizeof(GnTile) * nc);
-    const size_t bytes = kStageReqs + sizeof(GnTile) * (ns + nc);
+    const size_t bytes = toff + sizeof(GnTile) * (ns + nc);
     GN_HIP(hipMemcpyAsync(sl.d_stage, sl.h_stage, bytes, hipMemcpyHostToDevice, sl.st));
     const GnReq *d_rq = (const GnReq *)sl.d_stage;
-    const GnTile *d_tl = (const GnTile *)(sl.d_stage + kStageReqs);
+    const GnTile *d_tl = (const GnTile *)(sl.d_stage + toff);
     int rc = GN_OK;
     if (ns) rc = gn::launch_batch_invert(d_objs, d_tl, (int)ns, d_rq, spts, o, bpc, sl.st);
     if (!rc && nc) rc = gn::launch_curve_batch(d_curves, d_tl + ns, (int)nc, d_rq, max_samples, o, sl.st);
@@ -308,6 +313,7 @@ int gn_server_create(gn_batch *batch, gn_curve *const *curves, int ncurves,
         }
         s->batch = batch;
         s->d_objs = const_cast<GnBatchObj *>(objs);
+        s->waves = std::max(1, gn::batch_waves(s->bpc));
     }
     s->ncurve = ncurves;
     size_t curve_bytes = 0;

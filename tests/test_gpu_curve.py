@@ -125,3 +125,13 @@ def test_curve_errors():
     # empty input is a no-op
     t, st = g.Invert(np.zeros(0), np.zeros(0), np.zeros(0))
     assert t.numel() == 0
+
+
+def test_long_curve_seeds_from_global_memory():
+    # 400 spans: the seed data (48 KB) does not fit the LDS staging limit, samples stay in L2
+    cd = random_def(403, 3, 808)
+    g, o = pair(cd)
+    rng = np.random.default_rng(5)
+    P = o.eval(rng.uniform(0, 1, 3000)) + rng.normal(0, 0.05, (3, 3000))
+    t, _ = g.Invert(*P, G.default_opts(max_iter=0))
+    assert (t.cpu().numpy() == o.seed(*P)).mean() > 0.999
