@@ -34,7 +34,7 @@ __device__ __forceinline__ int step_cls(int cls, const GnSurfDev &d, T Px, T Py,
 #define GN_CLS(P, Q)                                                                            \
     case (P - 2) * 4 + (Q - 2):                                                                 \
         return gn_step<P + 1, Q + 1, true, T, NEWTON>(d, d.ku, d.kv, Px, Py, Pz, cu, cv, tol_dist, \
-                                                      tol_step, tol_quad, clamp, sc);
+                                                      tol_step, tol_quad, clamp, domain_of(d), sc);
         GN_CLS(2, 2) GN_CLS(2, 3) GN_CLS(2, 4) GN_CLS(2, 5)
         GN_CLS(3, 2) GN_CLS(3, 3) GN_CLS(3, 4) GN_CLS(3, 5)
         GN_CLS(4, 2) GN_CLS(4, 3) GN_CLS(4, 4) GN_CLS(4, 5)
@@ -43,7 +43,7 @@ __device__ __forceinline__ int step_cls(int cls, const GnSurfDev &d, T Px, T Py,
         default:
             return gn_step<GN_MAXK, GN_MAXK, false, T, NEWTON>(d, d.ku, d.kv, Px, Py, Pz, cu, cv,
                                                                tol_dist, tol_step, tol_quad, clamp,
-                                                               sc);
+                                                               domain_of(d), sc);
     }
 }
 

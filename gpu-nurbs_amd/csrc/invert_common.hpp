@@ -27,15 +27,19 @@ struct InvArgs {
     gn_invert_opts o;
 };
 
-// Lane 0 of every wave calls this once, after the wave's last claim.  The exit count is a
-// release/acquire atomic, so the last wave of the grid to arrive sees every claim of the launch
-// done; it zeroes the claim counters and the exit count, which readies the slot for its next
-// launch without a separate clearing kernel (and keeps captured graphs replayable).
+// Lane 0 of every wave calls this once, after the wave's last claim.  A wave has consumed the
+// value returned by each of its claim atomics before it gets here, so its claims are performed
+// before it counts itself out; the count itself needs no release ordering (an agent-scope
+// release would write back the XCD's L2 once per wave).  The last wave of the grid to arrive
+// takes an acquire fence and zeroes the claim counters and the exit count, which readies the
+// slot for its next launch without a separate clearing kernel (and keeps captured graphs
+// replayable).
 __device__ __forceinline__ void retire_wave(unsigned long long *counters, int slices) {
     const unsigned long long waves = (unsigned long long)gridDim.x * (blockDim.x / 64);
-    const unsigned long long prev = __hip_atomic_fetch_add(counters + kExitWord, 1ull, __ATOMIC_ACQ_REL,
+    const unsigned long long prev = __hip_atomic_fetch_add(counters + kExitWord, 1ull, __ATOMIC_RELAXED,
                                                            __HIP_MEMORY_SCOPE_AGENT);
     if (prev + 1 == waves) {
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         for (int s = 0; s < slices; s++)
             __hip_atomic_store(counters + (size_t)s * kCounterStride, 0ull, __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT);
@@ -45,18 +49,26 @@ __device__ __forceinline__ void retire_wave(unsigned long long *counters, int sl
 
 __device__ __forceinline__ uint8_t sat8(int x) { return (uint8_t)(x > 255 ? 255 : x); }
 
-__device__ __forceinline__ void stage_knots_inv(const GnSurfDev &d, double *sk, const double *&ku,
-                                                const double *&kv) {
+// Knots of the exact-degree inversion kernels live in LDS (the launchers route surfaces whose
+// knots do not fit to the generic kernel, which reads them from global memory), so the span
+// search and the span cache run on LDS loads.
+inline bool knots_fit_lds(const GnSurfDev &d) {
+    return d.nu + d.p + 1 + d.nv + d.q + 1 <= kLdsKnotsInv;
+}
+
+__device__ __forceinline__ void stage_knots_lds(const GnSurfDev &d, double *sk) {
     const int nku = d.nu + d.p + 1, nkv = d.nv + d.q + 1;
-    ku = d.ku;
-    kv = d.kv;
-    if (nku + nkv <= kLdsKnotsInv) {
-        for (int i = threadIdx.x; i < nku; i += blockDim.x) sk[i] = d.ku[i];
-        for (int i = threadIdx.x; i < nkv; i += blockDim.x) sk[nku + i] = d.kv[i];
-        __syncthreads();
-        ku = sk;
-        kv = sk + nku;
-    }
+    for (int i = threadIdx.x; i < nku; i += blockDim.x) sk[i] = d.ku[i];
+    for (int i = threadIdx.x; i < nkv; i += blockDim.x) sk[nku + i] = d.kv[i];
+    __syncthreads();
+}
+
+// The degree dispatch of the inversion launchers: exact instantiations when the surface's knots
+// fit in LDS, the generic (global-knot) kernel otherwise.
+template <typename F>
+__host__ int dispatch_inv(const GnSurfDev &d, F &&f) {
+    if (!knots_fit_lds(d)) return f.template operator()<GN_MAXK, GN_MAXK, false>();
+    return dispatch_degrees(d.p, d.q, f);
 }
 
 // One Gauss-Newton update in arithmetic T on the parameter state (cu, cv) (always FP64).
@@ -65,11 +77,21 @@ __device__ __forceinline__ void stage_knots_inv(const GnSurfDev &d, double *sk, 
 // the update), GN_PT_SINGULAR (det(J^T J) == 0) or GN_PT_NONFINITE.
 // With NEWTON the residual-curvature term sum_k r_k d2S_k is added to J^T J when the sum stays
 // positive definite (exact Hessian of |S - P|^2 / 2), otherwise the plain J^T J is used.
+// Parameter domain [u0, u1] x [v0, v1] of a surface: k[p], k[nu] and k[q], k[nv].
+struct Domain {
+    double u0, u1, v0, v1;
+};
+
+__device__ __forceinline__ Domain domain_of(const GnSurfDev &d) {
+    return Domain{d.ku[d.p], d.ku[d.nu], d.kv[d.q], d.kv[d.nv]};
+}
+
 template <int K1, int K2, bool EXACT, typename T, bool NEWTON>
 __device__ __forceinline__ int gn_step(const GnSurfDev &d, const double *__restrict__ ku,
                                        const double *__restrict__ kv, T Px, T Py, T Pz,
                                        double &cu, double &cv, double tol_dist, double tol_step,
-                                       double tol_quad, int clamp, SpanCache &sc) {
+                                       double tol_quad, int clamp, const Domain &dom,
+                                       SpanCache &sc) {
     Cart<T> o;
     eval_point_cached<K1, K2, NEWTON ? 2 : 1, EXACT, T>(d, ku, kv, cu, cv, sc, o);
     const T r0 = o.S[0] - Px, r1 = o.S[1] - Py, r2 = o.S[2] - Pz;
@@ -96,8 +118,8 @@ __device__ __forceinline__ int gn_step(const GnSurfDev &d, const double *__restr
     cu -= double(du);
     cv -= double(dv);
     if (clamp) {
-        cu = fmin(fmax(cu, ku[d.p]), ku[d.nu]);
-        cv = fmin(fmax(cv, kv[d.q]), kv[d.nv]);
+        cu = fmin(fmax(cu, dom.u0), dom.u1);
+        cv = fmin(fmax(cv, dom.v0), dom.v1);
     }
     if (!(cu == cu) || !(cv == cv)) return GN_PT_NONFINITE;
     if (tol_step > 0 && fabs(du) < T(tol_step) && fabs(dv) < T(tol_step)) return GN_PT_CONVERGED;
@@ -115,9 +137,11 @@ __device__ __forceinline__ int gn_loop(const GnSurfDev &d, const double *__restr
                                        double tol_quad = 0.0) {
     int it = 0, st = GN_PT_MAXITER;
     SpanCache sc;
+    const Domain dom = domain_of(d);
     while (it < max_iter) {
         const int s = gn_step<K1, K2, EXACT, T, NEWTON>(d, ku, kv, T(P[0]), T(P[1]), T(P[2]), cu,
-                                                        cv, tol_dist, tol_step, tol_quad, clamp, sc);
+                                                        cv, tol_dist, tol_step, tol_quad, clamp,
+                                                        dom, sc);
         if (s == kStepContinue || s == GN_PT_CONVERGED) it++;
         if (s != kStepContinue) { st = s; break; }
     }
@@ -152,9 +176,14 @@ __global__ __launch_bounds__(kInvBlock) void k_gn_refill(GnSurfDev d, InvArgs a,
                                                          int max_iter, int restart_iter,
                                                          double tol_dist, double tol_step,
                                                          double tol_quad) {
-    __shared__ double sk[kLdsKnotsInv];
-    const double *ku, *kv;
-    stage_knots_inv(d, sk, ku, kv);
+    __shared__ double sk[EXACT ? kLdsKnotsInv : 1];
+    const double *ku = d.ku, *kv = d.kv;
+    if constexpr (EXACT) {
+        stage_knots_lds(d, sk);
+        ku = sk;
+        kv = sk + d.nu + d.p + 1;
+    }
+    const Domain dom = domain_of(d);
     const int lane = threadIdx.x & 63;
     const unsigned long long lt = (1ull << lane) - 1ull;
     const int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
@@ -210,7 +239,7 @@ __global__ __launch_bounds__(kInvBlock) void k_gn_refill(GnSurfDev d, InvArgs a,
                 st = GN_PT_MAXITER;
             } else {
                 st = gn_step<K1, K2, EXACT, T, NEWTON>(d, ku, kv, Px, Py, Pz, cu, cv, tol_dist,
-                                                       tol_step, tol_quad, a.o.clamp, sc);
+                                                       tol_step, tol_quad, a.o.clamp, dom, sc);
                 if (st == kStepContinue || st == GN_PT_CONVERGED) it++;
                 if (st == kStepContinue && it >= lim) st = GN_PT_MAXITER;
             }

@@ -32,7 +32,7 @@ struct PlainLaunch {
 };
 
 int launch_gn_plain(const GnSurfDev &d, const InvArgs &a, hipStream_t st) {
-    return dispatch_degrees(d.p, d.q, PlainLaunch{d, a, st});
+    return dispatch_inv(d, PlainLaunch{d, a, st});
 }
 
 }  // namespace gn

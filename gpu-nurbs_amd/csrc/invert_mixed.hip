@@ -45,7 +45,7 @@ struct MixedLaunch {
 };
 
 int launch_gn_mixed(const GnSurfDev &d, const InvArgs &a, hipStream_t st) {
-    return dispatch_degrees(d.p, d.q, MixedLaunch{d, a, st});
+    return dispatch_inv(d, MixedLaunch{d, a, st});
 }
 
 }  // namespace gn

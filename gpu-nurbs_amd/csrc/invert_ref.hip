@@ -10,9 +10,13 @@ __global__ __launch_bounds__(kInvBlock) void k_invert_first_hit(GnSurfDev d, Inv
                                                                 const GnTreeNode *__restrict__ tree,
                                                                 const uint8_t *__restrict__ valid,
                                                                 long count) {
-    __shared__ double sk[kLdsKnotsInv];
-    const double *ku, *kv;
-    stage_knots_inv(d, sk, ku, kv);
+    __shared__ double sk[EXACT ? kLdsKnotsInv : 1];
+    const double *ku = d.ku, *kv = d.kv;
+    if constexpr (EXACT) {
+        stage_knots_lds(d, sk);
+        ku = sk;
+        kv = sk + d.nu + d.p + 1;
+    }
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.n) return;
     const double P[3] = {a.x[i], a.y[i], a.z[i]};
@@ -58,7 +62,7 @@ struct FirstHitLaunch {
 
 int launch_gn_first_hit(const GnSurfDev &d, const InvArgs &a, const GnTreeNode *tree,
                         const uint8_t *valid, long count, hipStream_t st) {
-    return dispatch_degrees(d.p, d.q, FirstHitLaunch{d, a, tree, valid, count, st});
+    return dispatch_inv(d, FirstHitLaunch{d, a, tree, valid, count, st});
 }
 
 }  // namespace gn
