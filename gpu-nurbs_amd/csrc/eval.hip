@@ -3,39 +3,22 @@
 //
 // One lane per query point, grid-stride over the batch.  Parameters stream in and results stream
 // out as SoA arrays, so every global load and store of the batch is a fully coalesced 256-byte
-// (FP32) or 512-byte (FP64) wave access.  Knots are staged in LDS for the span search.  Span
-// patches come from the L2-resident span table: staged through LDS by cooperative contiguous
-// loads for patches up to 512 bytes (k_eval_lds), gathered per lane otherwise (k_eval).  The
-// target roof is HBM traffic (8 or 16 bytes in, 12..144 bytes out per point).
+// (FP32) or 512-byte (FP64) wave access.  Span patches come from the L2-resident span table:
+// staged through LDS by cooperative contiguous loads for patches up to 512 bytes (k_eval_lds,
+// whose knots also live in LDS for the span search), gathered per lane otherwise (k_eval, knots
+// read through L1).  The target roof is HBM traffic (8 or 16 bytes in, 12..144 bytes out per
+// point).
 #include "gn_device.hpp"
 
 namespace gn {
 
 constexpr int kEvalBlock = 256;
-constexpr int kLdsKnots = 2048;  // knots (both directions) staged in LDS when they fit
-
-// Stages both knot vectors in LDS when they fit and returns the pointers to use.
-__device__ __forceinline__ void stage_knots(const GnSurfDev &d, double *sk, const double *&ku,
-                                            const double *&kv) {
-    const int nku = d.nu + d.p + 1, nkv = d.nv + d.q + 1;
-    ku = d.ku;
-    kv = d.kv;
-    if (nku + nkv <= kLdsKnots) {
-        for (int i = threadIdx.x; i < nku; i += blockDim.x) sk[i] = d.ku[i];
-        for (int i = threadIdx.x; i < nkv; i += blockDim.x) sk[nku + i] = d.kv[i];
-        __syncthreads();
-        ku = sk;
-        kv = sk + nku;
-    }
-}
 
 template <int K1, int K2, bool EXACT, int ORDER, typename T, typename TP>
 __global__ __launch_bounds__(kEvalBlock) void k_eval(GnSurfDev d, const TP *__restrict__ u,
                                                      const TP *__restrict__ v, size_t n,
                                                      T *__restrict__ out) {
-    __shared__ double sk[kLdsKnots];
-    const double *ku, *kv;
-    stage_knots(d, sk, ku, kv);
+    const double *ku = d.ku, *kv = d.kv;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (size_t)gridDim.x * blockDim.x) {
         Cart<T> o;
@@ -78,17 +61,11 @@ __global__ __launch_bounds__(kLdsBlock) void k_eval_lds(GnSurfDev d, const TP *_
     static_assert(kVec * 16 <= 512, "patch too large for LDS staging");
     __shared__ double sk[kLdsKnotsSmall];
     __shared__ float4 stage[kLdsWaves * 64 * kStride];
-    const double *ku = d.ku, *kv = d.kv;
-    {
-        const int nku = d.nu + d.p + 1, nkv = d.nv + d.q + 1;
-        if (nku + nkv <= kLdsKnotsSmall) {
-            for (int i = threadIdx.x; i < nku; i += blockDim.x) sk[i] = d.ku[i];
-            for (int i = threadIdx.x; i < nkv; i += blockDim.x) sk[nku + i] = d.kv[i];
-            __syncthreads();
-            ku = sk;
-            kv = sk + nku;
-        }
-    }
+    const int nku = d.nu + d.p + 1, nkv = d.nv + d.q + 1;  // nku + nkv <= kLdsKnotsSmall
+    for (int i = threadIdx.x; i < nku; i += blockDim.x) sk[i] = d.ku[i];
+    for (int i = threadIdx.x; i < nkv; i += blockDim.x) sk[nku + i] = d.kv[i];
+    __syncthreads();
+    const double *ku = sk, *kv = sk + nku;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     float4 *my = stage + (size_t)wv * 64 * kStride;
     const float4 *tab = reinterpret_cast<const float4 *>(sizeof(T) == 8 ? (const void *)d.tab64
@@ -161,7 +138,8 @@ struct EvalLaunch {
     template <int K1, int K2, bool EXACT>
     int operator()() {
         const unsigned grid = grid_for(n, kEvalBlock, 256 * 16);
-        if constexpr (EXACT && K1 * K2 * 4 * sizeof(T) <= 512) {
+        const bool knots_fit = d.nu + d.p + 1 + d.nv + d.q + 1 <= kLdsKnotsSmall;
+        if constexpr (EXACT && K1 * K2 * 4 * sizeof(T) <= 512) if (knots_fit) {
             const unsigned g2 = grid_for(n, kLdsBlock, 256 * 32);
             switch (order) {
                 case 0: k_eval_lds<K1, K2, 0, T, TP><<<g2, kLdsBlock, 0, st>>>(d, u, v, n, out); break;

@@ -68,9 +68,9 @@ __device__ __forceinline__ Hom<double> load_hom<double>(const double *__restrict
 // ORDER (MatrixSurfacePoint / MatrixSurfaceRd*, src/NURBS/Evaluate.cu:159-502).  The loop runs
 // over s outermost so that only the 4-vectors of one patch column are live at a time.
 // K1/K2 are compile-time bounds; k1/k2 the runtime sizes (equal to K1/K2 when EXACT).
-template <int K1, int K2, int ORDER, bool EXACT, typename T>
-__device__ __forceinline__ void contract(const T *__restrict__ patch, int k1, int k2, T t, T s,
-                                         HomDer<T> &o) {
+template <int K1, int K2, int ORDER, bool EXACT, typename T, typename LOAD>
+__device__ __forceinline__ void contract_with(LOAD &&entry, int k1, int k2, T t, T s,
+                                              HomDer<T> &o) {
     if (EXACT) { k1 = K1; k2 = K2; }
     T U[K1], dU[K1], ddU[K1], V[K2], dV[K2], ddV[K2];
     {
@@ -105,7 +105,7 @@ __device__ __forceinline__ void contract(const T *__restrict__ patch, int k1, in
 #pragma unroll
         for (int r = 0; r < K1; r++) {
             if (!EXACT && r >= k1) break;
-            Hom<T> m = load_hom<T>(patch + (r * k2 + c) * 4);
+            const Hom<T> m = entry(r, c);
             hfma(a, U[r], m);
             if (ORDER >= 1 && r >= 1) hfma(a1, dU[r], m);
             if (ORDER >= 2 && r >= 2) hfma(a2, ddU[r], m);
@@ -122,6 +122,21 @@ __device__ __forceinline__ void contract(const T *__restrict__ patch, int k1, in
         }
     }
 }
+
+template <int K1, int K2, int ORDER, bool EXACT, typename T>
+__device__ __forceinline__ void contract(const T *__restrict__ patch, int k1, int k2, T t, T s,
+                                         HomDer<T> &o) {
+    const int kk2 = EXACT ? K2 : k2;
+    contract_with<K1, K2, ORDER, EXACT, T>(
+        [&](int r, int c) { return load_hom<T>(patch + (r * kk2 + c) * 4); }, k1, k2, t, s, o);
+}
+
+// A lane's current span patch held in registers (exact sizes only): reloaded from the table only
+// when the lane's span pair changes, so iterations that stay in their span issue no gathers.
+template <typename T, int N>
+struct PatchRegs {
+    Hom<T> m[N];
+};
 
 template <typename T>
 struct Cart {  // Cartesian point and derivatives in the global parameters
@@ -201,16 +216,19 @@ struct SpanCache {
     int im = 0, in = 0;  // span slots m - p, n - q
 };
 
-template <int K1, int K2, int ORDER, bool EXACT, typename T>
+template <int K1, int K2, int ORDER, bool EXACT, typename T, bool PREG = false>
 __device__ __forceinline__ void eval_point_cached(const GnSurfDev &d, const double *__restrict__ ku,
                                                   const double *__restrict__ kv, double u, double v,
-                                                  SpanCache &sc, Cart<T> &o) {
+                                                  SpanCache &sc, Cart<T> &o,
+                                                  PatchRegs<T, K1 * K2> *pr = nullptr) {
+    bool moved = false;
     if (!(u >= sc.ulo && u < sc.uhi)) {
         const int m = find_span(ku, d.p, d.nu, u);
         sc.ulo = ku[m];
         sc.uhi = ku[m + 1];
         sc.im = m - d.p;
         sc.ihu = d.ihu[sc.im];
+        moved = true;
     }
     if (!(v >= sc.vlo && v < sc.vhi)) {
         const int n = find_span(kv, d.q, d.nv, v);
@@ -218,6 +236,7 @@ __device__ __forceinline__ void eval_point_cached(const GnSurfDev &d, const doub
         sc.vhi = kv[n + 1];
         sc.in = n - d.q;
         sc.ihv = d.ihv[sc.in];
+        moved = true;
     }
     const T t = T((u - sc.ulo) * sc.ihu);
     const T s = T((v - sc.vlo) * sc.ihv);
@@ -227,7 +246,18 @@ __device__ __forceinline__ void eval_point_cached(const GnSurfDev &d, const doub
     else tab = reinterpret_cast<const T *>(d.tab32);
     const T *patch = tab + (size_t)(sc.im * d.sv + sc.in) * (size_t)(k1 * k2 * 4);
     HomDer<T> h;
-    contract<K1, K2, ORDER, EXACT, T>(patch, k1, k2, t, s, h);
+    if constexpr (PREG) {
+        static_assert(EXACT, "register patch cache needs exact sizes");
+        if (moved) {
+#pragma unroll
+            for (int e = 0; e < K1 * K2; e++) pr->m[e] = load_hom<T>(patch + e * 4);
+        }
+        contract_with<K1, K2, ORDER, true, T>([&](int r, int c) { return pr->m[r * K2 + c]; }, k1,
+                                              k2, t, s, h);
+    } else {
+        (void)moved;
+        contract<K1, K2, ORDER, EXACT, T>(patch, k1, k2, t, s, h);
+    }
     quotient<ORDER, T>(h, T(sc.ihu), T(sc.ihv), o);
 }
 

@@ -8,10 +8,17 @@ namespace gn {
 
 __host__ __device__ inline long qoff(int l) { return ((1L << (2 * l)) - 1) / 3; }
 
-__device__ inline uint32_t morton2(uint32_t x, uint32_t y) {  // x bits odd, y bits even
-    uint32_t c = 0;
-    for (int b = 0; b < 16; b++) c |= ((x >> b) & 1u) << (2 * b + 1) | ((y >> b) & 1u) << (2 * b);
-    return c;
+__device__ __forceinline__ uint32_t spread2(uint32_t x) {  // bit b of x -> bit 2b (16-bit x)
+    x &= 0xffffu;
+    x = (x | (x << 8)) & 0x00ff00ffu;
+    x = (x | (x << 4)) & 0x0f0f0f0fu;
+    x = (x | (x << 2)) & 0x33333333u;
+    x = (x | (x << 1)) & 0x55555555u;
+    return x;
+}
+
+__device__ __forceinline__ uint32_t morton2(uint32_t x, uint32_t y) {  // x bits odd, y bits even
+    return (spread2(x) << 1) | spread2(y);
 }
 
 // Squared distance from p to an FP32 box (0 inside); a lower bound of the FP64 distance from P to
@@ -84,31 +91,44 @@ __device__ inline uint32_t nearest_leaf(const GnSearchDev &sd, const double P[3]
     return best_c;
 }
 
-__device__ inline uint32_t compact2(uint32_t c) {  // inverse of the interleave, even bits
-    uint32_t r = 0;
-    for (int b = 0; b < 16; b++) r |= ((c >> (2 * b)) & 1u) << b;
-    return r;
+__device__ __forceinline__ uint32_t compact2(uint32_t c) {  // even bits of c -> bits 0..15
+    c &= 0x55555555u;
+    c = (c | (c >> 1)) & 0x33333333u;
+    c = (c | (c >> 2)) & 0x0f0f0f0fu;
+    c = (c | (c >> 4)) & 0x00ff00ffu;
+    c = (c | (c >> 8)) & 0x0000ffffu;
+    return c;
+}
+
+// One level of the greedy descent: the child of node c (level l) whose box is nearest to p
+// (ties to the lower child); `level` points at the boxes of level l + 1.
+__device__ __forceinline__ uint32_t greedy_child(const float *__restrict__ level, uint32_t c,
+                                                 const float p[3]) {
+    float bd = INFINITY;
+    uint32_t bc = 4 * c;
+    const float *b = level + 4 * c * 6;
+#pragma unroll
+    for (int a = 0; a < 4; a++) {
+        const float dd = box_d2(b + a * 6, p);
+        if (dd < bd) { bd = dd; bc = 4 * c + a; }
+    }
+    return bc;
 }
 
 // Fast seed: greedy descent by FP32 box distance (ties to the lower child), then the nearest
 // leaf centre (FP64, ties to the lower reference index) among the 3 x 3 leaves around the
 // descent's leaf in the uv grid.  Fixed work per point: 4*D box tests and 9 distances.
-__device__ inline uint32_t greedy_leaf(const GnSearchDev &sd, const double P[3]) {
+// lds, when given, holds the FP32 boxes of levels 1 .. lds_levels (the first part of box32 after
+// the root), so those levels of the descent read LDS.
+__device__ inline uint32_t greedy_leaf(const GnSearchDev &sd, const double P[3],
+                                       const float *lds = nullptr, int lds_levels = 0) {
     const int D = sd.depth;
     if (D == 0) return 0;
     const float p[3] = {(float)P[0], (float)P[1], (float)P[2]};
     uint32_t c = 0;
-    for (int l = 0; l < D; l++) {
-        float bd = INFINITY;
-        uint32_t bc = 4 * c;
-        const float *b = sd.box32 + (qoff(l + 1) + 4 * c) * 6;
-#pragma unroll
-        for (int a = 0; a < 4; a++) {
-            const float dd = box_d2(b + a * 6, p);
-            if (dd < bd) { bd = dd; bc = 4 * c + a; }
-        }
-        c = bc;
-    }
+    int l = 0;
+    for (; l < lds_levels && l < D; l++) c = greedy_child(lds + (qoff(l + 1) - 1) * 6, c, p);
+    for (; l < D; l++) c = greedy_child(sd.box32 + qoff(l + 1) * 6, c, p);
     const int side = 1 << D;
     const int gx = (int)compact2(c >> 1), gy = (int)compact2(c);
     double best = INFINITY;

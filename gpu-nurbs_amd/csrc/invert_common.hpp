@@ -86,14 +86,14 @@ __device__ __forceinline__ Domain domain_of(const GnSurfDev &d) {
     return Domain{d.ku[d.p], d.ku[d.nu], d.kv[d.q], d.kv[d.nv]};
 }
 
-template <int K1, int K2, bool EXACT, typename T, bool NEWTON>
+template <int K1, int K2, bool EXACT, typename T, bool NEWTON, bool PREG = false>
 __device__ __forceinline__ int gn_step(const GnSurfDev &d, const double *__restrict__ ku,
                                        const double *__restrict__ kv, T Px, T Py, T Pz,
                                        double &cu, double &cv, double tol_dist, double tol_step,
                                        double tol_quad, int clamp, const Domain &dom,
-                                       SpanCache &sc) {
+                                       SpanCache &sc, PatchRegs<T, K1 * K2> *pr = nullptr) {
     Cart<T> o;
-    eval_point_cached<K1, K2, NEWTON ? 2 : 1, EXACT, T>(d, ku, kv, cu, cv, sc, o);
+    eval_point_cached<K1, K2, NEWTON ? 2 : 1, EXACT, T, PREG>(d, ku, kv, cu, cv, sc, o, pr);
     const T r0 = o.S[0] - Px, r1 = o.S[1] - Py, r2 = o.S[2] - Pz;
     const T rr = r0 * r0 + r1 * r1 + r2 * r2;
     if (!(rr == rr)) return GN_PT_NONFINITE;
@@ -171,7 +171,7 @@ __device__ __forceinline__ unsigned long long bcast64(unsigned long long x, int 
 enum { kPlain = 0, kBulk = 1, kClose = 2 };
 
 template <int K1, int K2, bool EXACT, typename T, bool NEWTON, int MODE>
-__global__ __launch_bounds__(kInvBlock) void k_gn_refill(GnSurfDev d, InvArgs a,
+__global__ __launch_bounds__(kInvBlock, (EXACT && sizeof(T) == 4 && K1 * K2 <= 16) ? 3 : 1) void k_gn_refill(GnSurfDev d, InvArgs a,
                                                          unsigned long long *counters, int slices,
                                                          int max_iter, int restart_iter,
                                                          double tol_dist, double tol_step,
@@ -198,6 +198,9 @@ __global__ __launch_bounds__(kInvBlock) void k_gn_refill(GnSurfDev d, InvArgs a,
     T Px = 0, Py = 0, Pz = 0;
     int it = 0, lim = max_iter;
     SpanCache sc;
+    // FP32 exact-degree kernels keep the lane's span patch in registers between iterations
+    constexpr bool PREG = EXACT && sizeof(T) == 4 && K1 * K2 <= 16;
+    PatchRegs<T, K1 * K2> pr;
     while (true) {
         while (!exhausted) {
             const unsigned long long idle = __ballot(idx < 0);
@@ -238,8 +241,9 @@ __global__ __launch_bounds__(kInvBlock) void k_gn_refill(GnSurfDev d, InvArgs a,
             if (it >= lim) {
                 st = GN_PT_MAXITER;
             } else {
-                st = gn_step<K1, K2, EXACT, T, NEWTON>(d, ku, kv, Px, Py, Pz, cu, cv, tol_dist,
-                                                       tol_step, tol_quad, a.o.clamp, dom, sc);
+                st = gn_step<K1, K2, EXACT, T, NEWTON, PREG>(d, ku, kv, Px, Py, Pz, cu, cv,
+                                                             tol_dist, tol_step, tol_quad,
+                                                             a.o.clamp, dom, sc, &pr);
                 if (st == kStepContinue || st == GN_PT_CONVERGED) it++;
                 if (st == kStepContinue && it >= lim) st = GN_PT_MAXITER;
             }

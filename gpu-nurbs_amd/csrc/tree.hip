@@ -217,16 +217,27 @@ int launch_search_build(const GnSurfDev &d, int levels, const GnTreeNode *tree, 
 }
 
 // One lane per query; mode 0 exact nearest leaf, mode 1 greedy + 3x3.
+// Levels of the seed quadtree whose FP32 boxes the greedy seed kernel stages in LDS: levels
+// 1..5 are 1364 boxes, 32 KB.
+constexpr int kSeedLdsLevels = 5;
+
 __global__ __launch_bounds__(256) void k_seed(GnSearchDev sd, int mode,
                                               const double *__restrict__ x,
                                               const double *__restrict__ y,
                                               const double *__restrict__ z, size_t n,
                                               int32_t *__restrict__ leaf, double *__restrict__ su,
                                               double *__restrict__ sv) {
+    __shared__ float lbox[(((1 << (2 * (kSeedLdsLevels + 1))) - 1) / 3 - 1) * 6];
+    const int ll = mode == 1 ? (sd.depth < kSeedLdsLevels ? sd.depth : kSeedLdsLevels) : 0;
+    if (ll > 0) {
+        const int nf = (int)(qoff(ll + 1) - 1) * 6;
+        for (int i = threadIdx.x; i < nf; i += blockDim.x) lbox[i] = sd.box32[6 + i];
+        __syncthreads();
+    }
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (size_t)gridDim.x * blockDim.x) {
         const double P[3] = {x[i], y[i], z[i]};
-        const uint32_t c = mode == 0 ? nearest_leaf(sd, P) : greedy_leaf(sd, P);
+        const uint32_t c = mode == 0 ? nearest_leaf(sd, P) : greedy_leaf(sd, P, lbox, ll);
         if (leaf) leaf[i] = sd.ref_index[c];
         su[i] = sd.seed_uv[c * 2 + 0];
         sv[i] = sd.seed_uv[c * 2 + 1];
@@ -237,7 +248,9 @@ int launch_seed(const GnSearchDev &sd, int mode, const double *x, const double *
                 const double *z, size_t n, int32_t *leaf, double *su, double *sv,
                 hipStream_t st) {
     if (n == 0) return GN_OK;
-    k_seed<<<grid_for(n, 256, 256 * 32), 256, 0, st>>>(sd, mode, x, y, z, n, leaf, su, sv);
+    // greedy: a few long-lived workgroups per CU (each stages the upper tree levels once)
+    const unsigned cap = mode == 1 ? 256 * 4 : 256 * 32;
+    k_seed<<<grid_for(n, 256, cap), 256, 0, st>>>(sd, mode, x, y, z, n, leaf, su, sv);
     GN_HIP(hipGetLastError());
     return GN_OK;
 }

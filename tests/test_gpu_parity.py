@@ -318,3 +318,34 @@ def test_result_independent_of_batch_and_order(precision, newton):
                                                                                            newton=newton))]
     for a, b in zip(big, small):
         assert np.array_equal(a[sel], b)
+
+
+def test_long_knot_vectors_take_the_global_knot_paths():
+    """Knot vectors too long for LDS (603 + 1003 knots): evaluation falls back from the
+    LDS-staged kernel and inversion from the exact-degree kernels to the generic kernels that
+    read knots through L1; results keep the FP64 and inversion tolerances."""
+    nu, nv = 40, 1000  # smooth rational net (no per-point noise, which would be 1000 bumps)
+    X, Y = np.meshgrid(np.arange(nu) / (nu - 1), np.arange(nv) / (nv - 1), indexing="ij")
+    cp = np.stack([X, Y, 0.2 * np.sin(2 * np.pi * X) * np.cos(2 * np.pi * Y)], axis=-1)
+    w = 1.0 + 0.3 * np.sin(3 * X + 5 * Y)
+    sd = synth.SurfaceDef(2, 2, synth.clamped_knots(nu, 2), synth.clamped_knots(nv, 2), cp, w)
+    g, o = _pair(sd)
+    rng = np.random.default_rng(3)
+    u, v = rng.random(20000), rng.random(20000)
+    got = blocks(_np(g.Evaluate(u, v, 2)))
+    ref = blocks(o.eval(u, v, 2))
+    floor = 1e-2 * np.max(np.abs(ref["S"]))
+    for f in FIELDS:
+        # second derivatives of a 998-span net carry chain factors ~1e6 and a cancelling quotient
+        # rule: the two FP64 computations (different basis-matrix rounding) agree to ~2e-12 there
+        tol = 1e-12 if f in ("S", "Su", "Sv") else 1e-11
+        ok, err = field_close(got[f], ref[f], tol, floor)
+        assert ok, (f, err)
+    P = o.eval(u[:5000], v[:5000], 0)
+    g.BuildTree(3)
+    for prec in (G.GN_FP64, G.GN_MIXED):
+        opts = G.default_opts(precision=prec, newton=G.NEWTON_FP32 | G.NEWTON_FP64, tol_quad=1e-6,
+                              tol_step32=1e-4)
+        gu, gv, st = (_np(t) for t in g.NewInverse(P[0], P[1], P[2], opts))
+        err = np.maximum(np.abs(gu - u[:5000]), np.abs(gv - v[:5000]))
+        assert np.mean(err <= 1e-9) > 0.97, prec
