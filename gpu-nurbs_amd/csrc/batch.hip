@@ -1,18 +1,23 @@
-// batch.hip -- inversion of many surfaces of uneven degree and net size in one persistent launch
-// (the batched form of NewInverse + GaussNewton, src/kernel/kernel.cu:151-249, 433-469).
+// batch.hip -- inversion of many surfaces of uneven degree and net size (the batched form of
+// NewInverse + GaussNewton, src/kernel/kernel.cu:151-249, 433-469).
 //
 // A batch packs the span tables, knots and seed trees of its surfaces into one allocation and
 // describes each surface by a GnBatchObj (pointers into the pack, degree class).  Work is a list
-// of tiles (object, contiguous point range) sorted by degree class.  Waves claim tiles from sliced
-// counters and keep their 64 lanes busy with the points of their tiles; all lanes of a wave
-// always belong to one degree class, so the per-class template (exact unrolled contraction for
-// degrees 2..5 in each direction, padded generic form otherwise) is chosen once per wave step
-// without divergence.  When the next tile is of another class, the wave drains its current points
-// before switching.  Seeds are found inline when a lane takes a point (greedy leaf of the
-// point's own surface).  The FP32 bulk and the FP64 closing pass are two launches of the same
-// kernel template, as in the single-surface mixed mode.  Point arrays are addressed through a
-// request table (GnReq): a tile names its request, so one launch can serve the points of many
-// independent requests (the serving loop) or of one call (a one-entry table).
+// of tiles (object, contiguous point range, request) sorted by degree class.  Each degree class
+// present gets its own persistent launch of a kernel instantiated for that class alone (exact
+// unrolled contraction for degrees 2..5 in each direction, padded generic form otherwise), so
+// every launch has the register footprint of its own class -- one kernel holding all classes
+// would be sized by the largest and run at one wave per SIMD.  Within a launch, waves claim the
+// class's tiles from sliced counters and keep their 64 lanes busy with the points of those
+// tiles (lane refill).  Seeds (greedy or nearest leaf of each point's own surface) come from one
+// launch over all tiles before the class launches: found inside the refill loop they would be
+// paid by the whole wave at the refilling lanes' occupancy.  The FP32 bulk and the FP64 closing
+// pass are launches of the same template, as in the single-surface mixed mode.  Point arrays are addressed through a request
+// table (GnReq): a tile names its request, so one launch serves the points of many independent
+// requests (the serving loop) or of one call (a one-entry table).
+#include <algorithm>
+#include <mutex>
+
 #include "gn_search.hpp"
 #include "invert_common.hpp"
 
@@ -20,40 +25,38 @@ namespace gn {
 
 struct BatchArgs {
     const GnBatchObj *objs;
-    const GnTile *tiles;
+    const GnTile *tiles;  // the tiles of one degree class
     int ntiles;
     const GnReq *reqs;
     gn_invert_opts o;
 };
 
-template <typename T, bool NEWTON>
-__device__ __forceinline__ int step_cls(int cls, const GnSurfDev &d, T Px, T Py, T Pz, double &cu,
-                                        double &cv, double tol_dist, double tol_step,
-                                        double tol_quad, int clamp, SpanCache &sc) {
-    switch (cls) {
-#define GN_CLS(P, Q)                                                                            \
-    case (P - 2) * 4 + (Q - 2):                                                                 \
-        return gn_step<P + 1, Q + 1, true, T, NEWTON>(d, d.ku, d.kv, Px, Py, Pz, cu, cv, tol_dist, \
-                                                      tol_step, tol_quad, clamp, domain_of(d), sc);
-        GN_CLS(2, 2) GN_CLS(2, 3) GN_CLS(2, 4) GN_CLS(2, 5)
-        GN_CLS(3, 2) GN_CLS(3, 3) GN_CLS(3, 4) GN_CLS(3, 5)
-        GN_CLS(4, 2) GN_CLS(4, 3) GN_CLS(4, 4) GN_CLS(4, 5)
-        GN_CLS(5, 2) GN_CLS(5, 3) GN_CLS(5, 4) GN_CLS(5, 5)
-#undef GN_CLS
-        default:
-            return gn_step<GN_MAXK, GN_MAXK, false, T, NEWTON>(d, d.ku, d.kv, Px, Py, Pz, cu, cv,
-                                                               tol_dist, tol_step, tol_quad, clamp,
-                                                               domain_of(d), sc);
+// Seeds of every point of the tiles into its request's u, v (one workgroup per tile).
+constexpr int kSeedBlock = 128;
+__global__ __launch_bounds__(kSeedBlock) void k_batch_seed(const GnBatchObj *__restrict__ objs,
+                                                           const GnTile *__restrict__ tiles,
+                                                           int ntiles, const GnReq *__restrict__ reqs,
+                                                           int mode) {
+    for (int ti = blockIdx.x; ti < ntiles; ti += gridDim.x) {
+        const GnTile tl = tiles[ti];
+        const GnReq R = reqs[tl.req];
+        const GnSearchDev sd = objs[tl.obj].sd;
+        for (int j = threadIdx.x; j < tl.count; j += blockDim.x) {
+            const int64_t i = tl.start + j;
+            const double P[3] = {R.x[i], R.y[i], R.z[i]};
+            const uint32_t c = mode == GN_SEED_NEAREST_LEAF ? nearest_leaf(sd, P) : greedy_leaf(sd, P);
+            R.u[i] = sd.seed_uv[c * 2 + 0];
+            R.v[i] = sd.seed_uv[c * 2 + 1];
+        }
     }
 }
 
-template <typename T, bool NEWTON, int MODE>
-__global__ __launch_bounds__(kInvBlock) void k_batch(BatchArgs a, unsigned long long *counters,
-                                                     int slices, int max_iter, int restart_iter,
-                                                     double tol_dist, double tol_step,
-                                                     double tol_quad) {
+template <int K1, int K2, bool EXACT, typename T, bool NEWTON, int MODE>
+__global__ __launch_bounds__(kInvBlock, (EXACT && sizeof(T) == 4 && K1 * K2 <= 16) ? 3 : 1)
+void k_batch(BatchArgs a, unsigned long long *counters, int slices, int max_iter,
+             int restart_iter, double tol_dist, double tol_step, double tol_quad) {
     const int lane = threadIdx.x & 63;
-    const unsigned long long lt = (1ull << lane) - 1ull, all = ~0ull;
+    const unsigned long long lt = (1ull << lane) - 1ull;
     const int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
     const int slice = wave % slices;
     const long long s_lo = (long long)a.ntiles * slice / slices;
@@ -61,42 +64,34 @@ __global__ __launch_bounds__(kInvBlock) void k_batch(BatchArgs a, unsigned long 
     unsigned long long *counter = counters + (size_t)slice * kCounterStride;
     // wave-uniform tile state
     long long q_lo = 0, q_hi = 0;
-    int cur_obj = -1, cur_cls = -1, cur_req = 0;
-    bool exhausted = false, pend = false;
-    GnTile pt{0, 0, 0};
-    int pcls = -1;
+    int cur_obj = 0, cur_req = 0;
+    bool exhausted = false;
     // lane state
     long long idx = -1;
-    int obj = 0, rq = 0, it = 0, lim = max_iter;
+    int rq = 0, it = 0, lim = max_iter;
     GnSurfDev d{};
     double cu = 0, cv = 0, su = 0, sv = 0;
     T Px = 0, Py = 0, Pz = 0;
     SpanCache sc;
+    constexpr bool PREG = EXACT && sizeof(T) == 4 && K1 * K2 <= 16;
+    PatchRegs<T, K1 * K2> pr;
     while (true) {
-        while (true) {
+        while (!exhausted) {
             const unsigned long long idle = __ballot(idx < 0);
             if (idle == 0ull) break;
             if (q_lo >= q_hi) {
-                if (!pend) {
-                    if (exhausted) break;
-                    unsigned long long k = 0;
-                    if (lane == 0) k = atomicAdd(counter, 1ull);
-                    k = bcast64(k, 0);
-                    if (s_lo + (long long)k >= s_hi) {
-                        exhausted = true;
-                        break;
-                    }
-                    pt = a.tiles[s_lo + (long long)k];
-                    pcls = a.objs[pt.obj].cls;
-                    pend = true;
+                unsigned long long k = 0;
+                if (lane == 0) k = atomicAdd(counter, 1ull);
+                k = bcast64(k, 0);
+                if (s_lo + (long long)k >= s_hi) {
+                    exhausted = true;
+                    break;
                 }
-                if (pcls != cur_cls && idle != all) break;  // drain the current class first
+                const GnTile pt = a.tiles[s_lo + (long long)k];
                 cur_obj = pt.obj;
-                cur_cls = pcls;
                 cur_req = pt.req;
                 q_lo = pt.start;
                 q_hi = pt.start + pt.count;
-                pend = false;
                 continue;
             }
             const unsigned long long avail = (unsigned long long)(q_hi - q_lo);
@@ -104,9 +99,8 @@ __global__ __launch_bounds__(kInvBlock) void k_batch(BatchArgs a, unsigned long 
                 const unsigned long long rank = (unsigned long long)__popcll(idle & lt);
                 if (rank < avail) {
                     idx = q_lo + (long long)rank;
-                    obj = cur_obj;
                     rq = cur_req;
-                    d = a.objs[obj].s;
+                    d = a.objs[cur_obj].s;
                     const GnReq &R = a.reqs[cur_req];
                     const double P[3] = {R.x[idx], R.y[idx], R.z[idx]};
                     Px = T(P[0]);
@@ -114,21 +108,11 @@ __global__ __launch_bounds__(kInvBlock) void k_batch(BatchArgs a, unsigned long 
                     Pz = T(P[2]);
                     it = 0;
                     lim = max_iter;
+                    cu = R.u[idx];  // the seed, or the FP32 result for the closing pass
+                    cv = R.v[idx];
                     if (MODE == kClose) {
-                        cu = R.u[idx];
-                        cv = R.v[idx];
                         const int sb = R.status[idx];
                         if (sb == GN_PT_SINGULAR || sb == GN_PT_NONFINITE) lim = restart_iter;
-                    } else if (a.o.seed_mode == GN_SEED_GIVEN) {
-                        cu = R.u[idx];
-                        cv = R.v[idx];
-                    } else {
-                        const GnSearchDev &sd = a.objs[obj].sd;
-                        const uint32_t c = a.o.seed_mode == GN_SEED_NEAREST_LEAF
-                                               ? nearest_leaf(sd, P)
-                                               : greedy_leaf(sd, P);
-                        cu = sd.seed_uv[c * 2 + 0];
-                        cv = sd.seed_uv[c * 2 + 1];
                     }
                     su = cu;
                     sv = cv;
@@ -144,8 +128,9 @@ __global__ __launch_bounds__(kInvBlock) void k_batch(BatchArgs a, unsigned long 
             if (it >= lim) {
                 st = GN_PT_MAXITER;
             } else {
-                st = step_cls<T, NEWTON>(cur_cls, d, Px, Py, Pz, cu, cv, tol_dist, tol_step,
-                                         tol_quad, a.o.clamp, sc);
+                st = gn_step<K1, K2, EXACT, T, NEWTON, PREG>(d, d.ku, d.kv, Px, Py, Pz, cu, cv,
+                                                             tol_dist, tol_step, tol_quad,
+                                                             a.o.clamp, domain_of(d), sc, &pr);
                 if (st == kStepContinue || st == GN_PT_CONVERGED) it++;
                 if (st == kStepContinue && it >= lim) st = GN_PT_MAXITER;
             }
@@ -185,13 +170,14 @@ static void batch_shape(const void *kernel, int ntiles, size_t n, int bpc_cap, u
     if (slices > kMaxSlices) slices = kMaxSlices;
 }
 
-template <typename T, bool NEWTON, int MODE>
-static int launch_one(const BatchArgs &a, size_t n, int bpc_cap, int max_iter, int restart_iter,
+// One launch of a class kernel over the class's tiles.
+template <int K1, int K2, bool EXACT, typename T, bool NEWTON, int MODE>
+static int launch_cls(const BatchArgs &a, size_t n, int bpc_cap, int max_iter, int restart_iter,
                       double tol_dist, double tol_step, double tol_quad, hipStream_t st) {
     unsigned long long *ctr = nullptr;
     int rc = work_counters(st, &ctr);
     if (rc) return rc;
-    auto k = k_batch<T, NEWTON, MODE>;
+    auto k = k_batch<K1, K2, EXACT, T, NEWTON, MODE>;
     unsigned grid;
     int slices;
     batch_shape((const void *)k, a.ntiles, n, bpc_cap, grid, slices);
@@ -201,32 +187,157 @@ static int launch_one(const BatchArgs &a, size_t n, int bpc_cap, int max_iter, i
     return GN_OK;
 }
 
+// The precision phases of one degree class (sizes K1, K2).
+struct ClassPhases {
+    const BatchArgs &a;
+    size_t n;
+    int bpc_cap;
+    hipStream_t st;
+    template <int K1, int K2, bool EXACT>
+    int operator()() {
+        const gn_invert_opts &o = a.o;
+        const bool n32 = o.newton & GN_NEWTON_FP32, n64 = o.newton & GN_NEWTON_FP64;
+        int rc;
+        if (o.precision == GN_MIXED) {
+            rc = n32 ? launch_cls<K1, K2, EXACT, float, true, kBulk>(a, n, bpc_cap, o.fp32_iters, 0,
+                                                                     0.0, o.tol_step32, 0.0, st)
+                     : launch_cls<K1, K2, EXACT, float, false, kBulk>(a, n, bpc_cap, o.fp32_iters,
+                                                                      0, 0.0, o.tol_step32, 0.0, st);
+            if (rc) return rc;
+            return n64 ? launch_cls<K1, K2, EXACT, double, true, kClose>(
+                             a, n, bpc_cap, o.fp64_tail, o.max_iter, o.tol_dist, o.tol_step,
+                             o.tol_quad, st)
+                       : launch_cls<K1, K2, EXACT, double, false, kClose>(
+                             a, n, bpc_cap, o.fp64_tail, o.max_iter, o.tol_dist, o.tol_step, 0.0,
+                             st);
+        }
+        return n64 ? launch_cls<K1, K2, EXACT, double, true, kPlain>(
+                         a, n, bpc_cap, o.max_iter, 0, o.tol_dist, o.tol_step, o.tol_quad, st)
+                   : launch_cls<K1, K2, EXACT, double, false, kPlain>(
+                         a, n, bpc_cap, o.max_iter, 0, o.tol_dist, o.tol_step, 0.0, st);
+    }
+};
+
 int batch_waves(int bpc_cap) {
-    auto k = k_batch<float, true, kBulk>;
+    auto k = k_batch<4, 4, true, float, true, kBulk>;
     int bpc = blocks_per_cu((const void *)k, kInvBlock);
     if (bpc_cap > 0 && bpc_cap < bpc) bpc = bpc_cap;
     return cu_count() * bpc * (kInvBlock / 64);
 }
 
-int launch_batch_invert(const GnBatchObj *objs, const GnTile *tiles, int ntiles, const GnReq *reqs,
-                        size_t n, const gn_invert_opts &o, int bpc_cap, hipStream_t st) {
-    if (n == 0 || ntiles == 0) return GN_OK;
-    BatchArgs a{objs, tiles, ntiles, reqs, o};
-    const bool n32 = o.newton & GN_NEWTON_FP32, n64 = o.newton & GN_NEWTON_FP64;
-    int rc;
-    if (o.precision == GN_MIXED) {
-        rc = n32 ? launch_one<float, true, kBulk>(a, n, bpc_cap, o.fp32_iters, 0, 0.0, o.tol_step32, 0.0, st)
-                 : launch_one<float, false, kBulk>(a, n, bpc_cap, o.fp32_iters, 0, 0.0, o.tol_step32, 0.0, st);
-        if (rc) return rc;
-        return n64 ? launch_one<double, true, kClose>(a, n, bpc_cap, o.fp64_tail, o.max_iter, o.tol_dist,
-                                                      o.tol_step, o.tol_quad, st)
-                   : launch_one<double, false, kClose>(a, n, bpc_cap, o.fp64_tail, o.max_iter, o.tol_dist,
-                                                       o.tol_step, 0.0, st);
+int class_tile(size_t points, int waves, int cap) {
+    size_t t = points / (2 * (size_t)(waves > 0 ? waves : 1)) / 64 * 64;
+    if (t > (size_t)cap) t = (size_t)cap;
+    if (t < 64) t = 64;
+    return (int)t;
+}
+
+std::vector<ClassRange> class_ranges(const GnTile *tiles, size_t ntiles, const std::vector<int> &cls) {
+    std::vector<ClassRange> r;
+    for (size_t t = 0; t < ntiles; t++) {
+        const int c = cls[tiles[t].obj];
+        if (r.empty() || r.back().cls != c) r.push_back(ClassRange{c, (int)t, (int)t, 0});
+        r.back().t1 = (int)t + 1;
+        r.back().points += (size_t)tiles[t].count;
     }
-    return n64 ? launch_one<double, true, kPlain>(a, n, bpc_cap, o.max_iter, 0, o.tol_dist, o.tol_step,
-                                                  o.tol_quad, st)
-               : launch_one<double, false, kPlain>(a, n, bpc_cap, o.max_iter, 0, o.tol_dist, o.tol_step,
-                                                   0.0, st);
+    return r;
+}
+
+int side_streams_create(SideStreams *ss) {
+    for (int k = 0; k < kSideStreams; k++) {
+        GN_HIP(hipStreamCreateWithFlags(&ss->s[k], hipStreamNonBlocking));
+        GN_HIP(hipEventCreateWithFlags(&ss->join[k], hipEventDisableTiming));
+    }
+    GN_HIP(hipEventCreateWithFlags(&ss->fork, hipEventDisableTiming));
+    return GN_OK;
+}
+
+void side_streams_destroy(SideStreams *ss) {
+    for (int k = 0; k < kSideStreams; k++) {
+        if (ss->s[k]) (void)hipStreamDestroy(ss->s[k]);
+        if (ss->join[k]) (void)hipEventDestroy(ss->join[k]);
+    }
+    if (ss->fork) (void)hipEventDestroy(ss->fork);
+    *ss = SideStreams();
+}
+
+namespace {
+std::mutex g_side_mu;
+SideStreams g_side[64];  // per device, created on first use
+}  // namespace
+
+static int launch_ranges(const GnBatchObj *objs, const GnTile *tiles,
+                         const std::vector<ClassRange> &ranges, const GnReq *reqs,
+                         const gn_invert_opts &o, int bpc_cap, hipStream_t st,
+                         const SideStreams &ss) {
+    if (o.seed_mode != GN_SEED_GIVEN) {
+        const int t0 = ranges.front().t0, nt = ranges.back().t1 - t0;
+        const unsigned grid = (unsigned)(nt < 256 * 32 ? nt : 256 * 32);
+        k_batch_seed<<<grid, kSeedBlock, 0, st>>>(objs, tiles + t0, nt, reqs, o.seed_mode);
+        GN_HIP(hipGetLastError());
+    }
+    // largest classes first, each to the side stream with the least work so far
+    std::vector<int> order(ranges.size());
+    for (size_t i = 0; i < ranges.size(); i++) order[i] = (int)i;
+    std::sort(order.begin(), order.end(),
+              [&](int x, int y) { return ranges[x].points > ranges[y].points; });
+    size_t load[kSideStreams] = {};
+    bool used[kSideStreams] = {};
+    GN_HIP(hipEventRecord(ss.fork, st));
+    for (int i : order) {
+        const ClassRange &cr = ranges[i];
+        if (cr.points == 0) continue;
+        int k = 0;
+        for (int j = 1; j < kSideStreams; j++)
+            if (load[j] < load[k]) k = j;
+        if (!used[k]) {
+            GN_HIP(hipStreamWaitEvent(ss.s[k], ss.fork, 0));
+            used[k] = true;
+        }
+        load[k] += cr.points;
+        const BatchArgs a{objs, tiles + cr.t0, cr.t1 - cr.t0, reqs, o};
+        ClassPhases f{a, cr.points, bpc_cap, ss.s[k]};
+        int rc;
+        if (cr.cls >= 0 && cr.cls < 16) rc = dispatch_degrees(cr.cls / 4 + 2, cr.cls % 4 + 2, f);
+        else rc = f.template operator()<GN_MAXK, GN_MAXK, false>();
+        if (rc) return rc;
+    }
+    for (int k = 0; k < kSideStreams; k++) {
+        if (!used[k]) continue;
+        GN_HIP(hipEventRecord(ss.join[k], ss.s[k]));
+        GN_HIP(hipStreamWaitEvent(st, ss.join[k], 0));
+    }
+    return GN_OK;
+}
+
+static int shared_side_locked(int dev) {
+    if (g_side[dev].fork) return GN_OK;
+    const int rc = side_streams_create(&g_side[dev]);
+    if (rc) side_streams_destroy(&g_side[dev]);
+    return rc;
+}
+
+int batch_prepare() {
+    int dev = 0;
+    GN_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) return fail(GN_EDEVICE, "device index out of range");
+    std::lock_guard<std::mutex> lk(g_side_mu);
+    return shared_side_locked(dev);
+}
+
+int launch_batch_invert(const GnBatchObj *objs, const GnTile *tiles,
+                        const std::vector<ClassRange> &ranges, const GnReq *reqs,
+                        const gn_invert_opts &o, int bpc_cap, hipStream_t st,
+                        const SideStreams *side) {
+    if (ranges.empty()) return GN_OK;
+    if (side) return launch_ranges(objs, tiles, ranges, reqs, o, bpc_cap, st, *side);
+    int dev = 0;
+    GN_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) return fail(GN_EDEVICE, "device index out of range");
+    std::lock_guard<std::mutex> lk(g_side_mu);  // one user of the shared set at a time
+    const int rc = shared_side_locked(dev);
+    if (rc) return rc;
+    return launch_ranges(objs, tiles, ranges, reqs, o, bpc_cap, st, g_side[dev]);
 }
 
 }  // namespace gn

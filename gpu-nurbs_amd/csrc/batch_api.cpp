@@ -52,11 +52,19 @@ int check_segments(const gn_batch *b, const gn_segment *segs, int nseg, size_t n
     return GN_OK;
 }
 
+// Tiles of every segment, sorted by degree class; each class gets its own tile size (at most
+// `tile`), sized for the class's own launch.
 std::vector<GnTile> build_tiles(const gn_batch *b, const gn_segment *segs, int nseg, int tile) {
+    std::vector<size_t> per(gn::kClasses, 0);
+    for (int i = 0; i < nseg; i++) per[b->cls[segs[i].obj]] += (size_t)segs[i].count;
+    const int waves = gn::batch_waves(b->bpc);
+    std::vector<int> tsz(gn::kClasses);
+    for (int c = 0; c < gn::kClasses; c++) tsz[c] = gn::class_tile(per[c], waves, tile);
     std::vector<GnTile> t;
     for (int i = 0; i < nseg; i++) {
-        for (int64_t off = 0; off < segs[i].count; off += tile) {
-            const int64_t c = std::min<int64_t>(tile, segs[i].count - off);
+        const int64_t T = tsz[b->cls[segs[i].obj]];
+        for (int64_t off = 0; off < segs[i].count; off += T) {
+            const int64_t c = std::min<int64_t>(T, segs[i].count - off);
             t.push_back(GnTile{segs[i].obj, (int32_t)c, segs[i].start + off, 0, 0});
         }
     }
@@ -71,11 +79,13 @@ std::vector<GnTile> build_tiles(const gn_batch *b, const gn_segment *segs, int n
 struct CallImage {
     std::vector<char> bytes;
     size_t ntiles = 0, req_off = 0;
+    std::vector<gn::ClassRange> ranges;
 };
 
-CallImage call_image(const std::vector<GnTile> &t, const GnReq &r) {
+CallImage call_image(const gn_batch *b, const std::vector<GnTile> &t, const GnReq &r) {
     CallImage c;
     c.ntiles = t.size();
+    c.ranges = gn::class_ranges(t.data(), t.size(), b->cls);
     c.req_off = align256(sizeof(GnTile) * t.size());
     c.bytes.resize(c.req_off + sizeof(GnReq));
     if (!t.empty()) std::memcpy(c.bytes.data(), t.data(), sizeof(GnTile) * t.size());
@@ -91,7 +101,7 @@ int run_batch(gn_batch *b, const gn_segment *segs, int nseg, const double *x, co
               const double *z, size_t n, const gn_invert_opts &o, double *u, double *v,
               uint8_t *status, uint8_t *it32, uint8_t *it64, int tile, int bpc, hipStream_t st) {
     const GnReq req{x, y, z, u, v, status, it32, it64};
-    auto *img = new CallImage(call_image(build_tiles(b, segs, nseg, tile), req));
+    auto *img = new CallImage(call_image(b, build_tiles(b, segs, nseg, tile), req));
     if (img->ntiles == 0) {
         delete img;
         return GN_OK;
@@ -102,8 +112,8 @@ int run_batch(gn_batch *b, const gn_segment *segs, int nseg, const double *x, co
         delete img;
         return gn::hip_fail(e, "hipMallocAsync(tiles)");
     }
-    const int nt = (int)img->ntiles;
     const size_t req_off = img->req_off;
+    const std::vector<gn::ClassRange> ranges = img->ranges;
     e = hipMemcpyAsync(d, img->bytes.data(), img->bytes.size(), hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipLaunchHostFunc(st, free_image_host, img);
     if (e != hipSuccess) {
@@ -112,8 +122,9 @@ int run_batch(gn_batch *b, const gn_segment *segs, int nseg, const double *x, co
         (void)hipFreeAsync(d, st);
         return gn::hip_fail(e, "tile upload");
     }
-    int rc = gn::launch_batch_invert(b->d_objs, (const GnTile *)d, nt, (const GnReq *)(d + req_off),
-                                     n, o, bpc, st);
+    (void)n;
+    int rc = gn::launch_batch_invert(b->d_objs, (const GnTile *)d, ranges,
+                                     (const GnReq *)(d + req_off), o, bpc, st);
     GN_HIP(hipFreeAsync(d, st));
     return rc;
 }
@@ -322,7 +333,7 @@ int gn_batch_tune(gn_batch *b, const gn_segment *segs, int nseg, const double *x
     hipEvent_t e0, e1;
     GN_HIP(hipEventCreate(&e0));
     GN_HIP(hipEventCreate(&e1));
-    const int tiles[] = {256, 512, 1024, 2048, 4096};
+    const int tiles[] = {128, 256, 512, 1024, 4096};
     const int caps[] = {0, 1};
     float best = 1e30f;
     int bt = b->tile, bc = b->bpc;
@@ -374,8 +385,9 @@ int gn_batch_graph_create(gn_batch *b, const gn_segment *segs, int nseg, const d
     if ((rc = check_segments(b, segs, nseg, n))) return rc;
     if (n && (!x || !y || !z || !u || !v || !status)) return fail(GN_EINVAL, "null buffer");
     if ((rc = gn::ensure_runtime())) return rc;
+    if ((rc = gn::batch_prepare())) return rc;
     const GnReq req{x, y, z, u, v, status, nullptr, nullptr};
-    const CallImage img = call_image(build_tiles(b, segs, nseg, b->tile), req);
+    const CallImage img = call_image(b, build_tiles(b, segs, nseg, b->tile), req);
     gn_graph *g = new gn_graph();
     hipStream_t st = (hipStream_t)stream;
     hipError_t e = hipMalloc((void **)&g->d_buf, img.bytes.size());
@@ -392,8 +404,8 @@ int gn_batch_graph_create(gn_batch *b, const gn_segment *segs, int nseg, const d
         delete g;
         return gn::hip_fail(e, "hipStreamBeginCapture");
     }
-    rc = gn::launch_batch_invert(b->d_objs, (const GnTile *)g->d_buf, (int)img.ntiles,
-                                 (const GnReq *)(g->d_buf + img.req_off), n, o, b->bpc, st);
+    rc = gn::launch_batch_invert(b->d_objs, (const GnTile *)g->d_buf, img.ranges,
+                                 (const GnReq *)(g->d_buf + img.req_off), o, b->bpc, st);
     e = hipStreamEndCapture(st, &g->graph);
     if (!rc && e == hipSuccess) e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
     if (rc || e != hipSuccess) {

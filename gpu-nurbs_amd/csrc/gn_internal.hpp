@@ -171,9 +171,35 @@ int launch_search_build(const GnSurfDev &d, int levels, const GnTreeNode *tree, 
 int launch_seed(const GnSearchDev &sd, int mode, const double *x, const double *y,
                 const double *z, size_t n, int32_t *leaf, double *su, double *sv,
                 hipStream_t st);
-// n: total points of the tiles (sizes the persistent grid)
-int launch_batch_invert(const GnBatchObj *objs, const GnTile *tiles, int ntiles, const GnReq *reqs,
-                        size_t n, const gn_invert_opts &o, int bpc_cap, hipStream_t st);
+// A run of tiles of one degree class in a class-sorted tile list, and its point count.
+struct ClassRange {
+    int cls, t0, t1;
+    size_t points;
+};
+std::vector<ClassRange> class_ranges(const GnTile *tiles, size_t ntiles, const std::vector<int> &cls);
+// Points per tile for one class launch of `points` points: about two tiles per resident wave,
+// between 64 and cap (a tile is the unit a wave claims; too large a tile leaves waves idle).
+int class_tile(size_t points, int waves, int cap);
+constexpr int kClasses = 17;  // 16 exact degree pairs + the generic class
+// Side streams and fork/join events for running independent launches concurrently.  A set is
+// used by one caller at a time (the batched entry points share one per device under a lock; a
+// server owns one per step slot).
+constexpr int kSideStreams = 4;
+struct SideStreams {
+    hipStream_t s[kSideStreams] = {};
+    hipEvent_t fork = nullptr, join[kSideStreams] = {};
+};
+int side_streams_create(SideStreams *ss);
+// Creates the device's shared side-stream set (call before a graph capture).
+int batch_prepare();
+void side_streams_destroy(SideStreams *ss);
+// One persistent launch per class range and precision phase.  The class ranges are spread over
+// the side streams (forked from and joined back into st); with side == nullptr the device's
+// shared set is used.
+int launch_batch_invert(const GnBatchObj *objs, const GnTile *tiles,
+                        const std::vector<ClassRange> &ranges, const GnReq *reqs,
+                        const gn_invert_opts &o, int bpc_cap, hipStream_t st,
+                        const SideStreams *side = nullptr);
 // resident waves of one persistent batch launch (sizes tiles for short serving steps)
 int batch_waves(int bpc_cap);
 // tiles of curve objects (obj indexes `curves`); max_samples: largest seed sample count

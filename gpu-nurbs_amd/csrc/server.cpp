@@ -58,6 +58,7 @@ size_t result_bytes(bool curve, size_t n) {
 struct Slot {
     hipStream_t st = nullptr;
     hipEvent_t ev = nullptr;
+    gn::SideStreams side;  // the step's class launches run concurrently on these
     char *h_stage = nullptr;  // pinned
     char *d_stage = nullptr;
     std::vector<Req *> reqs;
@@ -140,13 +141,26 @@ int gn_server::launch_step(Slot &sl) {
     const int nreq = (int)sl.reqs.size();
     size_t spts = 0, cpts = 0;
     for (Req *r : sl.reqs) (r->curve ? cpts : spts) += r->n;
-    // Surface tile size: the batch's tile, shrunk (to a multiple of 64, at least 64) so that a
-    // short step still gives every resident wave about two tiles; grown if the step would exceed
-    // kMaxStepTiles (sum of ceil(n/T) <= pts/T + nreq).
+    // Surface tile size per degree class (each class is its own launch): about two tiles per
+    // resident wave, at most the batch's tile; all grown by a common factor if the step would
+    // exceed kMaxStepTiles (sum of ceil(n/T) <= pts/T + nreq).
     const size_t room = (size_t)(kMaxStepTiles - 2 * nreq);
-    size_t ts = std::min<size_t>((size_t)tile, spts / (2 * (size_t)waves) / 64 * 64);
-    if (ts < 64) ts = 64;
-    if (spts / ts > room / 2) ts = (spts + room / 2 - 1) / (room / 2);
+    std::vector<size_t> per(gn::kClasses, 0);
+    for (Req *r : sl.reqs)
+        if (!r->curve) per[cls[r->obj]] += r->n;
+    std::vector<size_t> tsz(gn::kClasses);
+    size_t est = 0;
+    for (int c = 0; c < gn::kClasses; c++) {
+        tsz[c] = (size_t)gn::class_tile(per[c], waves, tile);
+        est += per[c] / tsz[c];
+    }
+    for (size_t grow = 2; est > room / 2; grow *= 2) {
+        est = 0;
+        for (int c = 0; c < gn::kClasses; c++) {
+            tsz[c] = (size_t)gn::class_tile(per[c], waves, tile) * grow;
+            est += per[c] / tsz[c];
+        }
+    }
     size_t tc = kCurveTile;
     if (cpts / tc > room / 2) tc = (cpts + room / 2 - 1) / (room / 2);
     GnReq *rq = (GnReq *)sl.h_stage;
@@ -154,7 +168,7 @@ int gn_server::launch_step(Slot &sl) {
     for (int k = 0; k < nreq; k++) {
         Req *r = sl.reqs[k];
         rq[k] = GnReq{r->x, r->y, r->z, r->u, r->v, r->status, nullptr, nullptr};
-        const size_t T = r->curve ? tc : ts;
+        const size_t T = r->curve ? tc : tsz[cls[r->obj]];
         for (size_t off = 0; off < r->n; off += T) {
             const size_t c = std::min(T, r->n - off);
             GnTile t{r->curve ? r->obj - nsurf : r->obj, (int32_t)c, (int64_t)off, k, 0};
@@ -175,7 +189,8 @@ int gn_server::launch_step(Slot &sl) {
     const GnReq *d_rq = (const GnReq *)sl.d_stage;
     const GnTile *d_tl = (const GnTile *)(sl.d_stage + toff);
     int rc = GN_OK;
-    if (ns) rc = gn::launch_batch_invert(d_objs, d_tl, (int)ns, d_rq, spts, o, bpc, sl.st);
+    if (ns) rc = gn::launch_batch_invert(d_objs, d_tl, gn::class_ranges(st_tiles.data(), ns, cls),
+                                         d_rq, o, bpc, sl.st, &sl.side);
     if (!rc && nc) rc = gn::launch_curve_batch(d_curves, d_tl + ns, (int)nc, d_rq, max_samples, o, sl.st);
     if (rc) return rc;
     GN_HIP(hipEventRecord(sl.ev, sl.st));
@@ -344,6 +359,7 @@ int gn_server_create(gn_batch *batch, gn_curve *const *curves, int ncurves,
         if (e == hipSuccess) e = hipEventCreateWithFlags(&sl.ev, hipEventDisableTiming);
         if (e == hipSuccess) e = hipHostMalloc((void **)&sl.h_stage, kStageBytes, hipHostMallocDefault);
         if (e == hipSuccess) e = hipMalloc((void **)&sl.d_stage, kStageBytes);
+        if (e == hipSuccess && gn::side_streams_create(&sl.side) != GN_OK) e = hipErrorOutOfMemory;
     }
     if (e != hipSuccess) {
         for (Slot &sl : s->slots) {
@@ -351,6 +367,7 @@ int gn_server_create(gn_batch *batch, gn_curve *const *curves, int ncurves,
             if (sl.ev) (void)hipEventDestroy(sl.ev);
             if (sl.h_stage) (void)hipHostFree(sl.h_stage);
             if (sl.d_stage) (void)hipFree(sl.d_stage);
+            gn::side_streams_destroy(&sl.side);
         }
         (void)hipFree(s->d_arena);
         (void)hipFree(s->d_curves);
@@ -474,6 +491,7 @@ int gn_server_destroy(gn_server *s) {
         (void)hipEventDestroy(sl.ev);
         (void)hipHostFree(sl.h_stage);
         (void)hipFree(sl.d_stage);
+        gn::side_streams_destroy(&sl.side);
     }
     (void)hipFree(s->d_arena);
     (void)hipFree(s->d_curves);
