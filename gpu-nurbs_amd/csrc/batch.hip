@@ -51,12 +51,45 @@ __global__ __launch_bounds__(kSeedBlock) void k_batch_seed(const GnBatchObj *__r
     }
 }
 
-template <int K1, int K2, bool EXACT, typename T, bool NEWTON, int MODE>
-__global__ __launch_bounds__(kInvBlock, (EXACT && sizeof(T) == 4 && K1 * K2 <= 16) ? 3 : 1)
+// One inversion step of a point of degree class cls, for the classes of group GRP (other classes
+// never reach a kernel of this group).  pr: the lane's register patch (FP32 group-0 classes).
+template <int GRP, typename T, bool NEWTON, bool PREG>
+__device__ __forceinline__ int step_group(int cls, const GnSurfDev &d, T Px, T Py, T Pz,
+                                          double &cu, double &cv, double tol_dist,
+                                          double tol_step, double tol_quad, int clamp,
+                                          SpanCache &sc, Hom<T> *pr) {
+    switch (cls) {
+#define GN_CLS(P, Q)                                                                            \
+    case (P - 2) * 4 + (Q - 2):                                                                 \
+        if constexpr (class_group((P - 2) * 4 + (Q - 2)) == GRP)                                \
+            return gn_step<P + 1, Q + 1, true, T, NEWTON, PREG && (P + 1) * (Q + 1) <= 16>(     \
+                d, d.ku, d.kv, Px, Py, Pz, cu, cv, tol_dist, tol_step, tol_quad, clamp,         \
+                domain_of(d), sc, pr);                                                          \
+        break;
+        GN_CLS(2, 2) GN_CLS(2, 3) GN_CLS(2, 4) GN_CLS(2, 5)
+        GN_CLS(3, 2) GN_CLS(3, 3) GN_CLS(3, 4) GN_CLS(3, 5)
+        GN_CLS(4, 2) GN_CLS(4, 3) GN_CLS(4, 4) GN_CLS(4, 5)
+        GN_CLS(5, 2) GN_CLS(5, 3) GN_CLS(5, 4) GN_CLS(5, 5)
+#undef GN_CLS
+        default:
+            break;
+    }
+    if constexpr (GRP == 2)
+        return gn_step<GN_MAXK, GN_MAXK, false, T, NEWTON>(d, d.ku, d.kv, Px, Py, Pz, cu, cv,
+                                                           tol_dist, tol_step, tol_quad, clamp,
+                                                           domain_of(d), sc);
+    return GN_PT_NONFINITE;  // not reached: the tiles of a group launch are of its classes
+}
+
+// Persistent lane-refill kernel over the tiles of one class group.  A wave's lanes always hold
+// points of one degree class (wave-uniform dispatch); when its next tile is of another class,
+// the wave drains its current points first.
+template <int GRP, typename T, bool NEWTON, int MODE>
+__global__ __launch_bounds__(kInvBlock, (GRP == 0 && sizeof(T) == 4) ? 3 : 1)
 void k_batch(BatchArgs a, unsigned long long *counters, int slices, int max_iter,
              int restart_iter, double tol_dist, double tol_step, double tol_quad) {
     const int lane = threadIdx.x & 63;
-    const unsigned long long lt = (1ull << lane) - 1ull;
+    const unsigned long long lt = (1ull << lane) - 1ull, all = ~0ull;
     const int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
     const int slice = wave % slices;
     const long long s_lo = (long long)a.ntiles * slice / slices;
@@ -64,8 +97,9 @@ void k_batch(BatchArgs a, unsigned long long *counters, int slices, int max_iter
     unsigned long long *counter = counters + (size_t)slice * kCounterStride;
     // wave-uniform tile state
     long long q_lo = 0, q_hi = 0;
-    int cur_obj = 0, cur_req = 0;
-    bool exhausted = false;
+    int cur_obj = 0, cur_cls = -1, cur_req = 0, pcls = -1;
+    bool exhausted = false, pend = false;
+    GnTile pt{0, 0, 0, 0, 0};
     // lane state
     long long idx = -1;
     int rq = 0, it = 0, lim = max_iter;
@@ -73,25 +107,33 @@ void k_batch(BatchArgs a, unsigned long long *counters, int slices, int max_iter
     double cu = 0, cv = 0, su = 0, sv = 0;
     T Px = 0, Py = 0, Pz = 0;
     SpanCache sc;
-    constexpr bool PREG = EXACT && sizeof(T) == 4 && K1 * K2 <= 16;
-    PatchRegs<T, K1 * K2> pr;
+    constexpr bool PREG = GRP == 0 && sizeof(T) == 4;
+    PatchRegs<T, 16> pr;
     while (true) {
-        while (!exhausted) {
+        while (true) {
             const unsigned long long idle = __ballot(idx < 0);
             if (idle == 0ull) break;
             if (q_lo >= q_hi) {
-                unsigned long long k = 0;
-                if (lane == 0) k = atomicAdd(counter, 1ull);
-                k = bcast64(k, 0);
-                if (s_lo + (long long)k >= s_hi) {
-                    exhausted = true;
-                    break;
+                if (!pend) {
+                    if (exhausted) break;
+                    unsigned long long k = 0;
+                    if (lane == 0) k = atomicAdd(counter, 1ull);
+                    k = bcast64(k, 0);
+                    if (s_lo + (long long)k >= s_hi) {
+                        exhausted = true;
+                        break;
+                    }
+                    pt = a.tiles[s_lo + (long long)k];
+                    pcls = a.objs[pt.obj].cls;
+                    pend = true;
                 }
-                const GnTile pt = a.tiles[s_lo + (long long)k];
+                if (pcls != cur_cls && idle != all) break;  // drain the current class first
                 cur_obj = pt.obj;
+                cur_cls = pcls;
                 cur_req = pt.req;
                 q_lo = pt.start;
                 q_hi = pt.start + pt.count;
+                pend = false;
                 continue;
             }
             const unsigned long long avail = (unsigned long long)(q_hi - q_lo);
@@ -102,14 +144,13 @@ void k_batch(BatchArgs a, unsigned long long *counters, int slices, int max_iter
                     rq = cur_req;
                     d = a.objs[cur_obj].s;
                     const GnReq &R = a.reqs[cur_req];
-                    const double P[3] = {R.x[idx], R.y[idx], R.z[idx]};
-                    Px = T(P[0]);
-                    Py = T(P[1]);
-                    Pz = T(P[2]);
-                    it = 0;
-                    lim = max_iter;
+                    Px = T(R.x[idx]);
+                    Py = T(R.y[idx]);
+                    Pz = T(R.z[idx]);
                     cu = R.u[idx];  // the seed, or the FP32 result for the closing pass
                     cv = R.v[idx];
+                    it = 0;
+                    lim = max_iter;
                     if (MODE == kClose) {
                         const int sb = R.status[idx];
                         if (sb == GN_PT_SINGULAR || sb == GN_PT_NONFINITE) lim = restart_iter;
@@ -128,9 +169,8 @@ void k_batch(BatchArgs a, unsigned long long *counters, int slices, int max_iter
             if (it >= lim) {
                 st = GN_PT_MAXITER;
             } else {
-                st = gn_step<K1, K2, EXACT, T, NEWTON, PREG>(d, d.ku, d.kv, Px, Py, Pz, cu, cv,
-                                                             tol_dist, tol_step, tol_quad,
-                                                             a.o.clamp, domain_of(d), sc, &pr);
+                st = step_group<GRP, T, NEWTON, PREG>(cur_cls, d, Px, Py, Pz, cu, cv, tol_dist,
+                                                      tol_step, tol_quad, a.o.clamp, sc, pr.m);
                 if (st == kStepContinue || st == GN_PT_CONVERGED) it++;
                 if (st == kStepContinue && it >= lim) st = GN_PT_MAXITER;
             }
@@ -170,14 +210,14 @@ static void batch_shape(const void *kernel, int ntiles, size_t n, int bpc_cap, u
     if (slices > kMaxSlices) slices = kMaxSlices;
 }
 
-// One launch of a class kernel over the class's tiles.
-template <int K1, int K2, bool EXACT, typename T, bool NEWTON, int MODE>
-static int launch_cls(const BatchArgs &a, size_t n, int bpc_cap, int max_iter, int restart_iter,
+// One launch of a group kernel over the group's tiles.
+template <int GRP, typename T, bool NEWTON, int MODE>
+static int launch_grp(const BatchArgs &a, size_t n, int bpc_cap, int max_iter, int restart_iter,
                       double tol_dist, double tol_step, double tol_quad, hipStream_t st) {
     unsigned long long *ctr = nullptr;
     int rc = work_counters(st, &ctr);
     if (rc) return rc;
-    auto k = k_batch<K1, K2, EXACT, T, NEWTON, MODE>;
+    auto k = k_batch<GRP, T, NEWTON, MODE>;
     unsigned grid;
     int slices;
     batch_shape((const void *)k, a.ntiles, n, bpc_cap, grid, slices);
@@ -187,39 +227,31 @@ static int launch_cls(const BatchArgs &a, size_t n, int bpc_cap, int max_iter, i
     return GN_OK;
 }
 
-// The precision phases of one degree class (sizes K1, K2).
-struct ClassPhases {
-    const BatchArgs &a;
-    size_t n;
-    int bpc_cap;
-    hipStream_t st;
-    template <int K1, int K2, bool EXACT>
-    int operator()() {
-        const gn_invert_opts &o = a.o;
-        const bool n32 = o.newton & GN_NEWTON_FP32, n64 = o.newton & GN_NEWTON_FP64;
-        int rc;
-        if (o.precision == GN_MIXED) {
-            rc = n32 ? launch_cls<K1, K2, EXACT, float, true, kBulk>(a, n, bpc_cap, o.fp32_iters, 0,
-                                                                     0.0, o.tol_step32, 0.0, st)
-                     : launch_cls<K1, K2, EXACT, float, false, kBulk>(a, n, bpc_cap, o.fp32_iters,
-                                                                      0, 0.0, o.tol_step32, 0.0, st);
-            if (rc) return rc;
-            return n64 ? launch_cls<K1, K2, EXACT, double, true, kClose>(
-                             a, n, bpc_cap, o.fp64_tail, o.max_iter, o.tol_dist, o.tol_step,
-                             o.tol_quad, st)
-                       : launch_cls<K1, K2, EXACT, double, false, kClose>(
-                             a, n, bpc_cap, o.fp64_tail, o.max_iter, o.tol_dist, o.tol_step, 0.0,
-                             st);
-        }
-        return n64 ? launch_cls<K1, K2, EXACT, double, true, kPlain>(
-                         a, n, bpc_cap, o.max_iter, 0, o.tol_dist, o.tol_step, o.tol_quad, st)
-                   : launch_cls<K1, K2, EXACT, double, false, kPlain>(
-                         a, n, bpc_cap, o.max_iter, 0, o.tol_dist, o.tol_step, 0.0, st);
+// The precision phases of one class group.
+template <int GRP>
+static int group_phases(const BatchArgs &a, size_t n, int bpc_cap, hipStream_t st) {
+    const gn_invert_opts &o = a.o;
+    const bool n32 = o.newton & GN_NEWTON_FP32, n64 = o.newton & GN_NEWTON_FP64;
+    int rc;
+    if (o.precision == GN_MIXED) {
+        rc = n32 ? launch_grp<GRP, float, true, kBulk>(a, n, bpc_cap, o.fp32_iters, 0, 0.0,
+                                                        o.tol_step32, 0.0, st)
+                 : launch_grp<GRP, float, false, kBulk>(a, n, bpc_cap, o.fp32_iters, 0, 0.0,
+                                                         o.tol_step32, 0.0, st);
+        if (rc) return rc;
+        return n64 ? launch_grp<GRP, double, true, kClose>(a, n, bpc_cap, o.fp64_tail, o.max_iter,
+                                                            o.tol_dist, o.tol_step, o.tol_quad, st)
+                   : launch_grp<GRP, double, false, kClose>(a, n, bpc_cap, o.fp64_tail, o.max_iter,
+                                                             o.tol_dist, o.tol_step, 0.0, st);
     }
-};
+    return n64 ? launch_grp<GRP, double, true, kPlain>(a, n, bpc_cap, o.max_iter, 0, o.tol_dist,
+                                                        o.tol_step, o.tol_quad, st)
+               : launch_grp<GRP, double, false, kPlain>(a, n, bpc_cap, o.max_iter, 0, o.tol_dist,
+                                                         o.tol_step, 0.0, st);
+}
 
 int batch_waves(int bpc_cap) {
-    auto k = k_batch<4, 4, true, float, true, kBulk>;
+    auto k = k_batch<0, float, true, kBulk>;
     int bpc = blocks_per_cu((const void *)k, kInvBlock);
     if (bpc_cap > 0 && bpc_cap < bpc) bpc = bpc_cap;
     return cu_count() * bpc * (kInvBlock / 64);
@@ -232,11 +264,11 @@ int class_tile(size_t points, int waves, int cap) {
     return (int)t;
 }
 
-std::vector<ClassRange> class_ranges(const GnTile *tiles, size_t ntiles, const std::vector<int> &cls) {
-    std::vector<ClassRange> r;
+std::vector<GroupRange> group_ranges(const GnTile *tiles, size_t ntiles, const std::vector<int> &cls) {
+    std::vector<GroupRange> r;
     for (size_t t = 0; t < ntiles; t++) {
-        const int c = cls[tiles[t].obj];
-        if (r.empty() || r.back().cls != c) r.push_back(ClassRange{c, (int)t, (int)t, 0});
+        const int g = class_group(cls[tiles[t].obj]);
+        if (r.empty() || r.back().grp != g) r.push_back(GroupRange{g, (int)t, (int)t, 0});
         r.back().t1 = (int)t + 1;
         r.back().points += (size_t)tiles[t].count;
     }
@@ -267,7 +299,7 @@ SideStreams g_side[64];  // per device, created on first use
 }  // namespace
 
 static int launch_ranges(const GnBatchObj *objs, const GnTile *tiles,
-                         const std::vector<ClassRange> &ranges, const GnReq *reqs,
+                         const std::vector<GroupRange> &ranges, const GnReq *reqs,
                          const gn_invert_opts &o, int bpc_cap, hipStream_t st,
                          const SideStreams &ss) {
     if (o.seed_mode != GN_SEED_GIVEN) {
@@ -276,36 +308,26 @@ static int launch_ranges(const GnBatchObj *objs, const GnTile *tiles,
         k_batch_seed<<<grid, kSeedBlock, 0, st>>>(objs, tiles + t0, nt, reqs, o.seed_mode);
         GN_HIP(hipGetLastError());
     }
-    // largest classes first, each to the side stream with the least work so far
-    std::vector<int> order(ranges.size());
-    for (size_t i = 0; i < ranges.size(); i++) order[i] = (int)i;
-    std::sort(order.begin(), order.end(),
-              [&](int x, int y) { return ranges[x].points > ranges[y].points; });
-    size_t load[kSideStreams] = {};
-    bool used[kSideStreams] = {};
-    GN_HIP(hipEventRecord(ss.fork, st));
-    for (int i : order) {
-        const ClassRange &cr = ranges[i];
-        if (cr.points == 0) continue;
-        int k = 0;
-        for (int j = 1; j < kSideStreams; j++)
-            if (load[j] < load[k]) k = j;
-        if (!used[k]) {
-            GN_HIP(hipStreamWaitEvent(ss.s[k], ss.fork, 0));
-            used[k] = true;
-        }
-        load[k] += cr.points;
-        const BatchArgs a{objs, tiles + cr.t0, cr.t1 - cr.t0, reqs, o};
-        ClassPhases f{a, cr.points, bpc_cap, ss.s[k]};
-        int rc;
-        if (cr.cls >= 0 && cr.cls < 16) rc = dispatch_degrees(cr.cls / 4 + 2, cr.cls % 4 + 2, f);
-        else rc = f.template operator()<GN_MAXK, GN_MAXK, false>();
-        if (rc) return rc;
+    if (ranges.size() == 1) {  // one group: no fork
+        const GroupRange &gr = ranges[0];
+        const BatchArgs a{objs, tiles + gr.t0, gr.t1 - gr.t0, reqs, o};
+        return gr.grp == 0 ? group_phases<0>(a, gr.points, bpc_cap, st)
+             : gr.grp == 1 ? group_phases<1>(a, gr.points, bpc_cap, st)
+                           : group_phases<2>(a, gr.points, bpc_cap, st);
     }
-    for (int k = 0; k < kSideStreams; k++) {
-        if (!used[k]) continue;
+    GN_HIP(hipEventRecord(ss.fork, st));
+    int k = 0;
+    for (const GroupRange &gr : ranges) {
+        if (gr.points == 0) continue;
+        GN_HIP(hipStreamWaitEvent(ss.s[k], ss.fork, 0));
+        const BatchArgs a{objs, tiles + gr.t0, gr.t1 - gr.t0, reqs, o};
+        const int rc = gr.grp == 0 ? group_phases<0>(a, gr.points, bpc_cap, ss.s[k])
+                     : gr.grp == 1 ? group_phases<1>(a, gr.points, bpc_cap, ss.s[k])
+                                   : group_phases<2>(a, gr.points, bpc_cap, ss.s[k]);
+        if (rc) return rc;
         GN_HIP(hipEventRecord(ss.join[k], ss.s[k]));
         GN_HIP(hipStreamWaitEvent(st, ss.join[k], 0));
+        k++;
     }
     return GN_OK;
 }
@@ -326,7 +348,7 @@ int batch_prepare() {
 }
 
 int launch_batch_invert(const GnBatchObj *objs, const GnTile *tiles,
-                        const std::vector<ClassRange> &ranges, const GnReq *reqs,
+                        const std::vector<GroupRange> &ranges, const GnReq *reqs,
                         const gn_invert_opts &o, int bpc_cap, hipStream_t st,
                         const SideStreams *side) {
     if (ranges.empty()) return GN_OK;

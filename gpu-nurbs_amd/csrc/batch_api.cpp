@@ -52,24 +52,24 @@ int check_segments(const gn_batch *b, const gn_segment *segs, int nseg, size_t n
     return GN_OK;
 }
 
-// Tiles of every segment, sorted by degree class; each class gets its own tile size (at most
-// `tile`), sized for the class's own launch.
+// Tiles of every segment, sorted by class group, then class; each group gets its own tile size
+// (at most `tile`), sized for the group's own launch.
 std::vector<GnTile> build_tiles(const gn_batch *b, const gn_segment *segs, int nseg, int tile) {
-    std::vector<size_t> per(gn::kClasses, 0);
-    for (int i = 0; i < nseg; i++) per[b->cls[segs[i].obj]] += (size_t)segs[i].count;
+    size_t per[3] = {0, 0, 0};
+    for (int i = 0; i < nseg; i++) per[gn::class_group(b->cls[segs[i].obj])] += (size_t)segs[i].count;
     const int waves = gn::batch_waves(b->bpc);
-    std::vector<int> tsz(gn::kClasses);
-    for (int c = 0; c < gn::kClasses; c++) tsz[c] = gn::class_tile(per[c], waves, tile);
+    int tsz[3];
+    for (int g = 0; g < 3; g++) tsz[g] = gn::class_tile(per[g], waves, tile);
     std::vector<GnTile> t;
     for (int i = 0; i < nseg; i++) {
-        const int64_t T = tsz[b->cls[segs[i].obj]];
+        const int64_t T = tsz[gn::class_group(b->cls[segs[i].obj])];
         for (int64_t off = 0; off < segs[i].count; off += T) {
             const int64_t c = std::min<int64_t>(T, segs[i].count - off);
             t.push_back(GnTile{segs[i].obj, (int32_t)c, segs[i].start + off, 0, 0});
         }
     }
     std::stable_sort(t.begin(), t.end(), [b](const GnTile &x, const GnTile &y) {
-        return b->cls[x.obj] < b->cls[y.obj];
+        return gn::class_order(b->cls[x.obj]) < gn::class_order(b->cls[y.obj]);
     });
     return t;
 }
@@ -79,13 +79,13 @@ std::vector<GnTile> build_tiles(const gn_batch *b, const gn_segment *segs, int n
 struct CallImage {
     std::vector<char> bytes;
     size_t ntiles = 0, req_off = 0;
-    std::vector<gn::ClassRange> ranges;
+    std::vector<gn::GroupRange> ranges;
 };
 
 CallImage call_image(const gn_batch *b, const std::vector<GnTile> &t, const GnReq &r) {
     CallImage c;
     c.ntiles = t.size();
-    c.ranges = gn::class_ranges(t.data(), t.size(), b->cls);
+    c.ranges = gn::group_ranges(t.data(), t.size(), b->cls);
     c.req_off = align256(sizeof(GnTile) * t.size());
     c.bytes.resize(c.req_off + sizeof(GnReq));
     if (!t.empty()) std::memcpy(c.bytes.data(), t.data(), sizeof(GnTile) * t.size());
@@ -113,7 +113,7 @@ int run_batch(gn_batch *b, const gn_segment *segs, int nseg, const double *x, co
         return gn::hip_fail(e, "hipMallocAsync(tiles)");
     }
     const size_t req_off = img->req_off;
-    const std::vector<gn::ClassRange> ranges = img->ranges;
+    const std::vector<gn::GroupRange> ranges = img->ranges;
     e = hipMemcpyAsync(d, img->bytes.data(), img->bytes.size(), hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipLaunchHostFunc(st, free_image_host, img);
     if (e != hipSuccess) {

@@ -133,6 +133,7 @@ __device__ __forceinline__ void contract(const T *__restrict__ patch, int k1, in
 
 // A lane's current span patch held in registers (exact sizes only): reloaded from the table only
 // when the lane's span pair changes, so iterations that stay in their span issue no gathers.
+// Kernels declare one array of the largest size they use and pass its first element.
 template <typename T, int N>
 struct PatchRegs {
     Hom<T> m[N];
@@ -220,7 +221,7 @@ template <int K1, int K2, int ORDER, bool EXACT, typename T, bool PREG = false>
 __device__ __forceinline__ void eval_point_cached(const GnSurfDev &d, const double *__restrict__ ku,
                                                   const double *__restrict__ kv, double u, double v,
                                                   SpanCache &sc, Cart<T> &o,
-                                                  PatchRegs<T, K1 * K2> *pr = nullptr) {
+                                                  Hom<T> *pr = nullptr) {
     bool moved = false;
     if (!(u >= sc.ulo && u < sc.uhi)) {
         const int m = find_span(ku, d.p, d.nu, u);
@@ -250,9 +251,9 @@ __device__ __forceinline__ void eval_point_cached(const GnSurfDev &d, const doub
         static_assert(EXACT, "register patch cache needs exact sizes");
         if (moved) {
 #pragma unroll
-            for (int e = 0; e < K1 * K2; e++) pr->m[e] = load_hom<T>(patch + e * 4);
+            for (int e = 0; e < K1 * K2; e++) pr[e] = load_hom<T>(patch + e * 4);
         }
-        contract_with<K1, K2, ORDER, true, T>([&](int r, int c) { return pr->m[r * K2 + c]; }, k1,
+        contract_with<K1, K2, ORDER, true, T>([&](int r, int c) { return pr[r * K2 + c]; }, k1,
                                               k2, t, s, h);
     } else {
         (void)moved;

@@ -171,16 +171,26 @@ int launch_search_build(const GnSurfDev &d, int levels, const GnTreeNode *tree, 
 int launch_seed(const GnSearchDev &sd, int mode, const double *x, const double *y,
                 const double *z, size_t n, int32_t *leaf, double *su, double *sv,
                 hipStream_t st);
-// A run of tiles of one degree class in a class-sorted tile list, and its point count.
-struct ClassRange {
-    int cls, t0, t1;
+// Degree classes: (p-2)*4 + (q-2) for p, q in 2..5 (exact kernels), 16 for any other degrees
+// (padded generic kernel).  Classes form three groups by patch size (k1*k2 <= 16, <= 25, larger
+// and generic); each group has its own kernel, whose register footprint is that of its largest
+// class.  Tile lists are sorted by class_order (group, then class).
+constexpr int kClasses = 17;
+__host__ __device__ constexpr int class_group(int cls) {
+    return cls < 0 || cls >= 16 ? 2
+         : (cls / 4 + 3) * (cls % 4 + 3) <= 16 ? 0
+         : (cls / 4 + 3) * (cls % 4 + 3) <= 25 ? 1 : 2;
+}
+inline int class_order(int cls) { return class_group(cls) * 32 + cls; }
+// A run of tiles of one class group in a class_order-sorted tile list, and its point count.
+struct GroupRange {
+    int grp, t0, t1;
     size_t points;
 };
-std::vector<ClassRange> class_ranges(const GnTile *tiles, size_t ntiles, const std::vector<int> &cls);
-// Points per tile for one class launch of `points` points: about two tiles per resident wave,
-// between 64 and cap (a tile is the unit a wave claims; too large a tile leaves waves idle).
+std::vector<GroupRange> group_ranges(const GnTile *tiles, size_t ntiles, const std::vector<int> &cls);
+// Points per tile for a launch of `points` points: about two tiles per resident wave, between 64
+// and cap (a tile is the unit a wave claims; too large a tile leaves waves idle).
 int class_tile(size_t points, int waves, int cap);
-constexpr int kClasses = 17;  // 16 exact degree pairs + the generic class
 // Side streams and fork/join events for running independent launches concurrently.  A set is
 // used by one caller at a time (the batched entry points share one per device under a lock; a
 // server owns one per step slot).
@@ -193,11 +203,11 @@ int side_streams_create(SideStreams *ss);
 // Creates the device's shared side-stream set (call before a graph capture).
 int batch_prepare();
 void side_streams_destroy(SideStreams *ss);
-// One persistent launch per class range and precision phase.  The class ranges are spread over
-// the side streams (forked from and joined back into st); with side == nullptr the device's
-// shared set is used.
+// Seeds (unless GN_SEED_GIVEN), then one persistent launch per group range and precision phase;
+// the groups run concurrently on the side streams (forked from and joined back into st); with
+// side == nullptr the device's shared set is used.
 int launch_batch_invert(const GnBatchObj *objs, const GnTile *tiles,
-                        const std::vector<ClassRange> &ranges, const GnReq *reqs,
+                        const std::vector<GroupRange> &ranges, const GnReq *reqs,
                         const gn_invert_opts &o, int bpc_cap, hipStream_t st,
                         const SideStreams *side = nullptr);
 // resident waves of one persistent batch launch (sizes tiles for short serving steps)

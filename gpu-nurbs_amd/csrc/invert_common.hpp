@@ -91,7 +91,7 @@ __device__ __forceinline__ int gn_step(const GnSurfDev &d, const double *__restr
                                        const double *__restrict__ kv, T Px, T Py, T Pz,
                                        double &cu, double &cv, double tol_dist, double tol_step,
                                        double tol_quad, int clamp, const Domain &dom,
-                                       SpanCache &sc, PatchRegs<T, K1 * K2> *pr = nullptr) {
+                                       SpanCache &sc, Hom<T> *pr = nullptr) {
     Cart<T> o;
     eval_point_cached<K1, K2, NEWTON ? 2 : 1, EXACT, T, PREG>(d, ku, kv, cu, cv, sc, o, pr);
     const T r0 = o.S[0] - Px, r1 = o.S[1] - Py, r2 = o.S[2] - Pz;
@@ -243,7 +243,7 @@ __global__ __launch_bounds__(kInvBlock, (EXACT && sizeof(T) == 4 && K1 * K2 <= 1
             } else {
                 st = gn_step<K1, K2, EXACT, T, NEWTON, PREG>(d, ku, kv, Px, Py, Pz, cu, cv,
                                                              tol_dist, tol_step, tol_quad,
-                                                             a.o.clamp, dom, sc, &pr);
+                                                             a.o.clamp, dom, sc, pr.m);
                 if (st == kStepContinue || st == GN_PT_CONVERGED) it++;
                 if (st == kStepContinue && it >= lim) st = GN_PT_MAXITER;
             }

@@ -145,20 +145,19 @@ int gn_server::launch_step(Slot &sl) {
     // resident wave, at most the batch's tile; all grown by a common factor if the step would
     // exceed kMaxStepTiles (sum of ceil(n/T) <= pts/T + nreq).
     const size_t room = (size_t)(kMaxStepTiles - 2 * nreq);
-    std::vector<size_t> per(gn::kClasses, 0);
+    size_t per[3] = {0, 0, 0}, tsz[3];
     for (Req *r : sl.reqs)
-        if (!r->curve) per[cls[r->obj]] += r->n;
-    std::vector<size_t> tsz(gn::kClasses);
+        if (!r->curve) per[gn::class_group(cls[r->obj])] += r->n;
     size_t est = 0;
-    for (int c = 0; c < gn::kClasses; c++) {
-        tsz[c] = (size_t)gn::class_tile(per[c], waves, tile);
-        est += per[c] / tsz[c];
+    for (int g = 0; g < 3; g++) {
+        tsz[g] = (size_t)gn::class_tile(per[g], waves, tile);
+        est += per[g] / tsz[g];
     }
     for (size_t grow = 2; est > room / 2; grow *= 2) {
         est = 0;
-        for (int c = 0; c < gn::kClasses; c++) {
-            tsz[c] = (size_t)gn::class_tile(per[c], waves, tile) * grow;
-            est += per[c] / tsz[c];
+        for (int g = 0; g < 3; g++) {
+            tsz[g] = (size_t)gn::class_tile(per[g], waves, tile) * grow;
+            est += per[g] / tsz[g];
         }
     }
     size_t tc = kCurveTile;
@@ -168,7 +167,7 @@ int gn_server::launch_step(Slot &sl) {
     for (int k = 0; k < nreq; k++) {
         Req *r = sl.reqs[k];
         rq[k] = GnReq{r->x, r->y, r->z, r->u, r->v, r->status, nullptr, nullptr};
-        const size_t T = r->curve ? tc : tsz[cls[r->obj]];
+        const size_t T = r->curve ? tc : tsz[gn::class_group(cls[r->obj])];
         for (size_t off = 0; off < r->n; off += T) {
             const size_t c = std::min(T, r->n - off);
             GnTile t{r->curve ? r->obj - nsurf : r->obj, (int32_t)c, (int64_t)off, k, 0};
@@ -176,7 +175,9 @@ int gn_server::launch_step(Slot &sl) {
         }
     }
     std::stable_sort(st_tiles.begin(), st_tiles.end(),
-                     [this](const GnTile &a, const GnTile &b) { return cls[a.obj] < cls[b.obj]; });
+                     [this](const GnTile &a, const GnTile &b) {
+                         return gn::class_order(cls[a.obj]) < gn::class_order(cls[b.obj]);
+                     });
     const size_t ns = st_tiles.size(), nc = ct_tiles.size();
     if (ns + nc > (size_t)kMaxStepTiles) return fail(GN_EDEVICE, "step tile list overflow");
     // staging image: request table, then the tiles (256-byte aligned); one copy of what is used
@@ -189,7 +190,7 @@ int gn_server::launch_step(Slot &sl) {
     const GnReq *d_rq = (const GnReq *)sl.d_stage;
     const GnTile *d_tl = (const GnTile *)(sl.d_stage + toff);
     int rc = GN_OK;
-    if (ns) rc = gn::launch_batch_invert(d_objs, d_tl, gn::class_ranges(st_tiles.data(), ns, cls),
+    if (ns) rc = gn::launch_batch_invert(d_objs, d_tl, gn::group_ranges(st_tiles.data(), ns, cls),
                                          d_rq, o, bpc, sl.st, &sl.side);
     if (!rc && nc) rc = gn::launch_curve_batch(d_curves, d_tl + ns, (int)nc, d_rq, max_samples, o, sl.st);
     if (rc) return rc;
