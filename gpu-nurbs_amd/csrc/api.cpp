@@ -65,6 +65,8 @@ static void free_tree(gn_surface *s) {
     (void)hipFree(s->d_sseed);
     (void)hipFree(s->d_sbox32);
     (void)hipFree(s->d_sref);
+    (void)hipFree(s->d_sleaf32);
+    s->d_sleaf32 = nullptr;
     s->d_tree = nullptr;
     s->d_valid = s->d_divided = nullptr;
     s->d_sbox = s->d_scentre = s->d_sseed = nullptr;
@@ -294,7 +296,8 @@ int gn_build_tree(gn_surface *s, int levels, int flags, gn_stream stream) {
         (rc = dalloc(&s->d_sbox32, (size_t)qnodes * 6, acc)) ||
         (rc = dalloc(&s->d_scentre, (size_t)leaves * 3, acc)) ||
         (rc = dalloc(&s->d_sseed, (size_t)leaves * 2, acc)) ||
-        (rc = dalloc(&s->d_sref, (size_t)leaves, acc))) {
+        (rc = dalloc(&s->d_sref, (size_t)leaves, acc)) ||
+        (rc = dalloc(&s->d_sleaf32, (size_t)leaves, acc))) {
         free_tree(s);
         return rc;
     }
@@ -304,7 +307,7 @@ int gn_build_tree(gn_surface *s, int levels, int flags, gn_stream stream) {
         if (rc) { free_tree(s); return rc; }
     }
     rc = gn::launch_search_build(d, levels, s->d_tree, s->d_sbox, s->d_sbox32, s->d_scentre,
-                                 s->d_sseed, s->d_sref, st);
+                                 s->d_sseed, s->d_sref, s->d_sleaf32, st);
     if (rc) { free_tree(s); return rc; }
     s->tree_levels = levels;
     s->device_bytes += acc;

@@ -39,6 +39,8 @@ struct GnSearchDev {
     const double *centre;     // per leaf (Morton order) centre point xyz
     const double *seed_uv;    // per leaf (Morton order) seed (u, v)
     const int32_t *ref_index; // per leaf (Morton order) reference tree index
+    const float4 *leaf32;     // per leaf (Morton order) centre in FP32 and the reference index
+                              // (bits of w), one 16-byte load for the greedy 3 x 3 search
 };
 
 // One object of a batch: the surface and its seed tree (pointers into the batch's pack) and its
@@ -111,6 +113,7 @@ struct gn_surface {
     double *d_sbox = nullptr, *d_scentre = nullptr, *d_sseed = nullptr;
     float *d_sbox32 = nullptr;
     int32_t *d_sref = nullptr;
+    float4 *d_sleaf32 = nullptr;
     size_t device_bytes = 0;
 
     GnSurfDev dev() const {
@@ -126,6 +129,7 @@ struct gn_surface {
         s.depth = 3 * (tree_levels - 1);
         s.box = d_sbox; s.box32 = d_sbox32; s.centre = d_scentre; s.seed_uv = d_sseed;
         s.ref_index = d_sref;
+        s.leaf32 = d_sleaf32;
         return s;
     }
 };
@@ -166,7 +170,7 @@ int launch_tree_level(const GnSurfDev &d, int level, int flags, GnTreeNode *tree
                       uint8_t *valid, uint8_t *divided, hipStream_t st);
 int launch_search_build(const GnSurfDev &d, int levels, const GnTreeNode *tree, double *sbox,
                         float *sbox32, double *scentre, double *sseed, int32_t *sref,
-                        hipStream_t st);
+                        float4 *sleaf32, hipStream_t st);
 // mode 0: exact nearest leaf centre; mode 1: greedy descent + 3x3 leaf neighbourhood
 int launch_seed(const GnSearchDev &sd, int mode, const double *x, const double *y,
                 const double *z, size_t n, int32_t *leaf, double *su, double *sv,

@@ -116,8 +116,9 @@ __device__ __forceinline__ uint32_t greedy_child(const float *__restrict__ level
 }
 
 // Fast seed: greedy descent by FP32 box distance (ties to the lower child), then the nearest
-// leaf centre (FP64, ties to the lower reference index) among the 3 x 3 leaves around the
-// descent's leaf in the uv grid.  Fixed work per point: 4*D box tests and 9 distances.
+// leaf centre (FP32 centres and distances, ties to the lower reference index) among the 3 x 3
+// leaves around the descent's leaf in the uv grid.  Fixed work per point: 4*D box tests and 9
+// distances.
 // lds, when given, holds the FP32 boxes of levels 1 .. lds_levels (the first part of box32 after
 // the root), so those levels of the descent read LDS.
 __device__ inline uint32_t greedy_leaf(const GnSearchDev &sd, const double P[3],
@@ -131,20 +132,30 @@ __device__ inline uint32_t greedy_leaf(const GnSearchDev &sd, const double P[3],
     for (; l < D; l++) c = greedy_child(sd.box32 + qoff(l + 1) * 6, c, p);
     const int side = 1 << D;
     const int gx = (int)compact2(c >> 1), gy = (int)compact2(c);
-    double best = INFINITY;
+    // the 9 neighbour leaves (FP32 centre + reference index, 16 bytes each) are loaded before
+    // any is compared (addresses clamped into the grid, out-of-grid neighbours skipped after)
+    uint32_t mk[9];
+    float4 q[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        const int x = min(max(gx + k / 3 - 1, 0), side - 1), y = min(max(gy + k % 3 - 1, 0), side - 1);
+        mk[k] = morton2((uint32_t)x, (uint32_t)y);
+        q[k] = sd.leaf32[mk[k]];
+    }
+    float best = INFINITY;
+    int best_ref = 0x7fffffff;
     uint32_t best_c = c;
-    for (int dx = -1; dx <= 1; dx++) {
-        for (int dy = -1; dy <= 1; dy++) {
-            const int x = gx + dx, y = gy + dy;
-            if (x < 0 || y < 0 || x >= side || y >= side) continue;
-            const uint32_t m = morton2((uint32_t)x, (uint32_t)y);
-            const double *q = sd.centre + m * 3;
-            const double ex = P[0] - q[0], ey = P[1] - q[1], ez = P[2] - q[2];
-            const double dd = fma(ez, ez, fma(ey, ey, ex * ex));
-            if (dd < best || (dd == best && sd.ref_index[m] < sd.ref_index[best_c])) {
-                best = dd;
-                best_c = m;
-            }
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        const int x = gx + k / 3 - 1, y = gy + k % 3 - 1;
+        if (x < 0 || y < 0 || x >= side || y >= side) continue;
+        const float ex = p[0] - q[k].x, ey = p[1] - q[k].y, ez = p[2] - q[k].z;
+        const float dd = fmaf(ez, ez, fmaf(ey, ey, ex * ex));
+        const int ref = __float_as_int(q[k].w);
+        if (dd < best || (dd == best && ref < best_ref)) {
+            best = dd;
+            best_ref = ref;
+            best_c = mk[k];
         }
     }
     return best_c;

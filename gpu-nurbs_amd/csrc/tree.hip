@@ -145,7 +145,8 @@ int launch_tree_level(const GnSurfDev &d, int level, int flags, GnTreeNode *tree
 // leaves: one thread per finest-level node
 __global__ void k_search_leaves(int levels, const GnTreeNode *__restrict__ tree,
                                 double *__restrict__ sbox, double *__restrict__ scentre,
-                                double *__restrict__ sseed, int32_t *__restrict__ sref) {
+                                double *__restrict__ sseed, int32_t *__restrict__ sref,
+                                float4 *__restrict__ sleaf32) {
     long g = 1, base = 0;
     for (int l = 0; l < levels - 1; l++) { base += g * g; g *= 8; }
     const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -163,6 +164,8 @@ __global__ void k_search_leaves(int levels, const GnTreeNode *__restrict__ tree,
     sseed[c * 2 + 0] = (nd.uv_lb[0] + nd.uv_rt[0]) / 2;
     sseed[c * 2 + 1] = (nd.uv_lb[1] + nd.uv_rt[1]) / 2;
     sref[c] = (int32_t)(base + idx);
+    sleaf32[c] = make_float4((float)nd.c[0], (float)nd.c[1], (float)nd.c[2],
+                             __int_as_float((int32_t)(base + idx)));
 }
 
 // internal level l from level l+1
@@ -202,12 +205,12 @@ __global__ void k_search_box32(long total, const double *__restrict__ sbox,
 
 int launch_search_build(const GnSurfDev &d, int levels, const GnTreeNode *tree, double *sbox,
                         float *sbox32, double *scentre, double *sseed, int32_t *sref,
-                        hipStream_t st) {
+                        float4 *sleaf32, hipStream_t st) {
     (void)d;
     const int D = 3 * (levels - 1);
     const long leaves = 1L << (2 * D);
     k_search_leaves<<<grid_for(leaves, 256, 1u << 30), 256, 0, st>>>(levels, tree, sbox, scentre,
-                                                                       sseed, sref);
+                                                                       sseed, sref, sleaf32);
     for (int l = D - 1; l >= 0; l--)
         k_search_level<<<grid_for(1L << (2 * l), 256, 1u << 30), 256, 0, st>>>(l, sbox);
     const long total = qoff(D + 1);
@@ -227,11 +230,15 @@ __global__ __launch_bounds__(256) void k_seed(GnSearchDev sd, int mode,
                                               const double *__restrict__ z, size_t n,
                                               int32_t *__restrict__ leaf, double *__restrict__ su,
                                               double *__restrict__ sv) {
-    __shared__ float lbox[(((1 << (2 * (kSeedLdsLevels + 1))) - 1) / 3 - 1) * 6];
+    __shared__ __align__(16) float lbox[(((1 << (2 * (kSeedLdsLevels + 1))) - 1) / 3 - 1) * 6];
     const int ll = mode == 1 ? (sd.depth < kSeedLdsLevels ? sd.depth : kSeedLdsLevels) : 0;
     if (ll > 0) {
-        const int nf = (int)(qoff(ll + 1) - 1) * 6;
-        for (int i = threadIdx.x; i < nf; i += blockDim.x) lbox[i] = sd.box32[6 + i];
+        // boxes of levels 1..ll start one box (24 bytes) into box32: float2 copies
+        const int nf2 = (int)(qoff(ll + 1) - 1) * 3;
+        const float2 *src = reinterpret_cast<const float2 *>(sd.box32 + 6);
+        float2 *dst = reinterpret_cast<float2 *>(lbox);
+#pragma unroll 4
+        for (int i = threadIdx.x; i < nf2; i += blockDim.x) dst[i] = src[i];
         __syncthreads();
     }
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;

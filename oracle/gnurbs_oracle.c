@@ -640,7 +640,8 @@ void or_seed_nearest_leaf(const double *nodes, int levels, const double *x, cons
  * finest level with boxes = leaf box U leaf centre, unions upward; FP32 copies rounded outward and
  * widened by (1 + max|root box|) * 2^-20; greedy descent by FP32 squared box distance (ties to
  * the lower child, children ordered (x,y) = (0,0),(0,1),(1,0),(1,1)); then the nearest leaf
- * centre (FP64, ties to the lower reference index) among the 3 x 3 leaves around it. */
+ * centre (FP32 centres and distances, ties to the lower reference index) among the 3 x 3 leaves
+ * around it. */
 void or_seed_greedy_leaf(const double *nodes, int levels, const double *x, const double *y,
                          const double *z, long n, int *leaf_out, double *seed_u, double *seed_v)
 {
@@ -712,7 +713,8 @@ void or_seed_greedy_leaf(const double *nodes, int levels, const double *x, const
             ix = bxx;
             iy = byy;
         }
-        double best = INFINITY;
+        /* 3 x 3 leaves around the descent's leaf: FP32 centres and distances */
+        float best = INFINITY;
         long bi = base + ix * side + iy;
         for (int dx = -1; dx <= 1; dx++)
             for (int dy = -1; dy <= 1; dy++) {
@@ -720,8 +722,8 @@ void or_seed_greedy_leaf(const double *nodes, int levels, const double *x, const
                 if (xx < 0 || yy < 0 || xx >= side || yy >= side) continue;
                 long t = base + xx * side + yy;
                 const double *q = nodes + t * 13 + 10;
-                double ex = P[0] - q[0], ey = P[1] - q[1], ez = P[2] - q[2];
-                double dd = fma(ez, ez, fma(ey, ey, ex * ex));
+                float ex = p[0] - (float)q[0], ey = p[1] - (float)q[1], ez = p[2] - (float)q[2];
+                float dd = fmaf(ez, ez, fmaf(ey, ey, ex * ex));
                 if (dd < best || (dd == best && t < bi)) { best = dd; bi = t; }
             }
         leaf_out[i] = (int)bi;
