@@ -31,23 +31,26 @@ struct BatchArgs {
     gn_invert_opts o;
 };
 
-// Seeds of every point of the tiles into its request's u, v (one workgroup per tile).
-constexpr int kSeedBlock = 128;
+// Seeds of every point of the tiles into its request's u, v.  Workgroup b takes the points
+// (b % per_tile) * kSeedBlock .. + kSeedBlock - 1 of tile b / per_tile, one point per lane, so
+// every point has a lane of its own (the seed search is a chain of dependent loads).
+constexpr int kSeedBlock = 256;
 __global__ __launch_bounds__(kSeedBlock) void k_batch_seed(const GnBatchObj *__restrict__ objs,
                                                            const GnTile *__restrict__ tiles,
-                                                           int ntiles, const GnReq *__restrict__ reqs,
+                                                           long long nblocks, int per_tile,
+                                                           const GnReq *__restrict__ reqs,
                                                            int mode) {
-    for (int ti = blockIdx.x; ti < ntiles; ti += gridDim.x) {
-        const GnTile tl = tiles[ti];
-        const GnReq R = reqs[tl.req];
-        const GnSearchDev sd = objs[tl.obj].sd;
-        for (int j = threadIdx.x; j < tl.count; j += blockDim.x) {
-            const int64_t i = tl.start + j;
-            const double P[3] = {R.x[i], R.y[i], R.z[i]};
-            const uint32_t c = mode == GN_SEED_NEAREST_LEAF ? nearest_leaf(sd, P) : greedy_leaf(sd, P);
-            R.u[i] = sd.seed_uv[c * 2 + 0];
-            R.v[i] = sd.seed_uv[c * 2 + 1];
-        }
+    for (long long b = blockIdx.x; b < nblocks; b += gridDim.x) {
+        const GnTile tl = tiles[b / per_tile];
+        const int j = (int)(b % per_tile) * kSeedBlock + threadIdx.x;
+        if (j >= tl.count) continue;
+        const GnReq &R = reqs[tl.req];
+        const GnSearchDev &sd = objs[tl.obj].sd;
+        const int64_t i = tl.start + j;
+        const double P[3] = {R.x[i], R.y[i], R.z[i]};
+        const uint32_t c = mode == GN_SEED_NEAREST_LEAF ? nearest_leaf(sd, P) : greedy_leaf(sd, P);
+        R.u[i] = sd.seed_uv[c * 2 + 0];
+        R.v[i] = sd.seed_uv[c * 2 + 1];
     }
 }
 
@@ -268,9 +271,10 @@ std::vector<GroupRange> group_ranges(const GnTile *tiles, size_t ntiles, const s
     std::vector<GroupRange> r;
     for (size_t t = 0; t < ntiles; t++) {
         const int g = class_group(cls[tiles[t].obj]);
-        if (r.empty() || r.back().grp != g) r.push_back(GroupRange{g, (int)t, (int)t, 0});
+        if (r.empty() || r.back().grp != g) r.push_back(GroupRange{g, (int)t, (int)t, 0, 0});
         r.back().t1 = (int)t + 1;
         r.back().points += (size_t)tiles[t].count;
+        if (tiles[t].count > r.back().max_count) r.back().max_count = tiles[t].count;
     }
     return r;
 }
@@ -304,8 +308,12 @@ static int launch_ranges(const GnBatchObj *objs, const GnTile *tiles,
                          const SideStreams &ss) {
     if (o.seed_mode != GN_SEED_GIVEN) {
         const int t0 = ranges.front().t0, nt = ranges.back().t1 - t0;
-        const unsigned grid = (unsigned)(nt < 256 * 32 ? nt : 256 * 32);
-        k_batch_seed<<<grid, kSeedBlock, 0, st>>>(objs, tiles + t0, nt, reqs, o.seed_mode);
+        int maxc = 1;
+        for (const GroupRange &gr : ranges) maxc = gr.max_count > maxc ? gr.max_count : maxc;
+        const int per_tile = (maxc + kSeedBlock - 1) / kSeedBlock;
+        const long long nb = (long long)nt * per_tile;
+        const unsigned grid = (unsigned)(nb < (1 << 20) ? nb : (1 << 20));
+        k_batch_seed<<<grid, kSeedBlock, 0, st>>>(objs, tiles + t0, nb, per_tile, reqs, o.seed_mode);
         GN_HIP(hipGetLastError());
     }
     if (ranges.size() == 1) {  // one group: no fork

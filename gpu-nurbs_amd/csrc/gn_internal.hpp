@@ -190,6 +190,7 @@ inline int class_order(int cls) { return class_group(cls) * 32 + cls; }
 struct GroupRange {
     int grp, t0, t1;
     size_t points;
+    int max_count;  // largest tile
 };
 std::vector<GroupRange> group_ranges(const GnTile *tiles, size_t ntiles, const std::vector<int> &cls);
 // Points per tile for a launch of `points` points: about two tiles per resident wave, between 64
