@@ -190,10 +190,22 @@ int gn_server::launch_step(Slot &sl) {
     const GnReq *d_rq = (const GnReq *)sl.d_stage;
     const GnTile *d_tl = (const GnTile *)(sl.d_stage + toff);
     int rc = GN_OK;
-    if (ns) rc = gn::launch_batch_invert(d_objs, d_tl, gn::group_ranges(st_tiles.data(), ns, cls),
-                                         d_rq, o, bpc, sl.st, &sl.side);
-    if (!rc && nc) rc = gn::launch_curve_batch(d_curves, d_tl + ns, (int)nc, d_rq, max_samples, o, sl.st);
+    // curves on the last side stream (the surface groups take the first ones), beside the
+    // surface seeds and group launches; joined back before the step's event
+    hipStream_t cs = sl.side.s[gn::kSideStreams - 1];
+    if (nc) {
+        GN_HIP(hipEventRecord(sl.side.fork, sl.st));
+        GN_HIP(hipStreamWaitEvent(cs, sl.side.fork, 0));
+        rc = gn::launch_curve_batch(d_curves, d_tl + ns, (int)nc, d_rq, max_samples, o, cs);
+    }
+    if (!rc && ns)
+        rc = gn::launch_batch_invert(d_objs, d_tl, gn::group_ranges(st_tiles.data(), ns, cls),
+                                     d_rq, o, bpc, sl.st, &sl.side);
     if (rc) return rc;
+    if (nc) {
+        GN_HIP(hipEventRecord(sl.side.join[gn::kSideStreams - 1], cs));
+        GN_HIP(hipStreamWaitEvent(sl.st, sl.side.join[gn::kSideStreams - 1], 0));
+    }
     GN_HIP(hipEventRecord(sl.ev, sl.st));
     return GN_OK;
 }
