@@ -65,7 +65,8 @@ __device__ __forceinline__ int step_group(int cls, const GnSurfDev &d, T Px, T P
 #define GN_CLS(P, Q)                                                                            \
     case (P - 2) * 4 + (Q - 2):                                                                 \
         if constexpr (class_group((P - 2) * 4 + (Q - 2)) == GRP)                                \
-            return gn_step<P + 1, Q + 1, true, T, NEWTON, PREG && (P + 1) * (Q + 1) <= 16>(     \
+            return gn_step<P + 1, Q + 1, true, T, NEWTON,                                       \
+                           (PREG && (P + 1) * (Q + 1) <= 16) ? kPatchRegs : kPatchGlobal>(     \
                 d, d.ku, d.kv, Px, Py, Pz, cu, cv, tol_dist, tol_step, tol_quad, clamp,         \
                 domain_of(d), sc, pr);                                                          \
         break;

@@ -66,11 +66,13 @@ __device__ __forceinline__ Hom<double> load_hom<double>(const double *__restrict
 // Contraction of one span patch M[r][s][c] (layout [K1][K2][4]) with the power bases of the
 // local parameters (t, s):  R = sum_rs t^r M[r][s] s^s  and its partial derivatives up to
 // ORDER (MatrixSurfacePoint / MatrixSurfaceRd*, src/NURBS/Evaluate.cu:159-502).  The loop runs
-// over s outermost so that only the 4-vectors of one patch column are live at a time.
-// K1/K2 are compile-time bounds; k1/k2 the runtime sizes (equal to K1/K2 when EXACT).
-template <int K1, int K2, int ORDER, bool EXACT, typename T, typename LOAD>
-__device__ __forceinline__ void contract_with(LOAD &&entry, int k1, int k2, T t, T s,
-                                              HomDer<T> &o) {
+// over the patch rows r outermost: row(r, m) fills m[0..k2-1] with the 4-vectors of row r (one
+// 128-byte line of an FP64 bicubic patch), so only one row is live at a time.  Every path
+// (global gathers, register-cached patch, LDS-staged rows) shares this summation order, so they
+// round identically.  K1/K2 are compile-time bounds; k1/k2 the runtime sizes (equal to K1/K2 when
+// EXACT).
+template <int K1, int K2, int ORDER, bool EXACT, typename T, typename ROW>
+__device__ __forceinline__ void contract_rows(ROW &&row, int k1, int k2, T t, T s, HomDer<T> &o) {
     if (EXACT) { k1 = K1; k2 = K2; }
     T U[K1], dU[K1], ddU[K1], V[K2], dV[K2], ddV[K2];
     {
@@ -96,29 +98,30 @@ __device__ __forceinline__ void contract_with(LOAD &&entry, int k1, int k2, T t,
     if (ORDER >= 1) { hzero(o.Ru); hzero(o.Rv); }
     if (ORDER >= 2) { hzero(o.Ruu); hzero(o.Ruv); hzero(o.Rvv); }
 #pragma unroll
-    for (int c = 0; c < K2; c++) {
-        if (!EXACT && c >= k2) break;
-        Hom<T> a, a1, a2;
-        hzero(a);
-        hzero(a1);
-        hzero(a2);
+    for (int r = 0; r < K1; r++) {
+        if (!EXACT && r >= k1) break;
+        Hom<T> m[K2];
+        row(r, m);
+        Hom<T> b, b1, b2;
+        hzero(b);
+        hzero(b1);
+        hzero(b2);
 #pragma unroll
-        for (int r = 0; r < K1; r++) {
-            if (!EXACT && r >= k1) break;
-            const Hom<T> m = entry(r, c);
-            hfma(a, U[r], m);
-            if (ORDER >= 1 && r >= 1) hfma(a1, dU[r], m);
-            if (ORDER >= 2 && r >= 2) hfma(a2, ddU[r], m);
+        for (int c = 0; c < K2; c++) {
+            if (!EXACT && c >= k2) break;
+            hfma(b, V[c], m[c]);
+            if (ORDER >= 1 && c >= 1) hfma(b1, dV[c], m[c]);
+            if (ORDER >= 2 && c >= 2) hfma(b2, ddV[c], m[c]);
         }
-        hfma(o.R, V[c], a);
+        hfma(o.R, U[r], b);
         if (ORDER >= 1) {
-            hfma(o.Ru, V[c], a1);
-            if (c >= 1) hfma(o.Rv, dV[c], a);
+            hfma(o.Rv, U[r], b1);
+            if (r >= 1) hfma(o.Ru, dU[r], b);
         }
         if (ORDER >= 2) {
-            hfma(o.Ruu, V[c], a2);
-            if (c >= 1) hfma(o.Ruv, dV[c], a1);
-            if (c >= 2) hfma(o.Rvv, ddV[c], a);
+            hfma(o.Rvv, U[r], b2);
+            if (r >= 1) hfma(o.Ruv, dU[r], b1);
+            if (r >= 2) hfma(o.Ruu, ddU[r], b);
         }
     }
 }
@@ -127,9 +130,57 @@ template <int K1, int K2, int ORDER, bool EXACT, typename T>
 __device__ __forceinline__ void contract(const T *__restrict__ patch, int k1, int k2, T t, T s,
                                          HomDer<T> &o) {
     const int kk2 = EXACT ? K2 : k2;
-    contract_with<K1, K2, ORDER, EXACT, T>(
-        [&](int r, int c) { return load_hom<T>(patch + (r * kk2 + c) * 4); }, k1, k2, t, s, o);
+    contract_rows<K1, K2, ORDER, EXACT, T>(
+        [&](int r, Hom<T>(&m)[K2]) {
+#pragma unroll
+            for (int c = 0; c < K2; c++)
+                if (EXACT || c < kk2) m[c] = load_hom<T>(patch + (r * kk2 + c) * 4);
+        },
+        k1, k2, t, s, o);
 }
+
+// Wave-cooperative staging of patch rows through LDS: row r of the 64 lanes' patches is copied
+// with contiguous 16-byte loads (one load instruction covers whole rows of several lanes, i.e.
+// a few cache lines, instead of one line per lane), then each lane reads its own row from LDS.
+// Every lane of the wave must take part (wave-uniform control flow); an idle lane passes any
+// valid patch.  lds: the wave's buffer of 64 * STRIDE 16-byte vectors.
+template <int K2, typename T>
+struct WaveRows {
+    static constexpr int VEC = K2 * 4 * (int)sizeof(T) / 16;  // 16-byte vectors per row
+    static constexpr int STRIDE = VEC + 1;                     // padded row stride
+    static_assert(K2 * 4 * sizeof(T) % 16 == 0, "row size");
+    const char *src[VEC];  // source of vector j*64 + lane of row 0
+    float4 *lds;
+    int lane;
+    __device__ __forceinline__ WaveRows(const T *patch, float4 *buf) : lds(buf) {
+        lane = threadIdx.x & 63;
+        const unsigned long long b = (unsigned long long)patch;
+        const unsigned lo = (unsigned)(b & 0xffffffffull), hi = (unsigned)(b >> 32);
+#pragma unroll
+        for (int j = 0; j < VEC; j++) {
+            const int vec = j * 64 + lane, pp = vec / VEC, o = vec - pp * VEC;
+            const unsigned long long pb = (unsigned long long)(unsigned)__shfl((int)lo, pp, 64) |
+                                          ((unsigned long long)(unsigned)__shfl((int)hi, pp, 64) << 32);
+            src[j] = reinterpret_cast<const char *>(pb) + o * 16;
+        }
+    }
+    __device__ __forceinline__ void operator()(int r, Hom<T> (&m)[K2]) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");  // earlier reads of the buffer
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+        for (int j = 0; j < VEC; j++) {
+            const int vec = j * 64 + lane, pp = vec / VEC, o = vec - pp * VEC;
+            lds[pp * STRIDE + o] = *reinterpret_cast<const float4 *>(src[j] + (size_t)r * VEC * 16);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const T *mine = reinterpret_cast<const T *>(lds + lane * STRIDE);
+#pragma unroll
+        for (int c = 0; c < K2; c++) m[c] = load_hom<T>(mine + c * 4);
+    }
+};
 
 // A lane's current span patch held in registers (exact sizes only): reloaded from the table only
 // when the lane's span pair changes, so iterations that stay in their span issue no gathers.
@@ -217,11 +268,16 @@ struct SpanCache {
     int im = 0, in = 0;  // span slots m - p, n - q
 };
 
-template <int K1, int K2, int ORDER, bool EXACT, typename T, bool PREG = false>
+// Patch sources of eval_point_cached: per-lane gathers, the register-cached patch, or LDS rows
+// staged by the whole wave (wave-uniform call required).
+enum { kPatchGlobal = 0, kPatchRegs = 1, kPatchWave = 2 };
+
+template <int K1, int K2, int ORDER, bool EXACT, typename T, int PSRC = kPatchGlobal>
 __device__ __forceinline__ void eval_point_cached(const GnSurfDev &d, const double *__restrict__ ku,
                                                   const double *__restrict__ kv, double u, double v,
                                                   SpanCache &sc, Cart<T> &o,
-                                                  Hom<T> *pr = nullptr) {
+                                                  Hom<T> *pr = nullptr, float4 *wave_lds = nullptr) {
+    constexpr bool PREG = PSRC == kPatchRegs;
     bool moved = false;
     if (!(u >= sc.ulo && u < sc.uhi)) {
         const int m = find_span(ku, d.p, d.nu, u);
@@ -253,8 +309,17 @@ __device__ __forceinline__ void eval_point_cached(const GnSurfDev &d, const doub
 #pragma unroll
             for (int e = 0; e < K1 * K2; e++) pr[e] = load_hom<T>(patch + e * 4);
         }
-        contract_with<K1, K2, ORDER, true, T>([&](int r, int c) { return pr[r * K2 + c]; }, k1,
-                                              k2, t, s, h);
+        contract_rows<K1, K2, ORDER, true, T>(
+            [&](int r, Hom<T>(&m)[K2]) {
+#pragma unroll
+                for (int c = 0; c < K2; c++) m[c] = pr[r * K2 + c];
+            },
+            k1, k2, t, s, h);
+    } else if constexpr (PSRC == kPatchWave) {
+        static_assert(EXACT, "LDS row staging needs exact sizes");
+        (void)moved;
+        WaveRows<K2, T> rows(patch, wave_lds);
+        contract_rows<K1, K2, ORDER, true, T>(rows, k1, k2, t, s, h);
     } else {
         (void)moved;
         contract<K1, K2, ORDER, EXACT, T>(patch, k1, k2, t, s, h);

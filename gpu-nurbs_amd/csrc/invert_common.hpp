@@ -86,14 +86,16 @@ __device__ __forceinline__ Domain domain_of(const GnSurfDev &d) {
     return Domain{d.ku[d.p], d.ku[d.nu], d.kv[d.q], d.kv[d.nv]};
 }
 
-template <int K1, int K2, bool EXACT, typename T, bool NEWTON, bool PREG = false>
+template <int K1, int K2, bool EXACT, typename T, bool NEWTON, int PSRC = kPatchGlobal>
 __device__ __forceinline__ int gn_step(const GnSurfDev &d, const double *__restrict__ ku,
                                        const double *__restrict__ kv, T Px, T Py, T Pz,
                                        double &cu, double &cv, double tol_dist, double tol_step,
                                        double tol_quad, int clamp, const Domain &dom,
-                                       SpanCache &sc, Hom<T> *pr = nullptr) {
+                                       SpanCache &sc, Hom<T> *pr = nullptr,
+                                       float4 *wave_lds = nullptr) {
     Cart<T> o;
-    eval_point_cached<K1, K2, NEWTON ? 2 : 1, EXACT, T, PREG>(d, ku, kv, cu, cv, sc, o, pr);
+    eval_point_cached<K1, K2, NEWTON ? 2 : 1, EXACT, T, PSRC>(d, ku, kv, cu, cv, sc, o, pr,
+                                                              wave_lds);
     const T r0 = o.S[0] - Px, r1 = o.S[1] - Py, r2 = o.S[2] - Pz;
     const T rr = r0 * r0 + r1 * r1 + r2 * r2;
     if (!(rr == rr)) return GN_PT_NONFINITE;
@@ -198,9 +200,15 @@ __global__ __launch_bounds__(kInvBlock, (EXACT && sizeof(T) == 4 && K1 * K2 <= 1
     T Px = 0, Py = 0, Pz = 0;
     int it = 0, lim = max_iter;
     SpanCache sc;
-    // FP32 exact-degree kernels keep the lane's span patch in registers between iterations
-    constexpr bool PREG = EXACT && sizeof(T) == 4 && K1 * K2 <= 16;
+    // FP32 exact-degree kernels keep the lane's span patch in registers between iterations; FP64
+    // exact-degree kernels (one or few iterations per point) stage patch rows through LDS
+    constexpr int PSRC = !EXACT ? kPatchGlobal
+                       : sizeof(T) == 4 ? (K1 * K2 <= 16 ? kPatchRegs : kPatchGlobal)
+                                        : kPatchWave;
     PatchRegs<T, K1 * K2> pr;
+    constexpr int kWaveVec = PSRC == kPatchWave ? WaveRows<K2, T>::STRIDE * 64 : 1;
+    __shared__ float4 wrows[kInvBlock / 64][kWaveVec];
+    float4 *wave_lds = wrows[threadIdx.x >> 6];
     while (true) {
         while (!exhausted) {
             const unsigned long long idle = __ballot(idx < 0);
@@ -236,14 +244,20 @@ __global__ __launch_bounds__(kInvBlock, (EXACT && sizeof(T) == 4 && K1 * K2 <= 1
             q_lo += nidle < avail ? nidle : avail;
         }
         if (__ballot(idx >= 0) == 0ull) break;
+        // a step for every lane holding a point; with LDS-staged rows the whole wave calls it
+        // (idle lanes compute on stale state and discard the result)
+        int st = kStepContinue;
+        const bool act = idx >= 0 && it < lim;
+        if (PSRC == kPatchWave || act) {
+            const int s2 = gn_step<K1, K2, EXACT, T, NEWTON, PSRC>(
+                d, ku, kv, Px, Py, Pz, cu, cv, tol_dist, tol_step, tol_quad, a.o.clamp, dom, sc,
+                pr.m, wave_lds);
+            if (act) st = s2;
+        }
         if (idx >= 0) {
-            int st;
-            if (it >= lim) {
+            if (!act) {
                 st = GN_PT_MAXITER;
             } else {
-                st = gn_step<K1, K2, EXACT, T, NEWTON, PREG>(d, ku, kv, Px, Py, Pz, cu, cv,
-                                                             tol_dist, tol_step, tol_quad,
-                                                             a.o.clamp, dom, sc, pr.m);
                 if (st == kStepContinue || st == GN_PT_CONVERGED) it++;
                 if (st == kStepContinue && it >= lim) st = GN_PT_MAXITER;
             }
