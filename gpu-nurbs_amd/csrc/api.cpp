@@ -252,7 +252,7 @@ int gn_get_span_matrix(const gn_surface *s, int i, int j, double *out) {
 int gn_evaluate(const gn_surface *s, const double *u, const double *v, size_t n, int max_order,
                 double *out, gn_stream stream) {
     if (!s) return fail(GN_EINVAL, "null surface");
-    if (max_order < 0 || max_order > 2) return fail(GN_EINVAL, "max_order must be 0..2");
+    if (max_order < 0 || max_order > 3) return fail(GN_EINVAL, "max_order must be 0..3");
     if (n && (!u || !v || !out)) return fail(GN_EINVAL, "null buffer");
     return gn::launch_eval64(s->dev(), u, v, n, max_order, out, (hipStream_t)stream);
 }
@@ -260,7 +260,7 @@ int gn_evaluate(const gn_surface *s, const double *u, const double *v, size_t n,
 int gn_evaluate_f32(const gn_surface *s, const float *u, const float *v, size_t n, int max_order,
                     float *out, gn_stream stream) {
     if (!s) return fail(GN_EINVAL, "null surface");
-    if (max_order < 0 || max_order > 2) return fail(GN_EINVAL, "max_order must be 0..2");
+    if (max_order < 0 || max_order > 3) return fail(GN_EINVAL, "max_order must be 0..3");
     if (n && (!u || !v || !out)) return fail(GN_EINVAL, "null buffer");
     return gn::launch_eval32(s->dev(), u, v, n, max_order, out, (hipStream_t)stream);
 }

@@ -1,18 +1,51 @@
-// eval.hip -- batch evaluation of a surface and its derivatives (EvaluateRandom,
-// src/NURBS/Evaluate.cu:1023-1038, extended to the derivatives of 288-502 and 535-930).
+// eval.hip -- batch evaluation of a surface and its derivatives up to third order (EvaluateRandom,
+// src/NURBS/Evaluate.cu:1023-1038, extended to the derivatives of 288-502 and 535-1020).
 //
 // One lane per query point, grid-stride over the batch.  Parameters stream in and results stream
 // out as SoA arrays, so every global load and store of the batch is a fully coalesced 256-byte
 // (FP32) or 512-byte (FP64) wave access.  Span patches come from the L2-resident span table:
 // staged through LDS by cooperative contiguous loads for patches up to 512 bytes (k_eval_lds,
 // whose knots also live in LDS for the span search), gathered per lane otherwise (k_eval, knots
-// read through L1).  The target roof is HBM traffic (8 or 16 bytes in, 12..144 bytes out per
+// read through L1).  The target roof is HBM traffic (8 or 16 bytes in, 12..240 bytes out per
 // point).
 #include "gn_device.hpp"
 
 namespace gn {
 
 constexpr int kEvalBlock = 256;
+
+// Streams one point's fields into the SoA output: S, then Su Sv, Suu Suv Svv, Suuu Suuv Suvv Svvv
+// up to ORDER, each field three blocks (x, y, z) of n values.
+template <int ORDER, typename T>
+__device__ __forceinline__ void store_fields(const Cart<T> &o, T *__restrict__ out, size_t n,
+                                             size_t i) {
+#pragma unroll
+    for (int c = 0; c < 3; c++) __builtin_nontemporal_store(o.S[c], out + c * n + i);
+    if (ORDER >= 1) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            __builtin_nontemporal_store(o.Su[c], out + (3 + c) * n + i);
+            __builtin_nontemporal_store(o.Sv[c], out + (6 + c) * n + i);
+        }
+    }
+    if (ORDER >= 2) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            __builtin_nontemporal_store(o.Suu[c], out + (9 + c) * n + i);
+            __builtin_nontemporal_store(o.Suv[c], out + (12 + c) * n + i);
+            __builtin_nontemporal_store(o.Svv[c], out + (15 + c) * n + i);
+        }
+    }
+    if (ORDER >= 3) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            __builtin_nontemporal_store(o.Suuu[c], out + (18 + c) * n + i);
+            __builtin_nontemporal_store(o.Suuv[c], out + (21 + c) * n + i);
+            __builtin_nontemporal_store(o.Suvv[c], out + (24 + c) * n + i);
+            __builtin_nontemporal_store(o.Svvv[c], out + (27 + c) * n + i);
+        }
+    }
+}
 
 template <int K1, int K2, bool EXACT, int ORDER, typename T, typename TP>
 __global__ __launch_bounds__(kEvalBlock) void k_eval(GnSurfDev d, const TP *__restrict__ u,
@@ -23,23 +56,7 @@ __global__ __launch_bounds__(kEvalBlock) void k_eval(GnSurfDev d, const TP *__re
          i += (size_t)gridDim.x * blockDim.x) {
         Cart<T> o;
         eval_point<K1, K2, ORDER, EXACT, T>(d, ku, kv, (double)u[i], (double)v[i], o);
-#pragma unroll
-        for (int c = 0; c < 3; c++) __builtin_nontemporal_store(o.S[c], out + c * n + i);
-        if (ORDER >= 1) {
-#pragma unroll
-            for (int c = 0; c < 3; c++) {
-                __builtin_nontemporal_store(o.Su[c], out + (3 + c) * n + i);
-                __builtin_nontemporal_store(o.Sv[c], out + (6 + c) * n + i);
-            }
-        }
-        if (ORDER >= 2) {
-#pragma unroll
-            for (int c = 0; c < 3; c++) {
-                __builtin_nontemporal_store(o.Suu[c], out + (9 + c) * n + i);
-                __builtin_nontemporal_store(o.Suv[c], out + (12 + c) * n + i);
-                __builtin_nontemporal_store(o.Svv[c], out + (15 + c) * n + i);
-            }
-        }
+        store_fields<ORDER>(o, out, n, i);
     }
 }
 
@@ -105,25 +122,7 @@ __global__ __launch_bounds__(kLdsBlock) void k_eval_lds(GnSurfDev d, const TP *_
         quotient<ORDER, T>(h, T(ihu), T(ihv), o);
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
-        if (live) {
-#pragma unroll
-            for (int c = 0; c < 3; c++) __builtin_nontemporal_store(o.S[c], out + c * n + i);
-            if (ORDER >= 1) {
-#pragma unroll
-                for (int c = 0; c < 3; c++) {
-                    __builtin_nontemporal_store(o.Su[c], out + (3 + c) * n + i);
-                    __builtin_nontemporal_store(o.Sv[c], out + (6 + c) * n + i);
-                }
-            }
-            if (ORDER >= 2) {
-#pragma unroll
-                for (int c = 0; c < 3; c++) {
-                    __builtin_nontemporal_store(o.Suu[c], out + (9 + c) * n + i);
-                    __builtin_nontemporal_store(o.Suv[c], out + (12 + c) * n + i);
-                    __builtin_nontemporal_store(o.Svv[c], out + (15 + c) * n + i);
-                }
-            }
-        }
+        if (live) store_fields<ORDER>(o, out, n, i);
     }
 }
 
@@ -144,7 +143,8 @@ struct EvalLaunch {
             switch (order) {
                 case 0: k_eval_lds<K1, K2, 0, T, TP><<<g2, kLdsBlock, 0, st>>>(d, u, v, n, out); break;
                 case 1: k_eval_lds<K1, K2, 1, T, TP><<<g2, kLdsBlock, 0, st>>>(d, u, v, n, out); break;
-                default: k_eval_lds<K1, K2, 2, T, TP><<<g2, kLdsBlock, 0, st>>>(d, u, v, n, out); break;
+                case 2: k_eval_lds<K1, K2, 2, T, TP><<<g2, kLdsBlock, 0, st>>>(d, u, v, n, out); break;
+                default: k_eval_lds<K1, K2, 3, T, TP><<<g2, kLdsBlock, 0, st>>>(d, u, v, n, out); break;
             }
             GN_HIP(hipGetLastError());
             return GN_OK;
@@ -152,7 +152,8 @@ struct EvalLaunch {
         switch (order) {
             case 0: k_eval<K1, K2, EXACT, 0, T, TP><<<grid, kEvalBlock, 0, st>>>(d, u, v, n, out); break;
             case 1: k_eval<K1, K2, EXACT, 1, T, TP><<<grid, kEvalBlock, 0, st>>>(d, u, v, n, out); break;
-            default: k_eval<K1, K2, EXACT, 2, T, TP><<<grid, kEvalBlock, 0, st>>>(d, u, v, n, out); break;
+            case 2: k_eval<K1, K2, EXACT, 2, T, TP><<<grid, kEvalBlock, 0, st>>>(d, u, v, n, out); break;
+            default: k_eval<K1, K2, EXACT, 3, T, TP><<<grid, kEvalBlock, 0, st>>>(d, u, v, n, out); break;
         }
         GN_HIP(hipGetLastError());
         return GN_OK;

@@ -28,7 +28,7 @@ struct Hom {  // homogeneous 4-vector
 
 template <typename T>
 struct HomDer {  // homogeneous point and partial derivatives in the local span parameters
-    Hom<T> R, Ru, Rv, Ruu, Ruv, Rvv;
+    Hom<T> R, Ru, Rv, Ruu, Ruv, Rvv, Ruuu, Ruuv, Ruvv, Rvvv;
 };
 
 template <typename T>
@@ -74,7 +74,7 @@ __device__ __forceinline__ Hom<double> load_hom<double>(const double *__restrict
 template <int K1, int K2, int ORDER, bool EXACT, typename T, typename ROW>
 __device__ __forceinline__ void contract_rows(ROW &&row, int k1, int k2, T t, T s, HomDer<T> &o) {
     if (EXACT) { k1 = K1; k2 = K2; }
-    T U[K1], dU[K1], ddU[K1], V[K2], dV[K2], ddV[K2];
+    T U[K1], dU[K1], ddU[K1], dddU[K1], V[K2], dV[K2], ddV[K2], dddV[K2];
     {
         T now = T(1);
 #pragma unroll
@@ -85,6 +85,8 @@ __device__ __forceinline__ void contract_rows(ROW &&row, int k1, int k2, T t, T 
         for (int r = 1; r < K1; r++) dU[r] = T(r) * U[r - 1];
 #pragma unroll
         for (int r = 0; r < K1; r++) ddU[r] = r >= 2 ? T(r * (r - 1)) * U[r - 2] : T(0);
+#pragma unroll
+        for (int r = 0; r < K1; r++) dddU[r] = r >= 3 ? T(r * (r - 1) * (r - 2)) * U[r - 3] : T(0);
         now = T(1);
 #pragma unroll
         for (int c = 0; c < K2; c++) { V[c] = now; now *= s; }
@@ -93,25 +95,30 @@ __device__ __forceinline__ void contract_rows(ROW &&row, int k1, int k2, T t, T 
         for (int c = 1; c < K2; c++) dV[c] = T(c) * V[c - 1];
 #pragma unroll
         for (int c = 0; c < K2; c++) ddV[c] = c >= 2 ? T(c * (c - 1)) * V[c - 2] : T(0);
+#pragma unroll
+        for (int c = 0; c < K2; c++) dddV[c] = c >= 3 ? T(c * (c - 1) * (c - 2)) * V[c - 3] : T(0);
     }
     hzero(o.R);
     if (ORDER >= 1) { hzero(o.Ru); hzero(o.Rv); }
     if (ORDER >= 2) { hzero(o.Ruu); hzero(o.Ruv); hzero(o.Rvv); }
+    if (ORDER >= 3) { hzero(o.Ruuu); hzero(o.Ruuv); hzero(o.Ruvv); hzero(o.Rvvv); }
 #pragma unroll
     for (int r = 0; r < K1; r++) {
         if (!EXACT && r >= k1) break;
         Hom<T> m[K2];
         row(r, m);
-        Hom<T> b, b1, b2;
+        Hom<T> b, b1, b2, b3;
         hzero(b);
         hzero(b1);
         hzero(b2);
+        hzero(b3);
 #pragma unroll
         for (int c = 0; c < K2; c++) {
             if (!EXACT && c >= k2) break;
             hfma(b, V[c], m[c]);
             if (ORDER >= 1 && c >= 1) hfma(b1, dV[c], m[c]);
             if (ORDER >= 2 && c >= 2) hfma(b2, ddV[c], m[c]);
+            if (ORDER >= 3 && c >= 3) hfma(b3, dddV[c], m[c]);
         }
         hfma(o.R, U[r], b);
         if (ORDER >= 1) {
@@ -122,6 +129,12 @@ __device__ __forceinline__ void contract_rows(ROW &&row, int k1, int k2, T t, T 
             hfma(o.Rvv, U[r], b2);
             if (r >= 1) hfma(o.Ruv, dU[r], b1);
             if (r >= 2) hfma(o.Ruu, ddU[r], b);
+        }
+        if (ORDER >= 3) {  // MatrixSurfaceRd{UUU,UUV,UVV,VVV}, src/NURBS/Evaluate.cu:408-502
+            hfma(o.Rvvv, U[r], b3);
+            if (r >= 1) hfma(o.Ruvv, dU[r], b2);
+            if (r >= 2) hfma(o.Ruuv, ddU[r], b1);
+            if (r >= 3) hfma(o.Ruuu, dddU[r], b);
         }
     }
 }
@@ -192,7 +205,7 @@ struct PatchRegs {
 
 template <typename T>
 struct Cart {  // Cartesian point and derivatives in the global parameters
-    T S[3], Su[3], Sv[3], Suu[3], Suv[3], Svv[3];
+    T S[3], Su[3], Sv[3], Suu[3], Suv[3], Svv[3], Suuu[3], Suuv[3], Suvv[3], Svvv[3];
 };
 
 // Reciprocal: correctly rounded in FP64; the hardware reciprocal (1 ulp) in FP32.
@@ -202,7 +215,9 @@ __device__ __forceinline__ float recip(float x) { return __builtin_amdgcn_rcpf(x
 // Rational quotient (the old-version rational derivatives of src/NURBS/Evaluate.cu:535-930,
 // written in the equivalent Leibniz form S_u = (A_u - w_u S)/w, ...) and the chain factors
 // 1/(k[m+1]-k[m]) of the span (the reference multiplies by (k[last]-k[0])/(k[m+1]-k[m]),
-// which is the same for knots on [0, 1]).
+// which is the same for knots on [0, 1]).  Third order follows the Leibniz form of
+// src/NURBS/Evaluate.cu:960-1020 with every lower-order term subtracted: the reference adds the
+// w_uv, w_uuv and w_uvv terms of S_uuv and S_uvv, which is exact only for constant weights.
 template <int ORDER, typename T>
 __device__ __forceinline__ void quotient(const HomDer<T> &h, T ihu, T ihv, Cart<T> &o) {
     T iw = recip(h.R.w);
@@ -216,6 +231,7 @@ __device__ __forceinline__ void quotient(const HomDer<T> &h, T ihu, T ihv, Cart<
         o.Su[c] = (Ru[c] - h.Ru.w * o.S[c]) * iw;
         o.Sv[c] = (Rv[c] - h.Rv.w * o.S[c]) * iw;
     }
+    T luu[3], luv[3], lvv[3];  // order-2 derivatives in the local parameters
     if (ORDER >= 2) {
         const T Ruu[3] = {h.Ruu.x, h.Ruu.y, h.Ruu.z}, Ruv[3] = {h.Ruv.x, h.Ruv.y, h.Ruv.z},
                 Rvv[3] = {h.Rvv.x, h.Rvv.y, h.Rvv.z};
@@ -224,9 +240,33 @@ __device__ __forceinline__ void quotient(const HomDer<T> &h, T ihu, T ihv, Cart<
             T uu = (Ruu[c] - T(2) * h.Ru.w * o.Su[c] - h.Ruu.w * o.S[c]) * iw;
             T uv = (Ruv[c] - h.Ru.w * o.Sv[c] - h.Rv.w * o.Su[c] - h.Ruv.w * o.S[c]) * iw;
             T vv = (Rvv[c] - T(2) * h.Rv.w * o.Sv[c] - h.Rvv.w * o.S[c]) * iw;
+            luu[c] = uu;
+            luv[c] = uv;
+            lvv[c] = vv;
             o.Suu[c] = uu * (ihu * ihu);
             o.Suv[c] = uv * (ihu * ihv);
             o.Svv[c] = vv * (ihv * ihv);
+        }
+    }
+    if (ORDER >= 3) {
+        const T A3[4][3] = {{h.Ruuu.x, h.Ruuu.y, h.Ruuu.z}, {h.Ruuv.x, h.Ruuv.y, h.Ruuv.z},
+                            {h.Ruvv.x, h.Ruvv.y, h.Ruvv.z}, {h.Rvvv.x, h.Rvvv.y, h.Rvvv.z}};
+        const T wu = h.Ru.w, wv = h.Rv.w, wuu = h.Ruu.w, wuv = h.Ruv.w, wvv = h.Rvv.w;
+        const T cuuu = ihu * ihu * ihu, cuuv = ihu * ihu * ihv, cuvv = ihu * ihv * ihv,
+                cvvv = ihv * ihv * ihv;
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const T su = o.Su[c], sv = o.Sv[c], s0 = o.S[c];
+            T uuu = (A3[0][c] - T(3) * wu * luu[c] - T(3) * wuu * su - h.Ruuu.w * s0) * iw;
+            T uuv = (A3[1][c] - wv * luu[c] - T(2) * wu * luv[c] - T(2) * wuv * su - wuu * sv -
+                     h.Ruuv.w * s0) * iw;
+            T uvv = (A3[2][c] - wu * lvv[c] - T(2) * wv * luv[c] - T(2) * wuv * sv - wvv * su -
+                     h.Ruvv.w * s0) * iw;
+            T vvv = (A3[3][c] - T(3) * wv * lvv[c] - T(3) * wvv * sv - h.Rvvv.w * s0) * iw;
+            o.Suuu[c] = uuu * cuuu;
+            o.Suuv[c] = uuv * cuuv;
+            o.Suvv[c] = uvv * cuvv;
+            o.Svvv[c] = vvv * cvvv;
         }
     }
 #pragma unroll

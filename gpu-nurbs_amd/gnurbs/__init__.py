@@ -256,9 +256,9 @@ class Surface:
 
     # ---- evaluation ----------------------------------------------------------------------
     def Evaluate(self, u, v, order: int = 0, precision: int = GN_FP64, out=None, stream=None):
-        """Point and derivatives; returns a device tensor of shape (3*(1, 3, 6)[order], n)."""
+        """Point and derivatives; returns a device tensor of shape (3*(1, 3, 6, 10)[order], n)."""
         torch = _torch()
-        nblk = (3, 9, 18)[order] if 0 <= order <= 2 else 3  # bad orders are rejected by the C ABI
+        nblk = (3, 9, 18, 30)[order] if 0 <= order <= 3 else 3  # bad orders are rejected by the C ABI
         if precision == GN_FP64:
             ud, vd = to_device(u, torch.float64), to_device(v, torch.float64)
             n = ud.numel()

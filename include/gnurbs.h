@@ -123,17 +123,19 @@ int gn_build_span_table(gn_surface *s, gn_stream stream);
  * host buffer of 4*(p+1)*(q+1) doubles; (i, j) are knot-span indices.  Synchronous. */
 int gn_get_span_matrix(const gn_surface *s, int i, int j, double *out);
 
-/* EvaluateRandom / MatrixSurfacePoint / MatrixSurfaceDerivative{U,V,UU,UV,VV}
- * (src/NURBS/Evaluate.cu:159-227, 535-930, 1023-1038).  u, v, out: device pointers.
- * out is SoA: blocks of n values S.x S.y S.z | Su.xyz | Sv.xyz | Suu.xyz | Suv.xyz | Svv.xyz;
- * max_order 0, 1, 2 writes 3, 9, 18 blocks. */
+/* EvaluateRandom / MatrixSurfacePoint / MatrixSurfaceRd* / MatrixSurfaceDerivative{U,V,UU,UV,VV,
+ * UUU,UUV,UVV,VVV} (src/NURBS/Evaluate.cu:159-502, 535-1020, 1023-1038).  u, v, out: device
+ * pointers.  out is SoA: blocks of n values S.x S.y S.z | Su.xyz | Sv.xyz | Suu.xyz | Suv.xyz |
+ * Svv.xyz | Suuu.xyz | Suuv.xyz | Suvv.xyz | Svvv.xyz; max_order 0, 1, 2, 3 writes 3, 9, 18, 30
+ * blocks.  Mixed derivatives use the exact quotient rule (the reference's new-version S_uv,
+ * S_uuv and S_uvv, src/NURBS/Evaluate.cu:958-990, differ from it unless the weights are constant). */
 int gn_evaluate(const gn_surface *s, const double *u, const double *v, size_t n, int max_order,
                 double *out, gn_stream stream);
 /* FP32 form: parameters are floats (span lookup still compares against FP64 knots). */
 int gn_evaluate_f32(const gn_surface *s, const float *u, const float *v, size_t n,
                     int max_order, float *out, gn_stream stream);
 /* Tensor-product grid (us[nu_pts] x vs[nv_pts]), FP32 MFMA span-GEMM form; out SoA blocks of
- * nu_pts*nv_pts values, point (a, b) at a*nv_pts + b.  us, vs ascending. */
+ * nu_pts*nv_pts values, point (a, b) at a*nv_pts + b.  us, vs ascending; max_order 0..2. */
 int gn_evaluate_grid_f32(const gn_surface *s, const float *us, int nu_pts, const float *vs,
                          int nv_pts, int max_order, float *out, gn_stream stream);
 

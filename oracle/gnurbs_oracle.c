@@ -227,7 +227,7 @@ static double dot(const double *a, const double *b, int n)
     return s;
 }
 
-static void monomials(double t, int k, double *V, double *dV, double *ddV)
+static void monomials(double t, int k, double *V, double *dV, double *ddV, double *dddV)
 {
     double now = 1;
     for (int i = 0; i < k; i++) { V[i] = now; now *= t; }
@@ -238,6 +238,10 @@ static void monomials(double t, int k, double *V, double *dV, double *ddV)
     ddV[0] = 0;
     if (k > 1) ddV[1] = 0;
     for (int i = 0; i < k - 2; i++) { ddV[i + 2] = (i + 2) * (i + 1) * now; now *= t; }
+    /* vectordUUU / vectordVVV of MatrixSurfaceRdUUU / RdVVV (src/NURBS/Evaluate.cu:408-502) */
+    for (int i = 0; i < k && i < 3; i++) dddV[i] = 0;
+    now = 1;
+    for (int i = 0; i < k - 3; i++) { dddV[i + 3] = (i + 3) * (i + 2) * (i + 1) * now; now *= t; }
 }
 
 typedef struct { double x, y, z, w; } d4;
@@ -269,11 +273,12 @@ static d4 contract_ufirst(const double *mid, int k1, int k2, const double *U, co
 
 typedef struct {
     double S[3], Su[3], Sv[3], Suu[3], Suv[3], Svv[3];
+    double Suuu[3], Suuv[3], Suvv[3], Svvv[3];
     double R[4];
     int m, n;
 } or_pt;
 
-/* Point and Cartesian derivatives up to `order` (0..2) at global (u, v). */
+/* Point and Cartesian derivatives up to `order` (0..3) at global (u, v). */
 static void eval_one(const or_surf *sf, double uu, double vv, int order, or_pt *o)
 {
     int p = sf->p, q = sf->q, k1 = p + 1, k2 = q + 1;
@@ -284,9 +289,10 @@ static void eval_one(const or_surf *sf, double uu, double vv, int order, or_pt *
     double v = (vv - sf->kv[n]) / hv;
     double spanU = 1.0 / hu, spanV = 1.0 / hv;
     const double *mid = sf->MatrixN + or_matrix_n_position(m, n, p, q, sf->nu, sf->nv, 0);
-    double U[OR_MAXK], dU[OR_MAXK], ddU[OR_MAXK], V[OR_MAXK], dV[OR_MAXK], ddV[OR_MAXK];
-    monomials(u, k1, U, dU, ddU);
-    monomials(v, k2, V, dV, ddV);
+    double U[OR_MAXK], dU[OR_MAXK], ddU[OR_MAXK], dddU[OR_MAXK];
+    double V[OR_MAXK], dV[OR_MAXK], ddV[OR_MAXK], dddV[OR_MAXK];
+    monomials(u, k1, U, dU, ddU, dddU);
+    monomials(v, k2, V, dV, ddV, dddV);
     d4 R = contract(mid, k1, k2, U, V);
     o->m = m;
     o->n = n;
@@ -321,11 +327,41 @@ static void eval_one(const or_surf *sf, double uu, double vv, int order, or_pt *
         o->Svv[c] = hvv * spanV * spanV;
         o->Suv[c] = huv * spanU * spanV;
     }
+    if (order < 3) return;
+    /* Third order: homogeneous MatrixSurfaceRd{UUU,UUV,UVV,VVV} (src/NURBS/Evaluate.cu:408-502),
+     * then the Leibniz quotient of MatrixSurfaceDerivative{UUU,UUV,UVV,VVV}
+     * (src/NURBS/Evaluate.cu:960-1020) with the w_uv, w_uuv and w_uvv terms of UUV and UVV
+     * subtracted (the reference adds them, which is exact only for constant weights).  All in the
+     * span's local parameters, scaled by the chain factors at the end. */
+    d4 RdUUU = contract(mid, k1, k2, dddU, V);
+    d4 RdUUV = contract(mid, k1, k2, ddU, dV);
+    d4 RdUVV = contract(mid, k1, k2, dU, ddV);
+    d4 RdVVV = contract_ufirst(mid, k1, k2, U, dddV);
+    const double A3[4][3] = {{RdUUU.x, RdUUU.y, RdUUU.z}, {RdUUV.x, RdUUV.y, RdUUV.z},
+                             {RdUVV.x, RdUVV.y, RdUVV.z}, {RdVVV.x, RdVVV.y, RdVVV.z}};
+    for (int c = 0; c < 3; c++) {
+        /* local-parameter derivatives of orders 1 and 2 */
+        double su = o->Su[c] / spanU, sv = o->Sv[c] / spanV;
+        double suu = o->Suu[c] / (spanU * spanU), svv = o->Svv[c] / (spanV * spanV);
+        double suv = o->Suv[c] / (spanU * spanV);
+        double s0 = o->S[c];
+        double uuu = (A3[0][c] - 3 * RdU.w * suu - 3 * RdUU.w * su - RdUUU.w * s0) / h;
+        double uuv = (A3[1][c] - RdV.w * suu - 2 * RdU.w * suv - 2 * RdUV.w * su - RdUU.w * sv -
+                      RdUUV.w * s0) / h;
+        double uvv = (A3[2][c] - RdU.w * svv - 2 * RdV.w * suv - 2 * RdUV.w * sv - RdVV.w * su -
+                      RdUVV.w * s0) / h;
+        double vvv = (A3[3][c] - 3 * RdV.w * svv - 3 * RdVV.w * sv - RdVVV.w * s0) / h;
+        o->Suuu[c] = uuu * spanU * spanU * spanU;
+        o->Suuv[c] = uuv * spanU * spanU * spanV;
+        o->Suvv[c] = uvv * spanU * spanV * spanV;
+        o->Svvv[c] = vvv * spanV * spanV * spanV;
+    }
 }
 
-/* Batch evaluation (EvaluateRandom, src/NURBS/Evaluate.cu:1023-1038, extended to order <= 2).
- * out is SoA: [S.x S.y S.z | Su.xyz | Sv.xyz | Suu.xyz | Suv.xyz | Svv.xyz], each block n long;
- * only the first 3*(1 + 2*(order>=1) + 3*(order>=2)) blocks are written. */
+/* Batch evaluation (EvaluateRandom, src/NURBS/Evaluate.cu:1023-1038, extended to order <= 3).
+ * out is SoA: [S.x S.y S.z | Su.xyz | Sv.xyz | Suu.xyz | Suv.xyz | Svv.xyz | Suuu.xyz | Suuv.xyz |
+ * Suvv.xyz | Svvv.xyz], each block n long; only the first 3, 9, 18 or 30 blocks (order 0..3) are
+ * written. */
 void or_eval(int p, int q, int nu, int nv, const double *ku, const double *kv,
              const double *MatrixN, const double *u, const double *v, long n, int order,
              double *out, int nthreads)
@@ -350,6 +386,12 @@ void or_eval(int p, int q, int nu, int nv, const double *ku, const double *kv,
                 out[(9 + c) * n + i] = o.Suu[c];
                 out[(12 + c) * n + i] = o.Suv[c];
                 out[(15 + c) * n + i] = o.Svv[c];
+            }
+            if (order >= 3) {
+                out[(18 + c) * n + i] = o.Suuu[c];
+                out[(21 + c) * n + i] = o.Suuv[c];
+                out[(24 + c) * n + i] = o.Suvv[c];
+                out[(27 + c) * n + i] = o.Svvv[c];
             }
         }
     }

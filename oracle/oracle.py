@@ -111,7 +111,7 @@ class OSurf:
     def eval(self, u, v, order=0, nthreads=0) -> np.ndarray:
         u, v = _a(u), _a(v)
         n = u.size
-        nblk = (3, 9, 18)[order]
+        nblk = (3, 9, 18, 30)[order]
         out = np.zeros((nblk, n))
         lib().or_eval(*self._args(), u.ctypes.data, v.ctypes.data, n, order, out.ctypes.data, nthreads)
         return out

@@ -56,6 +56,43 @@ def _basis(knots, deg, n, x, der):
     return out
 
 
+DER_NAMES = ["S", "Su", "Sv", "Suu", "Suv", "Svv", "Suuu", "Suuv", "Suvv", "Svvv"]
+DER_INDEX = {"S": (0, 0), "Su": (1, 0), "Sv": (0, 1), "Suu": (2, 0), "Suv": (1, 1), "Svv": (0, 2),
+             "Suuu": (3, 0), "Suuv": (2, 1), "Suvv": (1, 2), "Svvv": (0, 3)}
+
+
+def scipy_eval_general(p, q, ku, kv, cp, w, u, v, order=3, return_hom=False):
+    """Independent rational evaluation of every derivative S^(k,l), k + l <= order, by the generic
+    quotient rule of The NURBS Book eq. 4.20 (Piegl & Tiller, algorithm A4.4): with A = w S,
+    S^(k,l) = (A^(k,l) - sum_{(i,j) != (0,0)} C(k,i) C(l,j) w^(i,j) S^(k-i,l-j)) / w.
+    Returns a dict keyed by DER_NAMES, each (3, n); with return_hom also the homogeneous
+    derivatives A[k, l] (3, n) and W[k, l] (n)."""
+    from math import comb
+    u = np.asarray(u, dtype=np.float64)
+    v = np.asarray(v, dtype=np.float64)
+    nu, nv = w.shape
+    Bu = [_basis(ku, p, nu, u, d) for d in range(order + 1)]
+    Bv = [_basis(kv, q, nv, v, d) for d in range(order + 1)]
+    Pw = cp * w[..., None]
+    A, W, S = {}, {}, {}
+    for k in range(order + 1):
+        for l in range(order + 1 - k):
+            A[k, l] = np.einsum("ni,ijc,nj->cn", Bu[k], Pw, Bv[l])
+            W[k, l] = np.einsum("ni,ij,nj->n", Bu[k], w, Bv[l])
+    for tot in range(order + 1):
+        for k in range(tot, -1, -1):
+            l = tot - k
+            acc = A[k, l].copy()
+            for i in range(k + 1):
+                for j in range(l + 1):
+                    if i == 0 and j == 0:
+                        continue
+                    acc -= comb(k, i) * comb(l, j) * W[i, j] * S[k - i, l - j]
+            S[k, l] = acc / W[0, 0]
+    res = {name: S[idx] for name, idx in DER_INDEX.items() if sum(idx) <= order}
+    return (res, A, W) if return_hom else res
+
+
 def scipy_eval(p, q, ku, kv, cp, w, u, v, order=2):
     """Independent rational evaluation: returns dict of S, Su, Sv, Suu, Suv, Svv, each (3, n)."""
     u = np.asarray(u, dtype=np.float64)
@@ -91,8 +128,7 @@ def scipy_eval(p, q, ku, kv, cp, w, u, v, order=2):
 
 def blocks(out):
     """Split an SoA evaluation array (nblk, n) into named (3, n) blocks."""
-    names = ["S", "Su", "Sv", "Suu", "Suv", "Svv"]
-    return {names[i]: out[3 * i: 3 * i + 3] for i in range(out.shape[0] // 3)}
+    return {DER_NAMES[i]: out[3 * i: 3 * i + 3] for i in range(out.shape[0] // 3)}
 
 
 def field_close(a, b, rtol, floor=0.0):
