@@ -88,7 +88,7 @@ class NurbsSurface {
         check(gn_get_span_matrix(s_, i, j, m.data()));
         return m;
     }
-    // points (order 0) and derivatives up to order 2, SoA blocks of n on the device
+    // points (order 0) and derivatives up to order 3, SoA blocks of n on the device
     void EvaluateRandom(const double *u, const double *v, size_t n, int order, double *out,
                         gn_stream st = nullptr) const {
         check(gn_evaluate(s_, u, v, n, order, out, st));

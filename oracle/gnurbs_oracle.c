@@ -23,7 +23,9 @@
  *   or_get_matrix_n     contraction against scipy NURBS evaluation; known-answer points of
  *                       src/kernel/kernel.cu:451-454 on model/center_bk.nu and model/test_face2.nu
  *   or_eval             scipy NURBS value, finite differences of order k -> k+1, Leibniz form,
- *                       closed forms (bilinear patch, exact quarter cylinder)
+ *                       closed forms (bilinear patch, exact quarter cylinder); third order against
+ *                       the generic quotient rule (The NURBS Book eq. 4.20) on scipy's basis and
+ *                       sympy derivatives of rational Bezier patches (tests/test_oracle_order3.py)
  *   or_gauss_newton     round trips on on-surface points, orthogonality of the residual
  *   or_build_tree       containment of dense samples, index formula, NeedDivide truth table
  *   or_seed_*           brute force
