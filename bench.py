@@ -245,15 +245,29 @@ def bench_serve(G, synth, torch, duration=1.0, loads=(0.5, 0.8), budget=32 << 30
         info = G.RequestInfo()
         done = np.zeros(n_req, dtype=np.int64)
         subm = np.zeros(n_req, dtype=np.int64)
+        launch = np.zeros(n_req, dtype=np.int64)
         for i in range(n_req):
             rc = L.gn_server_wait(h, int(ids[i]), 120000, ctypes.byref(info))
             if rc:
                 raise G.GnError(rc, L.gn_last_error().decode())
-            done[i], subm[i] = info.done_ns, info.submit_ns
+            done[i], subm[i], launch[i] = info.done_ns, info.submit_ns, info.launch_ns
             L.gn_server_release(h, int(ids[i]))
         lat = (done - (t0 + arr_ns)) / 1e6
         span = (done.max() - t0 - arr_ns.min()) / 1e9
-        return lat, (subm - (t0 + arr_ns)) / 1e6, span
+        # steps: requests launched together; duration launch -> completion vs points in the step
+        _, step_of = np.unique(launch, return_inverse=True)
+        pts = np.bincount(step_of, weights=sizes.astype(np.float64))
+        dur = np.zeros(pts.size)
+        dur[step_of] = (done - launch) / 1e9
+        return lat, (subm - (t0 + arr_ns)) / 1e6, span, (pts, dur)
+
+    def step_model(pts, dur):
+        """Least-squares duration = fixed + points / rate over the steps of one run."""
+        A = np.stack([np.ones_like(pts), pts], axis=1)
+        (a, b), *_ = np.linalg.lstsq(A, dur, rcond=None)
+        return {"steps": int(pts.size), "mean_step_points": float(pts.mean()),
+                "mean_step_ms": float(dur.mean() * 1e3), "fixed_ms": float(a * 1e3),
+                "in_step_points_per_s": float(1.0 / b) if b > 0 else None}
 
     mixed = dict(precision=G.GN_MIXED, newton=G.NEWTON_FP32 | G.NEWTON_FP64, fp32_iters=20, fp64_tail=3,
                  tol_step32=1e-4, tol_step=1e-10, tol_quad=1e-6)
@@ -265,7 +279,7 @@ def bench_serve(G, synth, torch, duration=1.0, loads=(0.5, 0.8), budget=32 << 30
             srv = G.Server(batch, curves, G.default_opts(**opts), hbm_budget=budget, streams=streams)
             sizes, objs, offs, arr = trace(4000, 0, 77)
             run(srv, sizes[:200], objs[:200], offs[:200], arr[:200])  # warm-up
-            lat, _, span = run(srv, sizes, objs, offs, arr)  # saturating burst: capacity
+            lat, _, span, _ = run(srv, sizes, objs, offs, arr)  # saturating burst: capacity
             cap = float(sizes.sum()) / span
             out = {"capacity_points_per_s": cap}
             arena = srv.stats().arena_bytes
@@ -275,7 +289,7 @@ def bench_serve(G, synth, torch, duration=1.0, loads=(0.5, 0.8), budget=32 << 30
                 n_req = max(200, int(rate * dur))
                 sizes, objs, offs, arr = trace(n_req, rate, int(1004 + 100 * load))
                 st0 = srv.stats()
-                lat, lag, span = run(srv, sizes, objs, offs, arr)
+                lat, lag, span, steps = run(srv, sizes, objs, offs, arr)
                 st1 = srv.stats()
                 mid = (sizes >= 5000) & (sizes <= 20000)
                 out[f"load_{int(load * 100)}"] = {
@@ -286,6 +300,7 @@ def bench_serve(G, synth, torch, duration=1.0, loads=(0.5, 0.8), budget=32 << 30
                     "p99_ms_10k": float(np.percentile(lat[mid], 99)) if mid.any() else None,
                     "max_ms": float(lat.max()), "submit_lag_p99_ms": float(np.percentile(lag, 99)),
                     "steps": int(st1.steps - st0.steps), "max_step_requests": int(st1.max_step_requests),
+                    "step_model": step_model(*steps),
                 }
             out["arena_peak_bytes"] = int(srv.stats().arena_peak)
             srv.close()

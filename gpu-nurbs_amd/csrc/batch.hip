@@ -303,41 +303,52 @@ std::mutex g_side_mu;
 SideStreams g_side[64];  // per device, created on first use
 }  // namespace
 
+// Seeds of one group range (one lane per point; blocks cover tiles in per_tile pieces).
+static int seed_range(const GnBatchObj *objs, const GnTile *tiles, const GroupRange &gr,
+                      const GnReq *reqs, int seed_mode, hipStream_t st) {
+    const int per_tile = (std::max(gr.max_count, 1) + kSeedBlock - 1) / kSeedBlock;
+    const long long nb = (long long)(gr.t1 - gr.t0) * per_tile;
+    const unsigned grid = (unsigned)(nb < (1 << 20) ? nb : (1 << 20));
+    k_batch_seed<<<grid, kSeedBlock, 0, st>>>(objs, tiles + gr.t0, nb, per_tile, reqs, seed_mode);
+    GN_HIP(hipGetLastError());
+    return GN_OK;
+}
+
+static int run_range(const GnBatchObj *objs, const GnTile *tiles, const GroupRange &gr,
+                     const GnReq *reqs, const gn_invert_opts &o, int bpc_cap, hipStream_t st) {
+    if (o.seed_mode != GN_SEED_GIVEN) {
+        const int rc = seed_range(objs, tiles, gr, reqs, o.seed_mode, st);
+        if (rc) return rc;
+    }
+    const BatchArgs a{objs, tiles + gr.t0, gr.t1 - gr.t0, reqs, o};
+    return gr.grp == 0 ? group_phases<0>(a, gr.points, bpc_cap, st)
+         : gr.grp == 1 ? group_phases<1>(a, gr.points, bpc_cap, st)
+                       : group_phases<2>(a, gr.points, bpc_cap, st);
+}
+
 static int launch_ranges(const GnBatchObj *objs, const GnTile *tiles,
                          const std::vector<GroupRange> &ranges, const GnReq *reqs,
                          const gn_invert_opts &o, int bpc_cap, hipStream_t st,
-                         const SideStreams &ss) {
-    if (o.seed_mode != GN_SEED_GIVEN) {
-        const int t0 = ranges.front().t0, nt = ranges.back().t1 - t0;
-        int maxc = 1;
-        for (const GroupRange &gr : ranges) maxc = gr.max_count > maxc ? gr.max_count : maxc;
-        const int per_tile = (maxc + kSeedBlock - 1) / kSeedBlock;
-        const long long nb = (long long)nt * per_tile;
-        const unsigned grid = (unsigned)(nb < (1 << 20) ? nb : (1 << 20));
-        k_batch_seed<<<grid, kSeedBlock, 0, st>>>(objs, tiles + t0, nb, per_tile, reqs, o.seed_mode);
-        GN_HIP(hipGetLastError());
-    }
-    if (ranges.size() == 1) {  // one group: no fork
-        const GroupRange &gr = ranges[0];
-        const BatchArgs a{objs, tiles + gr.t0, gr.t1 - gr.t0, reqs, o};
-        return gr.grp == 0 ? group_phases<0>(a, gr.points, bpc_cap, st)
-             : gr.grp == 1 ? group_phases<1>(a, gr.points, bpc_cap, st)
-                           : group_phases<2>(a, gr.points, bpc_cap, st);
-    }
+                         const SideStreams &ss, const std::function<int(hipStream_t)> &beside) {
+    if (ranges.size() == 1 && !beside)  // one group, nothing beside it: no fork
+        return run_range(objs, tiles, ranges[0], reqs, o, bpc_cap, st);
+    if (ranges.size() > (size_t)kSideStreams)
+        return fail(GN_EDEVICE, "more group ranges than side streams (tiles not sorted by class)");
     GN_HIP(hipEventRecord(ss.fork, st));
     int k = 0;
     for (const GroupRange &gr : ranges) {
         if (gr.points == 0) continue;
         GN_HIP(hipStreamWaitEvent(ss.s[k], ss.fork, 0));
-        const BatchArgs a{objs, tiles + gr.t0, gr.t1 - gr.t0, reqs, o};
-        const int rc = gr.grp == 0 ? group_phases<0>(a, gr.points, bpc_cap, ss.s[k])
-                     : gr.grp == 1 ? group_phases<1>(a, gr.points, bpc_cap, ss.s[k])
-                                   : group_phases<2>(a, gr.points, bpc_cap, ss.s[k]);
+        const int rc = run_range(objs, tiles, gr, reqs, o, bpc_cap, ss.s[k]);
         if (rc) return rc;
         GN_HIP(hipEventRecord(ss.join[k], ss.s[k]));
-        GN_HIP(hipStreamWaitEvent(st, ss.join[k], 0));
         k++;
     }
+    if (beside) {
+        const int rc = beside(st);
+        if (rc) return rc;
+    }
+    for (int j = 0; j < k; j++) GN_HIP(hipStreamWaitEvent(st, ss.join[j], 0));
     return GN_OK;
 }
 
@@ -359,16 +370,16 @@ int batch_prepare() {
 int launch_batch_invert(const GnBatchObj *objs, const GnTile *tiles,
                         const std::vector<GroupRange> &ranges, const GnReq *reqs,
                         const gn_invert_opts &o, int bpc_cap, hipStream_t st,
-                        const SideStreams *side) {
-    if (ranges.empty()) return GN_OK;
-    if (side) return launch_ranges(objs, tiles, ranges, reqs, o, bpc_cap, st, *side);
+                        const SideStreams *side, const std::function<int(hipStream_t)> &beside) {
+    if (ranges.empty()) return beside ? beside(st) : GN_OK;
+    if (side) return launch_ranges(objs, tiles, ranges, reqs, o, bpc_cap, st, *side, beside);
     int dev = 0;
     GN_HIP(hipGetDevice(&dev));
     if (dev < 0 || dev >= 64) return fail(GN_EDEVICE, "device index out of range");
     std::lock_guard<std::mutex> lk(g_side_mu);  // one user of the shared set at a time
     const int rc = shared_side_locked(dev);
     if (rc) return rc;
-    return launch_ranges(objs, tiles, ranges, reqs, o, bpc_cap, st, g_side[dev]);
+    return launch_ranges(objs, tiles, ranges, reqs, o, bpc_cap, st, g_side[dev], beside);
 }
 
 }  // namespace gn

@@ -4,6 +4,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -199,7 +200,7 @@ int class_tile(size_t points, int waves, int cap);
 // Side streams and fork/join events for running independent launches concurrently.  A set is
 // used by one caller at a time (the batched entry points share one per device under a lock; a
 // server owns one per step slot).
-constexpr int kSideStreams = 4;
+constexpr int kSideStreams = 3;  // one per class group
 struct SideStreams {
     hipStream_t s[kSideStreams] = {};
     hipEvent_t fork = nullptr, join[kSideStreams] = {};
@@ -208,13 +209,15 @@ int side_streams_create(SideStreams *ss);
 // Creates the device's shared side-stream set (call before a graph capture).
 int batch_prepare();
 void side_streams_destroy(SideStreams *ss);
-// Seeds (unless GN_SEED_GIVEN), then one persistent launch per group range and precision phase;
-// the groups run concurrently on the side streams (forked from and joined back into st); with
-// side == nullptr the device's shared set is used.
+// Per group range: seeds (unless GN_SEED_GIVEN), then one persistent launch per precision
+// phase.  The groups run concurrently on the side streams (forked from and joined back into st);
+// `beside`, if given, enqueues further work on st that runs alongside them.  With side ==
+// nullptr the device's shared set is used.
 int launch_batch_invert(const GnBatchObj *objs, const GnTile *tiles,
                         const std::vector<GroupRange> &ranges, const GnReq *reqs,
                         const gn_invert_opts &o, int bpc_cap, hipStream_t st,
-                        const SideStreams *side = nullptr);
+                        const SideStreams *side = nullptr,
+                        const std::function<int(hipStream_t)> &beside = {});
 // resident waves of one persistent batch launch (sizes tiles for short serving steps)
 int batch_waves(int bpc_cap);
 // tiles of curve objects (obj indexes `curves`); max_samples: largest seed sample count

@@ -1,13 +1,14 @@
 // server.cpp -- continuous-batching inversion server (gn_server_*).
 //
-// One worker thread per server.  Submitted requests wait in arrival order until their result
-// buffers fit the HBM arena (admission); every admitted request that is not yet running joins
-// the next step.  A step is one upload of its request table and tile lists (pinned staging ->
-// device, one copy) followed by one batched launch per object kind on the step slot's stream and
-// an event; up to `streams` steps are in flight at once, so requests arriving while a step runs
-// are gathered into the next one as soon as a slot frees.  Completion is found by polling the
-// slots' events.  Every point is computed by the same per-point iteration whatever else shares
-// its step, so a request's results are the same alone and in a mix.
+// One dispatcher thread and one thread per step slot.  Submitted requests wait in arrival order
+// until their result buffers fit the HBM arena (admission); the dispatcher hands every admitted
+// request that is not yet running to the next free slot.  The slot's thread builds the step (its
+// request table and tile lists, written in class order straight into pinned staging), uploads it
+// with one copy, launches the class groups and curves on the slot's streams, and polls the
+// step's event; up to `streams` steps are in flight at once, so requests arriving while a step
+// runs are gathered into the next one as soon as a slot frees, and one slot's launch work never
+// delays another slot's completion.  Every point is computed by the same per-point iteration
+// whatever else shares its step, so a request's results are the same alone and in a mix.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -61,8 +62,11 @@ struct Slot {
     gn::SideStreams side;  // the step's class launches run concurrently on these
     char *h_stage = nullptr;  // pinned
     char *d_stage = nullptr;
-    std::vector<Req *> reqs;
-    bool busy = false;
+    std::vector<Req *> reqs;  // the step's requests (set by the dispatcher under the lock)
+    bool busy = false;        // assigned and not finished (under the lock)
+    bool go = false;          // a new assignment for the slot's thread (under the lock)
+    std::condition_variable cv;
+    std::thread th;
 };
 
 constexpr size_t kStageReqs = sizeof(GnReq) * kMaxStepReqs;
@@ -89,19 +93,20 @@ struct gn_server {
     size_t budget = 0, resident = 0;
     char *d_arena = nullptr;
     gn::Arena arena;
-    std::vector<Slot> slots;
+    std::deque<Slot> slots;  // not moved after creation (threads hold references)
 
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
     std::deque<Req *> waiting, admitted;
     std::unordered_map<int64_t, Req *> reqs;
     int64_t next_id = 1;
-    bool stop = false, dirty = false;
+    bool stop = false, dirty = false, quit = false;
     gn_server_stats stats{};
     std::thread worker;
 
     void admit_locked();
     void run();
+    void run_slot(Slot &sl);
     int launch_step(Slot &sl);
     void finish(Slot &sl, bool ok);
 };
@@ -135,8 +140,8 @@ void gn_server::admit_locked() {
     if (arena.peak() > stats.arena_peak) stats.arena_peak = arena.peak();
 }
 
-// Builds the step image in the slot's pinned staging and launches it.  Called without the lock;
-// sl.reqs were taken from `admitted` by the worker.
+// Builds the step image in the slot's pinned staging and launches it.  Runs on the slot's thread
+// without the lock; sl.reqs were handed over by the dispatcher.
 int gn_server::launch_step(Slot &sl) {
     const int nreq = (int)sl.reqs.size();
     size_t spts = 0, cpts = 0;
@@ -162,50 +167,56 @@ int gn_server::launch_step(Slot &sl) {
     }
     size_t tc = kCurveTile;
     if (cpts / tc > room / 2) tc = (cpts + room / 2 - 1) / (room / 2);
+    // Tiles in class_order (group, then class; stable in request order) by a counting pass:
+    // count tiles per order key, then write each request's tiles at its key's running offset.
+    constexpr int kKeys = 3 * 32;
+    size_t cnt[kKeys + 1] = {};  // [kKeys]: curves, after every surface tile
+    auto key_of = [&](const Req *r) { return r->curve ? kKeys : gn::class_order(cls[r->obj]); };
+    auto tile_of = [&](const Req *r) { return r->curve ? tc : tsz[gn::class_group(cls[r->obj])]; };
+    for (const Req *r : sl.reqs) {
+        const int key = key_of(r);
+        if (key < 0 || key > kKeys) return fail(GN_EDEVICE, "degree class out of range");
+        const size_t T = tile_of(r);
+        cnt[key] += (r->n + T - 1) / T;
+    }
+    size_t pos[kKeys + 1], total = 0;
+    for (int k = 0; k <= kKeys; k++) {
+        pos[k] = total;
+        total += cnt[k];
+    }
+    if (total > (size_t)kMaxStepTiles) return fail(GN_EDEVICE, "step tile list overflow");
+    const size_t ns = pos[kKeys], nc = cnt[kKeys];
+    // staging image: request table, then the tiles (256-byte aligned); one copy of what is used
     GnReq *rq = (GnReq *)sl.h_stage;
-    std::vector<GnTile> st_tiles, ct_tiles;
+    const size_t toff = a256(sizeof(GnReq) * nreq);
+    GnTile *tl = (GnTile *)(sl.h_stage + toff);
     for (int k = 0; k < nreq; k++) {
         Req *r = sl.reqs[k];
         rq[k] = GnReq{r->x, r->y, r->z, r->u, r->v, r->status, nullptr, nullptr};
-        const size_t T = r->curve ? tc : tsz[gn::class_group(cls[r->obj])];
-        for (size_t off = 0; off < r->n; off += T) {
-            const size_t c = std::min(T, r->n - off);
-            GnTile t{r->curve ? r->obj - nsurf : r->obj, (int32_t)c, (int64_t)off, k, 0};
-            (r->curve ? ct_tiles : st_tiles).push_back(t);
-        }
+        const size_t T = tile_of(r);
+        size_t &p = pos[key_of(r)];
+        const int obj = r->curve ? r->obj - nsurf : r->obj;
+        for (size_t off = 0; off < r->n; off += T)
+            tl[p++] = GnTile{obj, (int32_t)std::min(T, r->n - off), (int64_t)off, k, 0};
     }
-    std::stable_sort(st_tiles.begin(), st_tiles.end(),
-                     [this](const GnTile &a, const GnTile &b) {
-                         return gn::class_order(cls[a.obj]) < gn::class_order(cls[b.obj]);
-                     });
-    const size_t ns = st_tiles.size(), nc = ct_tiles.size();
-    if (ns + nc > (size_t)kMaxStepTiles) return fail(GN_EDEVICE, "step tile list overflow");
-    // staging image: request table, then the tiles (256-byte aligned); one copy of what is used
-    const size_t toff = a256(sizeof(GnReq) * nreq);
-    GnTile *tl = (GnTile *)(sl.h_stage + toff);
-    if (ns) std::memcpy(tl, st_tiles.data(), sizeof(GnTile) * ns);
-    if (nc) std::memcpy(tl + ns, ct_tiles.data(), sizeof(GnTile) * nc);
     const size_t bytes = toff + sizeof(GnTile) * (ns + nc);
     GN_HIP(hipMemcpyAsync(sl.d_stage, sl.h_stage, bytes, hipMemcpyHostToDevice, sl.st));
     const GnReq *d_rq = (const GnReq *)sl.d_stage;
     const GnTile *d_tl = (const GnTile *)(sl.d_stage + toff);
+    // surface groups (seeds and precision phases) on the slot's side streams, curves on the
+    // slot's stream beside them; all joined back before the step's event
+    std::function<int(hipStream_t)> curves;
+    if (nc)
+        curves = [&](hipStream_t cs) {
+            return gn::launch_curve_batch(d_curves, d_tl + ns, (int)nc, d_rq, max_samples, o, cs);
+        };
     int rc = GN_OK;
-    // curves on the last side stream (the surface groups take the first ones), beside the
-    // surface seeds and group launches; joined back before the step's event
-    hipStream_t cs = sl.side.s[gn::kSideStreams - 1];
-    if (nc) {
-        GN_HIP(hipEventRecord(sl.side.fork, sl.st));
-        GN_HIP(hipStreamWaitEvent(cs, sl.side.fork, 0));
-        rc = gn::launch_curve_batch(d_curves, d_tl + ns, (int)nc, d_rq, max_samples, o, cs);
-    }
-    if (!rc && ns)
-        rc = gn::launch_batch_invert(d_objs, d_tl, gn::group_ranges(st_tiles.data(), ns, cls),
-                                     d_rq, o, bpc, sl.st, &sl.side);
+    if (ns)
+        rc = gn::launch_batch_invert(d_objs, d_tl, gn::group_ranges(tl, ns, cls), d_rq, o, bpc,
+                                     sl.st, &sl.side, curves);
+    else if (nc)
+        rc = curves(sl.st);
     if (rc) return rc;
-    if (nc) {
-        GN_HIP(hipEventRecord(sl.side.join[gn::kSideStreams - 1], cs));
-        GN_HIP(hipStreamWaitEvent(sl.st, sl.side.join[gn::kSideStreams - 1], 0));
-    }
     GN_HIP(hipEventRecord(sl.ev, sl.st));
     return GN_OK;
 }
@@ -224,51 +235,65 @@ void gn_server::finish(Slot &sl, bool ok) {
         }
         sl.reqs.clear();
         sl.busy = false;
+        dirty = true;
     }
     cv_done.notify_all();
+    cv_work.notify_one();
 }
 
+// A slot's thread: launches each step handed to it, then polls the step's event.
+void gn_server::run_slot(Slot &sl) {
+    (void)hipSetDevice(device);
+    std::unique_lock<std::mutex> lk(mu);
+    while (true) {
+        sl.cv.wait(lk, [&] { return sl.go || quit; });
+        if (!sl.go) return;  // quit with nothing assigned
+        sl.go = false;
+        lk.unlock();
+        bool ok = launch_step(sl) == GN_OK;
+        if (!ok) {
+            (void)hipStreamSynchronize(sl.st);
+        } else {
+            hipError_t q;
+            while ((q = hipEventQuery(sl.ev)) == hipErrorNotReady) std::this_thread::yield();
+            ok = q == hipSuccess;
+        }
+        finish(sl, ok);
+        lk.lock();
+    }
+}
+
+// The dispatcher: admission and hand-over of admitted requests to free slots.
 void gn_server::run() {
     (void)hipSetDevice(device);
+    std::unique_lock<std::mutex> lk(mu);
     while (true) {
-        bool progressed = false, any_busy = false;
-        for (Slot &sl : slots) {
-            if (!sl.busy) continue;
-            const hipError_t q = hipEventQuery(sl.ev);
-            if (q == hipErrorNotReady) {
-                any_busy = true;
-                continue;
-            }
-            finish(sl, q == hipSuccess);
-            progressed = true;
-        }
-        Slot *free_slot = nullptr;
-        for (Slot &sl : slots)
-            if (!sl.busy) {
-                free_slot = &sl;
-                break;
-            }
-        {
-            std::unique_lock<std::mutex> lk(mu);
-            dirty = false;
-            admit_locked();
-            if (stop) {
-                free_slot = nullptr;
-                bool busy = false;
-                for (Slot &sl : slots) busy |= sl.busy;
-                if (!busy) {  // nothing in flight: fail what never ran, then exit
-                    for (auto *dq : {&waiting, &admitted})
-                        for (Req *r : *dq) {
-                            r->state = kFailed;
-                            r->done_ns = gn_now_ns_impl();
-                        }
-                    waiting.clear();
-                    admitted.clear();
-                    lk.unlock();
-                    cv_done.notify_all();
-                    return;
+        dirty = false;
+        admit_locked();
+        bool any_busy = false;
+        for (Slot &sl : slots) any_busy |= sl.busy;
+        if (stop && !any_busy) {  // nothing in flight: fail what never ran, stop the slots, exit
+            for (auto *dq : {&waiting, &admitted})
+                for (Req *r : *dq) {
+                    r->state = kFailed;
+                    r->done_ns = gn_now_ns_impl();
                 }
-            } else if (free_slot && !admitted.empty()) {
+            waiting.clear();
+            admitted.clear();
+            quit = true;
+            for (Slot &sl : slots) sl.cv.notify_one();
+            lk.unlock();
+            cv_done.notify_all();
+            return;
+        }
+        if (!stop && !admitted.empty()) {
+            Slot *free_slot = nullptr;
+            for (Slot &sl : slots)
+                if (!sl.busy) {
+                    free_slot = &sl;
+                    break;
+                }
+            if (free_slot) {
                 const int64_t now = gn_now_ns_impl();
                 while (!admitted.empty() && (int)free_slot->reqs.size() < kMaxStepReqs) {
                     Req *r = admitted.front();
@@ -278,27 +303,15 @@ void gn_server::run() {
                     admitted.pop_front();
                 }
                 free_slot->busy = true;
+                free_slot->go = true;
                 stats.steps++;
                 stats.max_step_requests =
                     std::max<int64_t>(stats.max_step_requests, (int64_t)free_slot->reqs.size());
-            } else {
-                free_slot = nullptr;
-            }
-            if (!free_slot && !progressed) {
-                if (!any_busy) {
-                    cv_work.wait_for(lk, std::chrono::milliseconds(2), [this] { return dirty || stop; });
-                    continue;
-                }
+                free_slot->cv.notify_one();
+                continue;
             }
         }
-        if (free_slot) {
-            if (launch_step(*free_slot) != GN_OK) {
-                (void)hipStreamSynchronize(free_slot->st);
-                finish(*free_slot, false);
-            }
-            continue;
-        }
-        if (!progressed && any_busy) std::this_thread::yield();
+        cv_work.wait_for(lk, std::chrono::milliseconds(2), [this] { return dirty || stop; });
     }
 }
 
@@ -366,7 +379,7 @@ int gn_server_create(gn_batch *batch, gn_curve *const *curves, int ncurves,
     }
     s->arena.reset(arena);
     s->stats.arena_bytes = arena;
-    s->slots.resize(streams);
+    for (int k = 0; k < streams; k++) s->slots.emplace_back();
     for (Slot &sl : s->slots) {
         if (e == hipSuccess) e = hipStreamCreateWithFlags(&sl.st, hipStreamNonBlocking);
         if (e == hipSuccess) e = hipEventCreateWithFlags(&sl.ev, hipEventDisableTiming);
@@ -386,6 +399,10 @@ int gn_server_create(gn_batch *batch, gn_curve *const *curves, int ncurves,
         (void)hipFree(s->d_curves);
         delete s;
         return gn::hip_fail(e, "server set-up");
+    }
+    for (Slot &sl : s->slots) {
+        Slot *p = &sl;
+        sl.th = std::thread([s, p] { s->run_slot(*p); });
     }
     s->worker = std::thread([s] { s->run(); });
     *out = s;
@@ -497,7 +514,9 @@ int gn_server_destroy(gn_server *s) {
         s->stop = true;
     }
     s->cv_work.notify_all();
-    if (s->worker.joinable()) s->worker.join();
+    if (s->worker.joinable()) s->worker.join();  // returns once no step is in flight
+    for (Slot &sl : s->slots)
+        if (sl.th.joinable()) sl.th.join();
     for (Slot &sl : s->slots) {
         (void)hipStreamSynchronize(sl.st);
         (void)hipStreamDestroy(sl.st);
