@@ -4,10 +4,10 @@
 // One lane per query point, grid-stride over the batch.  Parameters stream in and results stream
 // out as SoA arrays, so every global load and store of the batch is a fully coalesced 256-byte
 // (FP32) or 512-byte (FP64) wave access.  Span patches come from the L2-resident span table:
-// staged through LDS by cooperative contiguous loads for patches up to 512 bytes (k_eval_lds,
-// whose knots also live in LDS for the span search), gathered per lane otherwise (k_eval, knots
-// read through L1).  The target roof is HBM traffic (8 or 16 bytes in, 12..240 bytes out per
-// point).
+// for exact degrees they are staged through LDS one row at a time by cooperative contiguous
+// loads, with knots and reciprocal span lengths also in LDS (k_eval_rows); otherwise each lane
+// gathers its own patch and reads knots through L1 (k_eval).  The target roof is HBM traffic
+// (8 or 16 bytes in, 12..240 bytes out per point).
 #include "gn_device.hpp"
 
 namespace gn {
@@ -60,36 +60,38 @@ __global__ __launch_bounds__(kEvalBlock) void k_eval(GnSurfDev d, const TP *__re
     }
 }
 
-// LDS-staged form for small patches (at most 512 bytes: up to bicubic in FP64).  A wave takes 64
-// points at a time; it first gathers their 64 span patches cooperatively -- lane l loads 16-byte
-// vector l of the wave's patch list per instruction, so each load instruction covers whole
-// contiguous patches instead of 64 scattered lines -- into its own LDS area (patch stride padded
-// by 16 bytes against bank conflicts), then every lane contracts its own patch out of LDS.
-constexpr int kLdsBlock = 128;
-constexpr int kLdsWaves = kLdsBlock / 64;
 constexpr int kLdsKnotsSmall = 512;
 
+// Row-staged form for exact degrees with knots in LDS: a wave takes 64 points at a time and
+// stages their patches one row at a time through a small LDS buffer (WaveRowsPipe: contiguous
+// 16-byte loads covering whole rows of several lanes, the next row prefetched while the current
+// one is contracted).  The buffer is one row per lane (9 KB per wave for FP64 bicubic), so
+// occupancy is set by registers rather than LDS.
+constexpr int kRowBlock = 256;
+constexpr int kRowWaves = kRowBlock / 64;
+
 template <int K1, int K2, int ORDER, typename T, typename TP>
-__global__ __launch_bounds__(kLdsBlock) void k_eval_lds(GnSurfDev d, const TP *__restrict__ u,
+__global__ __launch_bounds__(kRowBlock) void k_eval_rows(GnSurfDev d, const TP *__restrict__ u,
                                                          const TP *__restrict__ v, size_t n,
                                                          T *__restrict__ out) {
-    constexpr int kVec = K1 * K2 * 4 * (int)sizeof(T) / 16;  // 16-byte vectors per patch
-    constexpr int kStride = kVec + 1;                         // padded patch stride (vectors)
-    static_assert(kVec * 16 <= 512, "patch too large for LDS staging");
-    __shared__ double sk[kLdsKnotsSmall];
-    __shared__ float4 stage[kLdsWaves * 64 * kStride];
-    const int nku = d.nu + d.p + 1, nkv = d.nv + d.q + 1;  // nku + nkv <= kLdsKnotsSmall
+    using Rows = WaveRowsPipe<K1, K2, T>;
+    // knots and reciprocal span lengths: nku + nkv + (nu - p) + (nv - q) <= 2 * kLdsKnotsSmall
+    __shared__ double sk[2 * kLdsKnotsSmall];
+    __shared__ float4 stage[kRowWaves * 64 * Rows::STRIDE];
+    const int nku = d.nu + d.p + 1, nkv = d.nv + d.q + 1, nsu = d.nu - d.p, nsv = d.nv - d.q;
     for (int i = threadIdx.x; i < nku; i += blockDim.x) sk[i] = d.ku[i];
     for (int i = threadIdx.x; i < nkv; i += blockDim.x) sk[nku + i] = d.kv[i];
+    for (int i = threadIdx.x; i < nsu; i += blockDim.x) sk[nku + nkv + i] = d.ihu[i];
+    for (int i = threadIdx.x; i < nsv; i += blockDim.x) sk[nku + nkv + nsu + i] = d.ihv[i];
     __syncthreads();
-    const double *ku = sk, *kv = sk + nku;
+    const double *ku = sk, *kv = sk + nku, *ihus = sk + nku + nkv, *ihvs = ihus + nsu;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    float4 *my = stage + (size_t)wv * 64 * kStride;
-    const float4 *tab = reinterpret_cast<const float4 *>(sizeof(T) == 8 ? (const void *)d.tab64
-                                                                        : (const void *)d.tab32);
-    const size_t waves = (size_t)gridDim.x * kLdsWaves;
-    size_t base = ((size_t)blockIdx.x * kLdsWaves + wv) * 64;
-    // parameters of the next tile are loaded one tile ahead (software pipelining of the stream)
+    float4 *my = stage + (size_t)wv * 64 * Rows::STRIDE;
+    const T *tab = sizeof(T) == 8 ? reinterpret_cast<const T *>(d.tab64)
+                                  : reinterpret_cast<const T *>(d.tab32);
+    const size_t waves = (size_t)gridDim.x * kRowWaves;
+    size_t base = ((size_t)blockIdx.x * kRowWaves + wv) * 64;
+    // parameters of the next tile are loaded one tile ahead
     TP pu = 0, pv = 0;
     if (base + lane < n) { pu = u[base + lane]; pv = v[base + lane]; }
     for (; base < n; base += waves * 64) {
@@ -101,27 +103,14 @@ __global__ __launch_bounds__(kLdsBlock) void k_eval_lds(GnSurfDev d, const TP *_
         const int m = find_span(ku, d.p, d.nu, uu);
         const int nn = find_span(kv, d.q, d.nv, vv);
         const int im = m - d.p, in = nn - d.q;
-        const int slot = im * d.sv + in;
-        // cooperative gather of the 64 patches
-#pragma unroll 4
-        for (int j = 0; j < kVec; j++) {
-            const int vec = j * 64 + lane;
-            const int pp = vec / kVec, o = vec - pp * kVec;
-            const int sp = __shfl(slot, pp, 64);
-            my[pp * kStride + o] = tab[(size_t)sp * kVec + o];
-        }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        const double ihu = d.ihu[im], ihv = d.ihv[in];
+        const T *patch = tab + (size_t)(im * d.sv + in) * (K1 * K2 * 4);
+        Rows rows(patch, my);
+        const double ihu = ihus[im], ihv = ihvs[in];
         const T t = T((uu - ku[m]) * ihu), s = T((vv - kv[nn]) * ihv);
         HomDer<T> h;
-        contract<K1, K2, ORDER, true, T>(reinterpret_cast<const T *>(my + lane * kStride), K1, K2, t,
-                                         s, h);
+        contract_rows<K1, K2, ORDER, true, T>(rows, K1, K2, t, s, h);
         Cart<T> o;
         quotient<ORDER, T>(h, T(ihu), T(ihv), o);
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
         if (live) store_fields<ORDER>(o, out, n, i);
     }
 }
@@ -138,13 +127,13 @@ struct EvalLaunch {
     int operator()() {
         const unsigned grid = grid_for(n, kEvalBlock, 256 * 16);
         const bool knots_fit = d.nu + d.p + 1 + d.nv + d.q + 1 <= kLdsKnotsSmall;
-        if constexpr (EXACT && K1 * K2 * 4 * sizeof(T) <= 512) if (knots_fit) {
-            const unsigned g2 = grid_for(n, kLdsBlock, 256 * 32);
+        if constexpr (EXACT) if (knots_fit) {
+            const unsigned g3 = grid_for(n, kRowBlock, 256 * 16);
             switch (order) {
-                case 0: k_eval_lds<K1, K2, 0, T, TP><<<g2, kLdsBlock, 0, st>>>(d, u, v, n, out); break;
-                case 1: k_eval_lds<K1, K2, 1, T, TP><<<g2, kLdsBlock, 0, st>>>(d, u, v, n, out); break;
-                case 2: k_eval_lds<K1, K2, 2, T, TP><<<g2, kLdsBlock, 0, st>>>(d, u, v, n, out); break;
-                default: k_eval_lds<K1, K2, 3, T, TP><<<g2, kLdsBlock, 0, st>>>(d, u, v, n, out); break;
+                case 0: k_eval_rows<K1, K2, 0, T, TP><<<g3, kRowBlock, 0, st>>>(d, u, v, n, out); break;
+                case 1: k_eval_rows<K1, K2, 1, T, TP><<<g3, kRowBlock, 0, st>>>(d, u, v, n, out); break;
+                case 2: k_eval_rows<K1, K2, 2, T, TP><<<g3, kRowBlock, 0, st>>>(d, u, v, n, out); break;
+                default: k_eval_rows<K1, K2, 3, T, TP><<<g3, kRowBlock, 0, st>>>(d, u, v, n, out); break;
             }
             GN_HIP(hipGetLastError());
             return GN_OK;

@@ -195,6 +195,58 @@ struct WaveRows {
     }
 };
 
+// WaveRows with the next DEPTH rows prefetched: the global loads of rows r+1..r+DEPTH are in
+// flight while row r goes through LDS and is contracted, so up to DEPTH L2 round trips overlap.
+// Rows must be requested in order 0, 1, ..., K1-1 (contract_rows does), wave-uniformly.
+template <int K1, int K2, typename T, int DEPTH = 1>
+struct WaveRowsPipe {
+    static constexpr int VEC = WaveRows<K2, T>::VEC;
+    static constexpr int STRIDE = VEC + 1;
+    const char *src[VEC];
+    float4 *lds;
+    float4 nxt[DEPTH][VEC];
+    int lane;
+    __device__ __forceinline__ void fetch(int r) {
+#pragma unroll
+        for (int j = 0; j < VEC; j++)
+            nxt[r % DEPTH][j] = *reinterpret_cast<const float4 *>(src[j] + (size_t)r * VEC * 16);
+    }
+    __device__ __forceinline__ WaveRowsPipe(const T *patch, float4 *buf) : lds(buf) {
+        lane = threadIdx.x & 63;
+        const unsigned long long b = (unsigned long long)patch;
+        const unsigned lo = (unsigned)(b & 0xffffffffull), hi = (unsigned)(b >> 32);
+#pragma unroll
+        for (int j = 0; j < VEC; j++) {
+            const int vec = j * 64 + lane, pp = vec / VEC, o = vec - pp * VEC;
+            const unsigned long long pb = (unsigned long long)(unsigned)__shfl((int)lo, pp, 64) |
+                                          ((unsigned long long)(unsigned)__shfl((int)hi, pp, 64) << 32);
+            src[j] = reinterpret_cast<const char *>(pb) + o * 16;
+        }
+#pragma unroll
+        for (int r = 0; r < DEPTH && r < K1; r++) fetch(r);
+    }
+    __device__ __forceinline__ void operator()(int r, Hom<T> (&m)[K2]) {
+        float4 cur[VEC];
+#pragma unroll
+        for (int j = 0; j < VEC; j++) cur[j] = nxt[r % DEPTH][j];
+        if (r + DEPTH < K1) fetch(r + DEPTH);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");  // earlier reads of the buffer
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+        for (int j = 0; j < VEC; j++) {
+            const int vec = j * 64 + lane, pp = vec / VEC, o = vec - pp * VEC;
+            lds[pp * STRIDE + o] = cur[j];
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const T *mine = reinterpret_cast<const T *>(lds + lane * STRIDE);
+#pragma unroll
+        for (int c = 0; c < K2; c++) m[c] = load_hom<T>(mine + c * 4);
+    }
+};
+
 // A lane's current span patch held in registers (exact sizes only): reloaded from the table only
 // when the lane's span pair changes, so iterations that stay in their span issue no gathers.
 // Kernels declare one array of the largest size they use and pass its first element.
