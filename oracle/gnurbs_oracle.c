@@ -687,7 +687,8 @@ void or_seed_nearest_leaf(const double *nodes, int levels, const double *x, cons
  * centre (FP32 centres and distances, ties to the lower reference index) among the 3 x 3 leaves
  * around it. */
 void or_seed_greedy_leaf(const double *nodes, int levels, const double *x, const double *y,
-                         const double *z, long n, int *leaf_out, double *seed_u, double *seed_v)
+                         const double *z, long n, int *leaf_out, double *seed_u, double *seed_v,
+                         int nthreads)
 {
     int D = 3 * (levels - 1);
     long side = 1L << D, base = or_tree_base(levels - 1);
@@ -736,6 +737,12 @@ void or_seed_greedy_leaf(const double *nodes, int levels, const double *x, const
                 b32[l][i * 6 + 3 + k] = hi32;
             }
     }
+#ifdef _OPENMP
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel for schedule(static) num_threads(nthreads)
+#else
+    (void)nthreads;
+#endif
     for (long i = 0; i < n; i++) {
         double P[3] = {x[i], y[i], z[i]};
         float p[3] = {(float)P[0], (float)P[1], (float)P[2]};

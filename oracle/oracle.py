@@ -50,7 +50,7 @@ def lib():
         L.or_invert_first_hit.argtypes = [_I, _I, _I, _I, _P, _P, _P, _P, _P, _L, _P, _P, _P, _L,
                                           _D, _D, _I, _I, _P, _P, _P, _P, _I]
         L.or_seed_nearest_leaf.argtypes = [_P, _I, _P, _P, _P, _L, _P, _P, _P]
-        L.or_seed_greedy_leaf.argtypes = [_P, _I, _P, _P, _P, _L, _P, _P, _P]
+        L.or_seed_greedy_leaf.argtypes = [_P, _I, _P, _P, _P, _L, _P, _P, _P, _I]
         L.or_curve_eval.argtypes = [_I, _I, _P, _P, _P, _P, _L, _I, _P, _I]
         L.or_curve_seed.argtypes = [_I, _I, _P, _P, _P, _P, _P, _P, _L, _P]
         L.or_curve_invert.argtypes = [_I, _I, _P, _P, _P, _P, _P, _P, _L, _D, _D, _I, _I, _I, _P, _P, _P,
@@ -177,14 +177,14 @@ def seed_nearest_leaf(nodes, levels, x, y, z):
     return leaf, su, sv
 
 
-def seed_greedy_leaf(nodes, levels, x, y, z):
+def seed_greedy_leaf(nodes, levels, x, y, z, nthreads=0):
     nodes = _a(nodes)
     x, y, z = (_a(a) for a in (x, y, z))
     n = x.size
     leaf = np.zeros(n, dtype=np.int32)
     su, sv = np.zeros(n), np.zeros(n)
     lib().or_seed_greedy_leaf(nodes.ctypes.data, levels, x.ctypes.data, y.ctypes.data, z.ctypes.data, n,
-                              leaf.ctypes.data, su.ctypes.data, sv.ctypes.data)
+                              leaf.ctypes.data, su.ctypes.data, sv.ctypes.data, nthreads)
     return leaf, su, sv
 
 
