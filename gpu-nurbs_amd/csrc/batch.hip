@@ -3,16 +3,19 @@
 //
 // A batch packs the span tables, knots and seed trees of its surfaces into one allocation and
 // describes each surface by a GnBatchObj (pointers into the pack, degree class).  Work is a list
-// of tiles (object, contiguous point range, request) sorted by degree class.  Each degree class
-// present gets its own persistent launch of a kernel instantiated for that class alone (exact
-// unrolled contraction for degrees 2..5 in each direction, padded generic form otherwise), so
-// every launch has the register footprint of its own class -- one kernel holding all classes
-// would be sized by the largest and run at one wave per SIMD.  Within a launch, waves claim the
-// class's tiles from sliced counters and keep their 64 lanes busy with the points of those
-// tiles (lane refill).  Seeds (greedy or nearest leaf of each point's own surface) come from one
-// launch over all tiles before the class launches: found inside the refill loop they would be
-// paid by the whole wave at the refilling lanes' occupancy.  The FP32 bulk and the FP64 closing
-// pass are launches of the same template, as in the single-surface mixed mode.  Point arrays are addressed through a request
+// of tiles (object, contiguous point range, request) sorted by degree class.  The 17 degree
+// classes (exact unrolled contraction for degrees 2..5 in each direction, padded generic form
+// otherwise) fall into three groups by patch size (K1*K2 <= 16, <= 25, larger or generic); each
+// group present gets its own persistent launch of a kernel instantiated for that group alone, so
+// a launch has the register footprint of its group -- one kernel holding every class would be
+// sized by the largest and run at one wave per SIMD.  Within a launch, waves claim the group's
+// tiles from sliced counters and keep their 64 lanes busy with the points of those tiles (lane
+// refill); a wave holds points of one class at a time, so the class dispatch is wave-uniform,
+// and it drains its lanes before it starts a tile of another class.  Each group runs on its own
+// side stream: a seed launch over its tiles (greedy or nearest leaf of each point's own surface,
+// one lane per point: found inside the refill loop a seed would be paid by the whole wave at the
+// refilling lanes' occupancy), then the FP32 bulk and the FP64 closing pass, launches of the same
+// template as in the single-surface mixed mode.  Point arrays are addressed through a request
 // table (GnReq): a tile names its request, so one launch serves the points of many independent
 // requests (the serving loop) or of one call (a one-entry table).
 #include <algorithm>
@@ -88,8 +91,10 @@ __device__ __forceinline__ int step_group(int cls, const GnSurfDev &d, T Px, T P
 // Persistent lane-refill kernel over the tiles of one class group.  A wave's lanes always hold
 // points of one degree class (wave-uniform dispatch); when its next tile is of another class,
 // the wave drains its current points first.
+// No minimum occupancy is forced: the group-0 FP32 kernel (six classes behind one switch) needs
+// about 220 VGPRs, and holding it to three waves per SIMD spilled its patch registers to scratch.
 template <int GRP, typename T, bool NEWTON, int MODE>
-__global__ __launch_bounds__(kInvBlock, (GRP == 0 && sizeof(T) == 4) ? 3 : 1)
+__global__ __launch_bounds__(kInvBlock, 1)
 void k_batch(BatchArgs a, unsigned long long *counters, int slices, int max_iter,
              int restart_iter, double tol_dist, double tol_step, double tol_quad) {
     const int lane = threadIdx.x & 63;

@@ -245,14 +245,20 @@ __global__ __launch_bounds__(kInvBlock, (EXACT && sizeof(T) == 4 && K1 * K2 <= 1
         }
         if (__ballot(idx >= 0) == 0ull) break;
         // a step for every lane holding a point; with LDS-staged rows the whole wave calls it
-        // (idle lanes compute on stale state and discard the result)
+        // (lanes that take no step compute on copies of their parameters, inside the domain, and
+        // discard the result)
         int st = kStepContinue;
         const bool act = idx >= 0 && it < lim;
         if (PSRC == kPatchWave || act) {
+            double tu = act ? cu : dom.u0, tv = act ? cv : dom.v0;
             const int s2 = gn_step<K1, K2, EXACT, T, NEWTON, PSRC>(
-                d, ku, kv, Px, Py, Pz, cu, cv, tol_dist, tol_step, tol_quad, a.o.clamp, dom, sc,
+                d, ku, kv, Px, Py, Pz, tu, tv, tol_dist, tol_step, tol_quad, a.o.clamp, dom, sc,
                 pr.m, wave_lds);
-            if (act) st = s2;
+            if (act) {
+                st = s2;
+                cu = tu;
+                cv = tv;
+            }
         }
         if (idx >= 0) {
             if (!act) {
