@@ -350,3 +350,36 @@ def test_long_knot_vectors_take_the_global_knot_paths():
         gu, gv, st = (_np(t) for t in g.NewInverse(P[0], P[1], P[2], opts))
         err = np.maximum(np.abs(gu - u[:5000]), np.abs(gv - v[:5000]))
         assert np.mean(err <= 1e-9) > 0.97, prec
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed_tail0"])
+def test_zero_iteration_limit_returns_the_start_point(prec):
+    """A lane that may take no step keeps its parameters, also in the kernels whose whole wave
+    steps together (LDS-staged patch rows): max_iter = 0 returns the given seeds exactly, and a
+    mixed run with no FP64 closing steps returns its FP32 result."""
+    sd = synth.surface_G(32, 32, 3, 3, seed=1002)
+    g, o = _pair(sd)
+    rng = np.random.default_rng(12)
+    n = 5000
+    u0, v0 = rng.random(n), rng.random(n)
+    P = o.eval(u0, v0, 0)
+    su, sv = np.clip(u0 + 0.01, 0, 1), np.clip(v0 - 0.01, 0, 1)
+    if prec == "fp64":
+        opts = G.default_opts(precision=G.GN_FP64, newton=G.NEWTON_FP64, max_iter=0)
+        gu, gv, st = (_np(t) for t in g.NewInverse(P[0], P[1], P[2], opts, seeds=(su, sv)))
+        assert np.array_equal(gu, su) and np.array_equal(gv, sv)
+        assert np.all(st == G.PT_MAXITER)
+    else:
+        kw = dict(precision=G.GN_MIXED, newton=G.NEWTON_FP32 | G.NEWTON_FP64, fp32_iters=20,
+                  tol_step32=1e-4)
+        g32 = G.default_opts(fp64_tail=0, **kw)
+        gu, gv, st = (_np(t) for t in g.NewInverse(P[0], P[1], P[2], g32, seeds=(su, sv)))
+        full = G.default_opts(fp64_tail=3, tol_quad=1e-6, **kw)
+        fu, fv, _ = (_np(t) for t in g.NewInverse(P[0], P[1], P[2], full, seeds=(su, sv)))
+        err = np.maximum(np.abs(gu - u0), np.abs(gv - v0))
+        ferr = np.maximum(np.abs(fu - u0), np.abs(fv - v0))
+        # without closing steps the result keeps the FP32 bulk's error (a single FP64 step would
+        # take it to ~1e-12); with them it reaches the FP64 tolerance
+        assert np.percentile(err, 99) < 1e-3 and np.median(err) > 1e-10
+        assert np.median(ferr) < 1e-12
+        assert np.mean(st == G.PT_MAXITER) > 0.99  # points whose FP32 bulk broke down restart
