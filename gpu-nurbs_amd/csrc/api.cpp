@@ -202,6 +202,26 @@ int gn_surface_from_nu(const char *path, gn_stream stream, gn_surface **out) {
                              out);
 }
 
+// Writes the definition in the same tagged format (NU / CP / KNu / KNv / DP, u-major control
+// points), 17 significant digits, so gn_surface_from_nu reads back the identical surface.
+int gn_surface_to_nu(const gn_surface *s, const char *path) {
+    if (!s || !path) return fail(GN_EINVAL, "null argument");
+    FILE *f = std::fopen(path, "w");
+    if (!f) return fail(GN_EIO, std::string("cannot open ") + path + " for writing");
+    std::fprintf(f, "** NURBS degree\nNU %d %d\nCP %d %d\n\n** NURBS Knots\nKNu", s->p, s->q, s->nu,
+                 s->nv);
+    for (double k : s->ku) std::fprintf(f, " %.17g", k);
+    std::fprintf(f, "\nKNv");
+    for (double k : s->kv) std::fprintf(f, " %.17g", k);
+    std::fprintf(f, "\n\n** control points and weights\n");
+    for (size_t i = 0; i < s->w.size(); i++)
+        std::fprintf(f, "DP %.17g %.17g %.17g %.17g\n", s->xyz[3 * i], s->xyz[3 * i + 1],
+                     s->xyz[3 * i + 2], s->w[i]);
+    const bool ok = std::ferror(f) == 0;
+    if (std::fclose(f) != 0 || !ok) return fail(GN_EIO, std::string("write failed: ") + path);
+    return GN_OK;
+}
+
 int gn_surface_destroy(gn_surface *s) {
     if (!s) return GN_OK;
     (void)hipDeviceSynchronize();

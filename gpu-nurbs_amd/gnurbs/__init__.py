@@ -67,6 +67,7 @@ _SIGS = [
     ("gn_invert_opts_default", None, [ctypes.POINTER(InvertOpts)]),
     ("gn_surface_create", _c_int, [_c_int, _c_int, _c_int, _c_int, _c_void, _c_void, _c_void, _c_void, _c_void, ctypes.POINTER(_c_void)]),
     ("gn_surface_from_nu", _c_int, [ctypes.c_char_p, _c_void, ctypes.POINTER(_c_void)]),
+    ("gn_surface_to_nu", _c_int, [_c_void, ctypes.c_char_p]),
     ("gn_surface_destroy", _c_int, [_c_void]),
     ("gn_surface_info", _c_int, [_c_void, ctypes.POINTER(_c_int), ctypes.POINTER(_c_int), ctypes.POINTER(_c_int), ctypes.POINTER(_c_int)]),
     ("gn_surface_device_bytes", _c_size, [_c_void]),
@@ -81,6 +82,8 @@ _SIGS = [
     ("gn_seed", _c_int, [_c_void, _c_int, _c_void, _c_void, _c_void, _c_size, _c_void, _c_void, _c_void, _c_void]),
     ("gn_invert", _c_int, [_c_void, _c_void, _c_void, _c_void, _c_size, ctypes.POINTER(InvertOpts), _c_void, _c_void, _c_void, _c_void]),
     ("gn_invert_iters", _c_int, [_c_void, _c_void, _c_void, _c_void, _c_size, ctypes.POINTER(InvertOpts), _c_void, _c_void, _c_void, _c_void, _c_void, _c_void]),
+    ("gn_pcurve", _c_int, [_c_void, _c_void, _c_size, _c_int, _c_int, ctypes.POINTER(InvertOpts), _c_void, _c_void,
+                           ctypes.POINTER(_c_double), ctypes.POINTER(_c_size), _c_void]),
     ("gn_batch_create", _c_int, [_c_void, _c_int, _c_void, ctypes.POINTER(_c_void)]),
     ("gn_batch_destroy", _c_int, [_c_void]),
     ("gn_batch_device_bytes", _c_size, [_c_void]),
@@ -222,6 +225,10 @@ class Surface:
         _check(lib().gn_surface_from_nu(path.encode(), _c_void(stream_handle(stream)), ctypes.byref(h)))
         return cls(0, 0, None, None, None, None, _handle=h)
 
+    def writeNu(self, path: str):
+        """Writes the definition as a .nu file that readNu reads back identically."""
+        _check(lib().gn_surface_to_nu(self._h, path.encode()))
+
     @property
     def handle(self):
         return self._h
@@ -338,6 +345,21 @@ class Surface:
         return u, v, st, it32, it64
 
     Invert = NewInverse
+
+    def PCurve(self, curve, samples: int, degree: int = 3, ncp: int = 16, opts: Optional[InvertOpts] = None,
+               stream=None):
+        """p-curve of `curve` on this surface (gn_pcurve): the clamped uniform B-spline (u(t), v(t))
+        fitted to the inversions of `samples` curve points.  Returns (control points (ncp, 2),
+        knots, largest 3-D deviation over the samples, samples that did not converge)."""
+        import numpy as np
+
+        opts = opts if opts is not None else default_opts()
+        cp = np.zeros(2 * ncp)
+        kn = np.zeros(ncp + degree + 1)
+        dev, nf = _c_double(), _c_size()
+        _check(lib().gn_pcurve(self._h, curve.handle, samples, degree, ncp, ctypes.byref(opts), cp.ctypes.data,
+                               kn.ctypes.data, ctypes.byref(dev), ctypes.byref(nf), _c_void(stream_handle(stream))))
+        return cp.reshape(ncp, 2), kn, dev.value, nf.value
 
 
 class Curve:

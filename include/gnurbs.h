@@ -107,6 +107,9 @@ int gn_surface_create(int degree_u, int degree_v, int cp_u, int cp_v, const doub
                       gn_stream stream, gn_surface **out);
 /* readNu (src/main.cu:45-113): NU/CP/KNu/KNv/DP text format. */
 int gn_surface_from_nu(const char *path, gn_stream stream, gn_surface **out);
+/* The writer the reference lacks (its models are read-only inputs, src/main.cu:45-113): the same
+ * tagged format, 17 significant digits, so gn_surface_from_nu reads back the identical surface. */
+int gn_surface_to_nu(const gn_surface *s, const char *path);
 int gn_surface_destroy(gn_surface *s);
 int gn_surface_info(const gn_surface *s, int *degree_u, int *degree_v, int *cp_u, int *cp_v);
 /* HBM bytes held by the object (span tables, knots, tree). */
@@ -228,6 +231,20 @@ int gn_curve_evaluate(const gn_curve *c, const double *t, size_t n, int max_orde
  * seeds from 8 samples per span (or GN_SEED_GIVEN: seeds in t_out).  GN_FP64 or GN_MIXED. */
 int gn_curve_invert(const gn_curve *c, const double *x, const double *y, const double *z, size_t n,
                     const gn_invert_opts *opts, double *t_out, uint8_t *status, gn_stream stream);
+
+/* ---- p-curves ---- */
+/* The p-curve of curve c on surface s: the use README.md:10 names (inverting a 3-D curve onto a
+ * surface and fitting its 2-D pre-image), with no code in the reference.  Samples c at `samples`
+ * uniformly spaced parameters over its knot domain, inverts them onto s with opts (the tree must
+ * be built unless opts->seed_mode is GN_SEED_GIVEN, which is rejected here: there are no seeds),
+ * and fits by least squares a clamped uniform B-spline (u(t), v(t)) of `degree` (1..5) with `ncp`
+ * control points over the same domain to the samples that converged.  Host outputs:
+ * cp_uv[2*ncp] (u, v per control point), knots[ncp+degree+1]; optional max_dev (largest distance
+ * between S(u(t_i), v(t_i)) and c(t_i) over all samples) and failed (samples that did not
+ * converge).  Synchronous; allocates its scratch (about 90 bytes per sample). */
+int gn_pcurve(const gn_surface *s, const gn_curve *c, size_t samples, int degree, int ncp,
+              const gn_invert_opts *opts, double *cp_uv, double *knots, double *max_dev,
+              size_t *failed, gn_stream stream);
 
 /* ---- serving: continuous batching of inversion requests ------------------------------------ */
 /* No counterpart in the reference (its driver inverts one fixed batch, src/main.cu:277-327).

@@ -58,6 +58,8 @@ class NurbsSurface {
         check(gn_surface_from_nu(path.c_str(), stream, &s));
         return NurbsSurface(s);
     }
+    // the same text format, read back identically by readNu
+    void writeNu(const std::string &path) const { check(gn_surface_to_nu(s_, path.c_str())); }
     NurbsSurface(const NurbsSurface &) = delete;
     NurbsSurface &operator=(const NurbsSurface &) = delete;
     NurbsSurface(NurbsSurface &&o) noexcept { *this = std::move(o); }
