@@ -373,8 +373,10 @@ int gn_invert_iters(const gn_surface *s, const double *x, const double *y, const
     if (n && (!x || !y || !z || !u || !v || !status)) return fail(GN_EINVAL, "null buffer");
     if (o.precision != GN_FP64 && o.precision != GN_MIXED)
         return fail(GN_EINVAL, "precision must be GN_FP64 or GN_MIXED");
-    if (o.seed_mode < GN_SEED_NEAREST_LEAF || o.seed_mode > GN_SEED_GREEDY_LEAF)
+    if (o.seed_mode < GN_SEED_NEAREST_LEAF || o.seed_mode > GN_SEED_GLOBAL)
         return fail(GN_EINVAL, "bad seed_mode");
+    if (o.seed_mode == GN_SEED_GLOBAL && o.precision != GN_FP64)
+        return fail(GN_EINVAL, "global projection runs in FP64 (precision GN_FP64)");
     if (o.max_iter < 0 || o.fp32_iters < 0 || o.fp64_tail < 0 || o.max_iter > 10000 ||
         o.fp32_iters > 10000 || o.fp64_tail > 10000)
         return fail(GN_EINVAL, "iteration counts must be 0..10000");

@@ -18,6 +18,7 @@ int launch_invert(const GnSurfDev &d, const GnSearchDev &sd, const GnTreeNode *t
         for (int l = 0; l < tree_levels; l++) { count += g * g; g *= 8; }
         return launch_gn_first_hit(d, a, tree, valid, count, st);
     }
+    if (o.seed_mode == GN_SEED_GLOBAL) return launch_project_global(d, sd, a, st);
     if (o.seed_mode == GN_SEED_NEAREST_LEAF || o.seed_mode == GN_SEED_GREEDY_LEAF) {
         int rc = launch_seed(sd, o.seed_mode == GN_SEED_NEAREST_LEAF ? 0 : 1, x, y, z, n, nullptr,
                              u, v, st);

@@ -86,18 +86,24 @@ __device__ __forceinline__ Domain domain_of(const GnSurfDev &d) {
     return Domain{d.ku[d.p], d.ku[d.nu], d.kv[d.q], d.kv[d.nv]};
 }
 
-template <int K1, int K2, bool EXACT, typename T, bool NEWTON, int PSRC = kPatchGlobal>
+// ACTIVE: projected Newton on the clamp rectangle -- a coordinate on its bound whose descent
+// direction points outward is held and the other takes its one-dimensional step, with the exact
+// one-dimensional second derivative where that is positive (a plain clamp of the coupled step can
+// walk uphill along an edge).  rr_out: the squared residual at the point evaluated.
+template <int K1, int K2, bool EXACT, typename T, bool NEWTON, int PSRC = kPatchGlobal,
+          bool ACTIVE = false>
 __device__ __forceinline__ int gn_step(const GnSurfDev &d, const double *__restrict__ ku,
                                        const double *__restrict__ kv, T Px, T Py, T Pz,
                                        double &cu, double &cv, double tol_dist, double tol_step,
                                        double tol_quad, int clamp, const Domain &dom,
                                        SpanCache &sc, Hom<T> *pr = nullptr,
-                                       float4 *wave_lds = nullptr) {
+                                       float4 *wave_lds = nullptr, T *rr_out = nullptr) {
     Cart<T> o;
     eval_point_cached<K1, K2, NEWTON ? 2 : 1, EXACT, T, PSRC>(d, ku, kv, cu, cv, sc, o, pr,
                                                               wave_lds);
     const T r0 = o.S[0] - Px, r1 = o.S[1] - Py, r2 = o.S[2] - Pz;
     const T rr = r0 * r0 + r1 * r1 + r2 * r2;
+    if (rr_out) *rr_out = rr;
     if (!(rr == rr)) return GN_PT_NONFINITE;
     if (tol_dist > 0 && sqrt(rr) < T(tol_dist)) return GN_PT_CONVERGED_DIST;
     T a = o.Su[0] * o.Su[0] + o.Su[1] * o.Su[1] + o.Su[2] * o.Su[2];
@@ -105,18 +111,34 @@ __device__ __forceinline__ int gn_step(const GnSurfDev &d, const double *__restr
     T c = o.Sv[0] * o.Sv[0] + o.Sv[1] * o.Sv[1] + o.Sv[2] * o.Sv[2];
     const T gu = o.Su[0] * r0 + o.Su[1] * r1 + o.Su[2] * r2;
     const T gv = o.Sv[0] * r0 + o.Sv[1] * r1 + o.Sv[2] * r2;
+    T a1 = a, c1 = c;  // one-dimensional curvatures for a held coordinate (ACTIVE)
     if (NEWTON) {
         const T a2 = a + r0 * o.Suu[0] + r1 * o.Suu[1] + r2 * o.Suu[2];
         const T b2 = b + r0 * o.Suv[0] + r1 * o.Suv[1] + r2 * o.Suv[2];
         const T c2 = c + r0 * o.Svv[0] + r1 * o.Svv[1] + r2 * o.Svv[2];
+        if (ACTIVE) {
+            if (a2 > T(0)) a1 = a2;
+            if (c2 > T(0)) c1 = c2;
+        }
         if (a2 > T(0) && a2 * c2 - b2 * b2 > T(0)) { a = a2; b = b2; c = c2; }
     }
     const T det = a * c - b * b;
     if (!(det == det)) return GN_PT_NONFINITE;
     if (det == T(0)) return GN_PT_SINGULAR;
     const T idet = recip(det);
-    const T du = (c * gu - b * gv) * idet;
-    const T dv = (a * gv - b * gu) * idet;
+    T du = (c * gu - b * gv) * idet;
+    T dv = (a * gv - b * gu) * idet;
+    if (ACTIVE) {
+        const bool hu = (cu <= dom.u0 && gu > T(0)) || (cu >= dom.u1 && gu < T(0));
+        const bool hv = (cv <= dom.v0 && gv > T(0)) || (cv >= dom.v1 && gv < T(0));
+        if (hu) {
+            du = T(0);
+            dv = hv ? T(0) : gv / c1;
+        } else if (hv) {
+            dv = T(0);
+            du = gu / a1;
+        }
+    }
     cu -= double(du);
     cv -= double(dv);
     if (clamp) {
@@ -303,10 +325,13 @@ inline void refill_shape(const void *kernel, size_t n, unsigned &grid, int &slic
     if (slices > kMaxSlices) slices = kMaxSlices;
 }
 
-// launchers, one translation unit each (invert_fp64.hip, invert_mixed.hip, invert_ref.hip)
+// launchers, one translation unit each (invert_fp64.hip, invert_mixed.hip, invert_ref.hip,
+// project.hip)
 int launch_gn_plain(const GnSurfDev &d, const InvArgs &a, hipStream_t st);
 int launch_gn_mixed(const GnSurfDev &d, const InvArgs &a, hipStream_t st);
 int launch_gn_first_hit(const GnSurfDev &d, const InvArgs &a, const GnTreeNode *tree,
                         const uint8_t *valid, long count, hipStream_t st);
+int launch_project_global(const GnSurfDev &d, const GnSearchDev &sd, const InvArgs &a,
+                          hipStream_t st);  // project.hip
 
 }  // namespace gn

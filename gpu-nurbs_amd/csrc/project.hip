@@ -1,0 +1,155 @@
+// project.hip -- global projection (GN_SEED_GLOBAL): the nearest point of the surface to each
+// query, not just the local minimum that Newton reaches from one seed.  The reference's version
+// (normal-cone culling, src/kernel/kernel.cu:471-513, 623-704) is unfinished; this one is a branch
+// and bound over the seed quadtree whose leaf boxes contain their pieces of surface:
+//   1. bound: exact nearest leaf centre, then projected Newton over the whole domain;
+//   2. every leaf whose FP32 box (a conservative lower bound) is not farther than the best foot
+//      point so far gets a projected Newton solve clamped to its uv cell, from the cell centre;
+//      a nearer result tightens the bound, which prunes the rest of the depth-first walk;
+//   3. the nearest foot point is polished by projected Newton over the whole domain (kept unless it
+//      is farther by more than rounding, 1e-9 relative in squared distance).
+// The minimum is found whenever the solve inside the leaf that holds it reaches it: leaves of the
+// depth-3 tree (64 x 64 cells) are nearly flat, so the distance restricted to one is close to
+// convex.  Projected Newton (gn_step with ACTIVE) holds a coordinate on its bound while the
+// descent points outward, so minima on cell edges and on the surface boundary are reached too.
+// FP64, one lane per query, exact-Hessian steps.
+#include "gn_search.hpp"
+#include "invert_common.hpp"
+
+namespace gn {
+
+constexpr int kLeafIter = 8;  // projected Newton steps inside one leaf cell
+
+// Projected Newton from (cu, cv) inside dom, with backtracking: a step after which the squared
+// distance grew by more than 1e-6 relative is halved, up to kHalvings times in a row, before a new
+// step is taken.  (Near a minimum the squared distance carries rounding of the surface point
+// relative to the residual, ~1e-12 of it at a 1e-3 offset; smaller growth is not a failed step.)  Returns the status
+// and leaves in (cu, cv), d2 the nearest point evaluated and its squared distance.
+constexpr int kHalvings = 8;
+template <int K1, int K2, bool EXACT>
+__device__ __noinline__ int solve_in(const GnSurfDev &d, const double *P, double &cu, double &cv,
+                                     const Domain &dom, int iters, double tol_step, double &d2) {
+    SpanCache sc;
+    int st = GN_PT_MAXITER, back = 0;
+    double pu = cu, pv = cv, prr = INFINITY;  // the last accepted point
+    for (int it = 0; it < iters; it++) {
+        const double su = cu, sv = cv;
+        double rr = INFINITY;
+        const int s = gn_step<K1, K2, EXACT, double, true, kPatchGlobal, true>(
+            d, d.ku, d.kv, P[0], P[1], P[2], cu, cv, 0.0, tol_step, 0.0, 1, dom, sc, nullptr,
+            nullptr, &rr);
+        if (!(rr <= prr * (1.0 + 1e-6)) && back < kHalvings) {  // grew, beyond rounding
+            cu = pu + 0.5 * (su - pu);
+            cv = pv + 0.5 * (sv - pv);
+            back++;
+            continue;
+        }
+        back = 0;
+        pu = su;
+        pv = sv;
+        prr = rr;
+        if (s != kStepContinue) {
+            st = s;
+            break;
+        }
+    }
+    Cart<double> o;
+    eval_point<K1, K2, 0, EXACT, double>(d, d.ku, d.kv, cu, cv, o);
+    const double dx = o.S[0] - P[0], dy = o.S[1] - P[1], dz = o.S[2] - P[2];
+    d2 = fma(dz, dz, fma(dy, dy, dx * dx));
+    if (!(d2 <= prr)) {  // the final step did not help: keep the last accepted point
+        cu = pu;
+        cv = pv;
+        d2 = prr;
+    }
+    return st;
+}
+
+template <int K1, int K2, bool EXACT>
+__global__ __launch_bounds__(256) void k_project_global(GnSurfDev d, GnSearchDev sd, InvArgs a) {
+    const Domain full = domain_of(d);
+    const int D = sd.depth;
+    const double cell = 1.0 / (double)(1u << D);  // leaf cells of [0,1]^2 (the tree's uv layout)
+    const int max_iter = a.o.max_iter;
+    const double tol_step = a.o.tol_step;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < a.n;
+         i += (size_t)gridDim.x * blockDim.x) {
+        const double P[3] = {a.x[i], a.y[i], a.z[i]};
+        const uint32_t c0 = nearest_leaf(sd, P);
+        double bu = sd.seed_uv[2 * c0], bv = sd.seed_uv[2 * c0 + 1], best;
+        int bst = solve_in<K1, K2, EXACT>(d, P, bu, bv, full, max_iter, tol_step, best);
+        int ncand = 1;
+        if (D > 0) {
+            const float p[3] = {(float)P[0], (float)P[1], (float)P[2]};
+            // stackless depth-first walk (Morton-code successor), pruned by the current best
+            int l = 1;
+            uint32_t c = 0;
+            while (true) {
+                const double lb = (double)box_d2(sd.box32 + (qoff(l) + c) * 6, p);
+                bool descend = false;
+                if (lb <= best) {
+                    if (l < D) {
+                        descend = true;
+                    } else if (c != c0) {
+                        const uint32_t ix = compact2(c >> 1), iy = compact2(c);
+                        const Domain cr{ix * cell, (ix + 1) * cell, iy * cell, (iy + 1) * cell};
+                        double cu = sd.seed_uv[2 * c], cv = sd.seed_uv[2 * c + 1], d2;
+                        const int s2 = solve_in<K1, K2, EXACT>(d, P, cu, cv, cr, kLeafIter,
+                                                               tol_step, d2);
+                        ncand++;
+                        if (d2 < best) {
+                            best = d2;
+                            bu = cu;
+                            bv = cv;
+                            bst = s2;
+                        }
+                    }
+                }
+                if (descend) {
+                    c <<= 2;
+                    l++;
+                    continue;
+                }
+                while (l > 1 && (c & 3u) == 3u) {
+                    c >>= 2;
+                    l--;
+                }
+                if (l == 1 && (c & 3u) == 3u) break;
+                c++;
+            }
+        }
+        double pu = bu, pv = bv, pd2;
+        const int pst = solve_in<K1, K2, EXACT>(d, P, pu, pv, full, max_iter, tol_step, pd2);
+        if (pd2 <= best * (1.0 + 1e-9)) {  // not farther beyond rounding: the polished point
+            bu = pu;
+            bv = pv;
+            bst = pst;
+        }
+        a.u[i] = bu;
+        a.v[i] = bv;
+        a.status[i] = (uint8_t)bst;
+        if (a.it32) a.it32[i] = sat8(ncand);
+        if (a.it64) a.it64[i] = 0;
+    }
+}
+
+struct ProjectLaunch {
+    const GnSurfDev &d;
+    const GnSearchDev &sd;
+    const InvArgs &a;
+    hipStream_t st;
+    template <int K1, int K2, bool EXACT>
+    int operator()() {
+        k_project_global<K1, K2, EXACT><<<grid_for(a.n, 256, 256 * 16), 256, 0, st>>>(d, sd, a);
+        GN_HIP(hipGetLastError());
+        return GN_OK;
+    }
+};
+
+int launch_project_global(const GnSurfDev &d, const GnSearchDev &sd, const InvArgs &a,
+                          hipStream_t st) {
+    if (a.n == 0) return GN_OK;
+    return dispatch_degrees(d.p, d.q, ProjectLaunch{d, sd, a, st});
+}
+
+}  // namespace gn

@@ -27,7 +27,7 @@ LIB_PATH = os.path.join(_HERE, "libgnurbs.so")
 GN_OK, GN_EINVAL, GN_ENOMEM, GN_EDEVICE, GN_EIO, GN_ESTATE, GN_EBUDGET, GN_ENOTFOUND, GN_EAGAIN = range(9)
 GN_FP64, GN_FP32, GN_MIXED = 0, 1, 2
 PT_CONVERGED, PT_CONVERGED_DIST, PT_MAXITER, PT_SINGULAR, PT_NONFINITE, PT_NOSEED = range(6)
-SEED_NEAREST_LEAF, SEED_REF_FIRST_HIT, SEED_GIVEN, SEED_GREEDY_LEAF = 0, 1, 2, 3
+SEED_NEAREST_LEAF, SEED_REF_FIRST_HIT, SEED_GIVEN, SEED_GREEDY_LEAF, SEED_GLOBAL = 0, 1, 2, 3, 4
 TREE_REF_PARENT = 1
 NEWTON_FP32, NEWTON_FP64 = 1, 2
 

@@ -60,7 +60,13 @@ typedef enum {
     GN_SEED_NEAREST_LEAF = 0, /* leaf of the tree whose centre point is nearest to P       */
     GN_SEED_REF_FIRST_HIT = 1,/* reference NewInverse: index-order scan, first box hit     */
     GN_SEED_GIVEN = 2,        /* caller supplies seeds in u_out / v_out                    */
-    GN_SEED_GREEDY_LEAF = 3   /* greedy box-distance descent, then the nearest leaf centre */
+    GN_SEED_GREEDY_LEAF = 3,  /* greedy box-distance descent, then the nearest leaf centre */
+    GN_SEED_GLOBAL = 4        /* global projection (FP64): Newton from every leaf whose box
+                                 could hold a nearer point (branch and bound over the seed tree),
+                                 each clamped to its leaf cell, then polished; the nearest foot
+                                 point wins.  iters_fp32 of gn_invert_iters receives the number
+                                 of leaves solved.  The reference's normal-cone projection
+                                 (src/kernel/kernel.cu:471-513, 623-704) is unfinished. */
                               /* among the 3 x 3 leaves around it (fixed work per point)   */
 } gn_seed_mode;
 

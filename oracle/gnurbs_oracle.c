@@ -29,6 +29,8 @@
  *   or_gauss_newton     round trips on on-surface points, orthogonality of the residual
  *   or_build_tree       containment of dense samples, index formula, NeedDivide truth table
  *   or_seed_*           brute force
+ *   or_project_global   never farther than any densely sampled surface point; never farther than
+ *                       the local method; GPU branch and bound equals this exhaustive form
  *   or_curve_*          (no reference counterpart) exact circle arcs, scipy BSpline with textbook
  *                       derivative control points, finite differences, closed-form foot points
  *                       on the circle, brute-force nearest sample (tests/test_oracle_curve.py)
@@ -424,20 +426,41 @@ typedef struct {
     int max_iter;    /* reference: 10 */
     int clamp;       /* clamp (u,v) to the knot domain after each update */
     int newton;      /* 1: add the residual-curvature term to J^T J (exact Hessian) */
+    int active;      /* 1: projected Newton -- a coordinate on its clamp bound whose descent
+                        points outward is held and the other takes its one-dimensional step
+                        (exact curvature where positive); a step after which the squared
+                        distance grew by more than 1e-6 relative is halved (up to 8 times in a
+                        row); the nearest accepted point is kept */
 } or_gn_opts;
 
 /* Returns status; (u,v) is updated in place; *iters gets the number of iterations begun. */
-static int gn_one(const or_surf *sf, const double P[3], const or_gn_opts *op, double *pu, double *pv,
-                  int *iters)
+/* dom (optional): clamp rectangle {u0, u1, v0, v1} instead of the knot domain */
+static int gn_one_in(const or_surf *sf, const double P[3], const or_gn_opts *op, double *pu,
+                     double *pv, int *iters, const double *dom)
 {
     double cu = *pu, cv = *pv;
     double ulo = sf->ku[sf->p], uhi = sf->ku[sf->nu], vlo = sf->kv[sf->q], vhi = sf->kv[sf->nv];
-    int st = OR_MAXITER, it;
+    if (dom) { ulo = dom[0]; uhi = dom[1]; vlo = dom[2]; vhi = dom[3]; }
+    int st = OR_MAXITER, it, back = 0;
+    double au = cu, av = cv, prr = INFINITY; /* active mode: the last accepted point */
     for (it = 0; it < op->max_iter; it++) {
         or_pt o;
         eval_one(sf, cu, cv, op->newton ? 2 : 1, &o);
         double d[3] = {o.S[0] - P[0], o.S[1] - P[1], o.S[2] - P[2]};
-        double dist = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+        double rr = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+        double dist = sqrt(rr);
+        if (op->active) { /* backtracking: a step after which the distance grew is halved */
+            if (!(rr <= prr * (1.0 + 1e-6)) && back < 8) { /* grew, beyond rounding */
+                cu = au + 0.5 * (cu - au);
+                cv = av + 0.5 * (cv - av);
+                back++;
+                continue;
+            }
+            back = 0;
+            au = cu;
+            av = cv;
+            prr = rr;
+        }
         if (!(dist == dist)) { st = OR_NONFINITE; break; }
         if (dist < op->tol_dist) { st = OR_CONV_DIST; break; }
         /* J = [Su Sv]; (J^T J)^-1 J^T (S - P), src/kernel/kernel.cu:195-239 */
@@ -449,6 +472,7 @@ static int gn_one(const or_surf *sf, const double P[3], const or_gn_opts *op, do
             gu += o.Su[k] * d[k];
             gv += o.Sv[k] * d[k];
         }
+        double a1 = a, c1 = c; /* one-dimensional curvatures for a held coordinate */
         if (op->newton) {
             double a2 = a, b2 = b, c2 = c;
             for (int k = 0; k < 3; k++) {
@@ -456,6 +480,8 @@ static int gn_one(const or_surf *sf, const double P[3], const or_gn_opts *op, do
                 b2 += d[k] * o.Suv[k];
                 c2 += d[k] * o.Svv[k];
             }
+            if (a2 > 0) a1 = a2;
+            if (c2 > 0) c1 = c2;
             if (a2 > 0 && a2 * c2 - b2 * b2 > 0) { a = a2; b = b2; c = c2; }
         }
         double det = a * c - b * b;
@@ -463,6 +489,12 @@ static int gn_one(const or_surf *sf, const double P[3], const or_gn_opts *op, do
         if (det == 0) { st = OR_SINGULAR; break; }
         double du = (c * gu - b * gv) / det;
         double dv = (a * gv - b * gu) / det;
+        if (op->active) {
+            int hu = (cu <= ulo && gu > 0) || (cu >= uhi && gu < 0);
+            int hv = (cv <= vlo && gv > 0) || (cv >= vhi && gv < 0);
+            if (hu) { du = 0; dv = hv ? 0 : gv / c1; }
+            else if (hv) { dv = 0; du = gu / a1; }
+        }
         cu -= du;
         cv -= dv;
         if (op->clamp) {
@@ -476,10 +508,22 @@ static int gn_one(const or_surf *sf, const double P[3], const or_gn_opts *op, do
             break;
         }
     }
+    if (op->active) { /* the final step did not help: keep the last accepted point */
+        or_pt o;
+        eval_one(sf, cu, cv, 0, &o);
+        double dx = o.S[0] - P[0], dy = o.S[1] - P[1], dz = o.S[2] - P[2];
+        if (!(dx * dx + dy * dy + dz * dz <= prr)) { cu = au; cv = av; }
+    }
     *pu = cu;
     *pv = cv;
     if (iters) *iters = it;
     return st;
+}
+
+static int gn_one(const or_surf *sf, const double P[3], const or_gn_opts *op, double *pu, double *pv,
+                  int *iters)
+{
+    return gn_one_in(sf, P, op, pu, pv, iters, NULL);
 }
 
 /* Batch Gauss-Newton from given seeds.  xyz SoA (x[n], y[n], z[n]). */
@@ -490,7 +534,7 @@ void or_gauss_newton(int p, int q, int nu, int nv, const double *ku, const doubl
                      unsigned char *status, int *iters, int nthreads)
 {
     or_surf sf = {p, q, nu, nv, ku, kv, MatrixN};
-    or_gn_opts op = {tol_dist, tol_step, max_iter, clamp, newton};
+    or_gn_opts op = {tol_dist, tol_step, max_iter, clamp, newton, 0};
 #ifdef _OPENMP
     if (nthreads <= 0) nthreads = omp_get_max_threads();
 #pragma omp parallel for schedule(dynamic, 256) num_threads(nthreads)
@@ -628,7 +672,7 @@ void or_invert_first_hit(int p, int q, int nu, int nv, const double *ku, const d
                          int nthreads)
 {
     or_surf sf = {p, q, nu, nv, ku, kv, MatrixN};
-    or_gn_opts op = {tol_dist, tol_step, max_iter, clamp, 0};
+    or_gn_opts op = {tol_dist, tol_step, max_iter, clamp, 0, 0};
 #ifdef _OPENMP
     if (nthreads <= 0) nthreads = omp_get_max_threads();
 #pragma omp parallel for schedule(dynamic, 64) num_threads(nthreads)
@@ -795,6 +839,70 @@ void or_seed_greedy_leaf(const double *nodes, int levels, const double *x, const
 /* Gauss-Newton / Newton on f(t) = C'(t).(C(t) - P).  It shares no code with the matrix form of  */
 /* the GPU path.  Pinned by: exact circle arcs (w = cos(pi/4) quarter arcs), scipy BSpline for   */
 /* the non-rational value and derivatives, finite differences, round trips.                     */
+/* Global projection (the library's GN_SEED_GLOBAL; the reference's normal-cone projection,
+ * src/kernel/kernel.cu:471-513, 623-704, is unfinished).  Exhaustive form of the library's
+ * branch and bound: exact-Hessian projected Newton (a coordinate held on its bound while the
+ * descent points outward) from the nearest leaf centre over the whole domain, then
+ * from the centre of EVERY leaf with the iterate clamped to that leaf's uv cell (leaf_iter
+ * steps), the nearest foot point kept (ties to the earlier candidate), then a full-domain Newton
+ * polish from it that is kept unless it is farther by more than rounding (1e-9 relative).  The tree is the reference's (uv cells of
+ * [0,1]^2, nodes as or_build_tree); leaves in Morton order of their cell (x bits odd). */
+static double dist2_at(const or_surf *sf, const double P[3], double u, double v)
+{
+    or_pt o;
+    eval_one(sf, u, v, 0, &o);
+    double dx = o.S[0] - P[0], dy = o.S[1] - P[1], dz = o.S[2] - P[2];
+    return dx * dx + dy * dy + dz * dz;
+}
+
+void or_project_global(int p, int q, int nu, int nv, const double *ku, const double *kv,
+                       const double *MatrixN, const double *nodes, int levels, const double *x,
+                       const double *y, const double *z, long n, int max_iter, int leaf_iter,
+                       double tol_step, double *u_out, double *v_out, unsigned char *status,
+                       int nthreads)
+{
+    or_surf sf = {p, q, nu, nv, ku, kv, MatrixN};
+    long base = or_tree_base(levels - 1), side = 1L << (3 * (levels - 1));
+    or_gn_opts full = {0.0, tol_step, max_iter, 1, 1, 1}, leaf = {0.0, tol_step, leaf_iter, 1, 1, 1};
+#ifdef _OPENMP
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 16) num_threads(nthreads)
+#else
+    (void)nthreads;
+#endif
+    for (long i = 0; i < n; i++) {
+        double P[3] = {x[i], y[i], z[i]};
+        /* nearest leaf centre (ties to the lower index) */
+        long bl = 0;
+        double bd = INFINITY;
+        for (long t = 0; t < side * side; t++) {
+            const double *nd = nodes + (base + t) * 13;
+            double dx = P[0] - nd[10], dy = P[1] - nd[11], dz = P[2] - nd[12];
+            double d2 = dx * dx + dy * dy + dz * dz;
+            if (d2 < bd) { bd = d2; bl = t; }
+        }
+        const double *nd0 = nodes + (base + bl) * 13;
+        double bu = (nd0[0] + nd0[2]) / 2, bv = (nd0[1] + nd0[3]) / 2;
+        int bst = gn_one_in(&sf, P, &full, &bu, &bv, NULL, NULL);
+        double best = dist2_at(&sf, P, bu, bv);
+        for (long t = 0; t < side * side; t++) {
+            if (t == bl) continue;
+            const double *nd = nodes + (base + t) * 13;
+            double dom[4] = {nd[0], nd[2], nd[1], nd[3]};
+            double cu = (nd[0] + nd[2]) / 2, cv = (nd[1] + nd[3]) / 2;
+            int st = gn_one_in(&sf, P, &leaf, &cu, &cv, NULL, dom);
+            double d2 = dist2_at(&sf, P, cu, cv);
+            if (d2 < best) { best = d2; bu = cu; bv = cv; bst = st; }
+        }
+        double pu = bu, pv = bv;
+        int pst = gn_one_in(&sf, P, &full, &pu, &pv, NULL, NULL);
+        if (dist2_at(&sf, P, pu, pv) <= best * (1.0 + 1e-9)) { bu = pu; bv = pv; bst = pst; }
+        u_out[i] = bu;
+        v_out[i] = bv;
+        status[i] = (unsigned char)bst;
+    }
+}
+
 /* ------------------------------------------------------------------------------------------ */
 #define OR_CURVE_SAMPLES 8
 

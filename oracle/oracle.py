@@ -50,6 +50,8 @@ def lib():
         L.or_invert_first_hit.argtypes = [_I, _I, _I, _I, _P, _P, _P, _P, _P, _L, _P, _P, _P, _L,
                                           _D, _D, _I, _I, _P, _P, _P, _P, _I]
         L.or_seed_nearest_leaf.argtypes = [_P, _I, _P, _P, _P, _L, _P, _P, _P]
+        L.or_project_global.argtypes = [_I, _I, _I, _I, _P, _P, _P, _P, _I, _P, _P, _P, _L, _I, _I, _D,
+                                        _P, _P, _P, _I]
         L.or_seed_greedy_leaf.argtypes = [_P, _I, _P, _P, _P, _L, _P, _P, _P, _I]
         L.or_curve_eval.argtypes = [_I, _I, _P, _P, _P, _P, _L, _I, _P, _I]
         L.or_curve_seed.argtypes = [_I, _I, _P, _P, _P, _P, _P, _P, _L, _P]
@@ -133,6 +135,19 @@ class OSurf:
                               max_iter, clamp, newton, su.ctypes.data, sv.ctypes.data, u.ctypes.data,
                               v.ctypes.data, st.ctypes.data, it.ctypes.data, nthreads)
         return u, v, st, it
+
+    def project_global(self, nodes, levels, x, y, z, max_iter=20, leaf_iter=8, tol_step=1e-12, nthreads=0):
+        """Exhaustive global projection: Newton from every leaf (clamped to its uv cell), best kept,
+        then polished over the whole domain."""
+        x, y, z = (_a(a) for a in (x, y, z))
+        nodes = _a(nodes)
+        n = x.size
+        u, v = np.zeros(n), np.zeros(n)
+        st = np.zeros(n, dtype=np.uint8)
+        lib().or_project_global(*self._args(), nodes.ctypes.data, levels, x.ctypes.data, y.ctypes.data,
+                                z.ctypes.data, n, max_iter, leaf_iter, tol_step, u.ctypes.data, v.ctypes.data,
+                                st.ctypes.data, nthreads)
+        return u, v, st
 
     def build_tree(self, levels, flags=0):
         cnt = lib().or_tree_base(levels)
