@@ -533,6 +533,21 @@ def main():
             "kernel_ms": gn_ms, "flops_fp32": f32, "flops_fp64": f64,
             "iters_fp32_mean": float(it32.mean()), "iters_fp64_mean": float(it64.mean())}
 
+    # global projection (branch and bound over the seed tree) on the same queries: its cost
+    # against the local mixed mode, and the leaves it solves per query
+    gopts = G.default_opts(precision=G.GN_FP64, newton=G.NEWTON_FP64, seed_mode=G.SEED_GLOBAL, max_iter=20,
+                           tol_step=1e-12)
+    gs.NewInverse(x, y, z, gopts, out=(u, v, st))
+    torch.cuda.synchronize()
+    g0 = time.perf_counter()
+    _, _, _, ncand, _ = gs.NewInverse(x, y, z, gopts, out=(u, v, st), iters=True)
+    torch.cuda.synchronize()
+    g_ms = (time.perf_counter() - g0) * 1e3
+    global_res = {"ms": g_ms, "points_per_s": ok_count() / (g_ms / 1e3), "ok_frac": ok_count() / N,
+                  "leaves_solved_mean": float(ncand.float().mean().item()),
+                  "note": "GN_SEED_GLOBAL: exact nearest leaf + Newton, then leaf-clamped projected Newton in "
+                          "every leaf whose box may hold a nearer point, then a polish (FP64); one timed call"}
+
     cpu = cpu_case = None
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -587,6 +602,7 @@ def main():
         "fp64_newton": res["fp64_newton"],
         "speedup_vs_plain_fp64": mixed["points_per_s"] / max(res["plain_fp64"]["points_per_s"], 1e-30),
         "seed_ms": seed_ms,
+        "global_projection": global_res,
         "cpu_case_8x8": cpu_case,
         "eval_16M": eval_res,
         "uneven_batch": batch_res,
