@@ -10,6 +10,8 @@
 //   EvaluateRandom src/NURBS/Evaluate.cu:1023-1038 (plus the derivatives of 535-930)
 //   BuildTree     src/kernel/kernel.cu:375-431, 544-621, 707-733 (launch sequence main.cu:280-309)
 //   NewInverse    src/kernel/kernel.cu:433-469 with GaussNewton 151-249
+// NurbsCurve (rational curves: the reference's README names them, its code has none) and
+// NurbsSurface::PCurve (the p-curve use of README.md:10) have no reference entry points.
 // Errors throw gnurbs::Error carrying the gn_status code and message (the reference exits the
 // process in HANDLE_ERROR, src/main.cu:34-43).  Query and result buffers are device pointers
 // owned by the caller, as in the reference.
@@ -40,6 +42,8 @@ inline gn_invert_opts default_opts() {
     gn_invert_opts_default(&o);
     return o;
 }
+
+class NurbsCurve;
 
 class NurbsSurface {
   public:
@@ -114,6 +118,14 @@ class NurbsSurface {
                     gn_stream st = nullptr) const {
         check(gn_invert(s_, x, y, z, n, &o, u, v, status, st));
     }
+    // p-curve of `c` on this surface: control points (u, v) and knots of the fitted B-spline
+    struct PCurveFit {
+        std::vector<double> cp_uv, knots;
+        double max_dev = 0;
+        size_t failed = 0;
+    };
+    inline PCurveFit PCurve(const NurbsCurve &c, size_t samples, int degree, int ncp,
+                            const gn_invert_opts &o = default_opts(), gn_stream st = nullptr) const;
 
   private:
     explicit NurbsSurface(gn_surface *s) : s_(s) { check(gn_surface_info(s_, &p_, &q_, &nu_, &nv_)); }
@@ -124,6 +136,60 @@ class NurbsSurface {
     gn_surface *s_ = nullptr;
     int p_ = 0, q_ = 0, nu_ = 0, nv_ = 0, levels_ = 0;
 };
+
+// A rational B-spline curve resident in HBM (degree, control points xyz[cp][3], weights,
+// clamped knots[cp + degree + 1]).
+class NurbsCurve {
+  public:
+    NurbsCurve(int degree, const std::vector<double> &knots,
+               const std::vector<double> &control_points_xyz, const std::vector<double> &weights,
+               gn_stream stream = nullptr) {
+        const int cp = (int)weights.size();
+        if ((long)knots.size() != (long)cp + degree + 1 || (long)control_points_xyz.size() != 3L * cp)
+            throw Error(GN_EINVAL, "curve arrays do not match the degree and control point count");
+        check(gn_curve_create(degree, cp, knots.data(), control_points_xyz.data(), weights.data(),
+                              stream, &c_));
+    }
+    NurbsCurve(const NurbsCurve &) = delete;
+    NurbsCurve &operator=(const NurbsCurve &) = delete;
+    NurbsCurve(NurbsCurve &&o) noexcept : c_(o.c_) { o.c_ = nullptr; }
+    NurbsCurve &operator=(NurbsCurve &&o) noexcept {
+        if (this != &o) {
+            if (c_) gn_curve_destroy(c_);
+            c_ = o.c_;
+            o.c_ = nullptr;
+        }
+        return *this;
+    }
+    ~NurbsCurve() {
+        if (c_) gn_curve_destroy(c_);
+    }
+    // C (and C', C'') at device parameters t, SoA blocks of n on the device
+    void Evaluate(const double *t, size_t n, int order, double *out, gn_stream st = nullptr) const {
+        check(gn_curve_evaluate(c_, t, n, order, out, st));
+    }
+    // nearest-point parameters t of device points (x, y, z)
+    void Invert(const double *x, const double *y, const double *z, size_t n, double *t,
+                uint8_t *status, const gn_invert_opts &o = default_opts(),
+                gn_stream st = nullptr) const {
+        check(gn_curve_invert(c_, x, y, z, n, &o, t, status, st));
+    }
+    const gn_curve *handle() const { return c_; }
+
+  private:
+    gn_curve *c_ = nullptr;
+};
+
+inline NurbsSurface::PCurveFit NurbsSurface::PCurve(const NurbsCurve &c, size_t samples, int degree,
+                                                    int ncp, const gn_invert_opts &o,
+                                                    gn_stream st) const {
+    PCurveFit f;
+    f.cp_uv.resize(2 * (size_t)ncp);
+    f.knots.resize((size_t)ncp + degree + 1);
+    check(gn_pcurve(s_, c.handle(), samples, degree, ncp, &o, f.cp_uv.data(), f.knots.data(),
+                    &f.max_dev, &f.failed, st));
+    return f;
+}
 
 }  // namespace gnurbs
 
