@@ -56,11 +56,20 @@ inline bool knots_fit_lds(const GnSurfDev &d) {
     return d.nu + d.p + 1 + d.nv + d.q + 1 <= kLdsKnotsInv;
 }
 
-__device__ __forceinline__ void stage_knots_lds(const GnSurfDev &d, double *sk) {
-    const int nku = d.nu + d.p + 1, nkv = d.nv + d.q + 1;
+// Knots, then the reciprocal span lengths, into LDS (2 * kLdsKnotsInv doubles hold both when
+// knots_fit_lds: a surface has fewer spans than knots in each direction).  Returns the surface
+// view whose span lengths are read from LDS.
+__device__ __forceinline__ GnSurfDev stage_knots_lds(const GnSurfDev &d, double *sk) {
+    const int nku = d.nu + d.p + 1, nkv = d.nv + d.q + 1, nsu = d.nu - d.p, nsv = d.nv - d.q;
     for (int i = threadIdx.x; i < nku; i += blockDim.x) sk[i] = d.ku[i];
     for (int i = threadIdx.x; i < nkv; i += blockDim.x) sk[nku + i] = d.kv[i];
+    for (int i = threadIdx.x; i < nsu; i += blockDim.x) sk[nku + nkv + i] = d.ihu[i];
+    for (int i = threadIdx.x; i < nsv; i += blockDim.x) sk[nku + nkv + nsu + i] = d.ihv[i];
     __syncthreads();
+    GnSurfDev dl = d;
+    dl.ihu = sk + nku + nkv;
+    dl.ihv = sk + nku + nkv + nsu;
+    return dl;
 }
 
 // The degree dispatch of the inversion launchers: exact instantiations when the surface's knots
@@ -195,15 +204,16 @@ __device__ __forceinline__ unsigned long long bcast64(unsigned long long x, int 
 enum { kPlain = 0, kBulk = 1, kClose = 2 };
 
 template <int K1, int K2, bool EXACT, typename T, bool NEWTON, int MODE>
-__global__ __launch_bounds__(kInvBlock, (EXACT && sizeof(T) == 4 && K1 * K2 <= 16) ? 3 : 1) void k_gn_refill(GnSurfDev d, InvArgs a,
+__global__ __launch_bounds__(kInvBlock, (EXACT && sizeof(T) == 4 && K1 * K2 <= 16) ? 3 : 1) void k_gn_refill(GnSurfDev d_in, InvArgs a,
                                                          unsigned long long *counters, int slices,
                                                          int max_iter, int restart_iter,
                                                          double tol_dist, double tol_step,
                                                          double tol_quad) {
-    __shared__ double sk[EXACT ? kLdsKnotsInv : 1];
-    const double *ku = d.ku, *kv = d.kv;
+    __shared__ double sk[EXACT ? 2 * kLdsKnotsInv : 1];
+    const double *ku = d_in.ku, *kv = d_in.kv;
+    GnSurfDev d = d_in;
     if constexpr (EXACT) {
-        stage_knots_lds(d, sk);
+        d = stage_knots_lds(d_in, sk);
         ku = sk;
         kv = sk + d.nu + d.p + 1;
     }

@@ -6,14 +6,15 @@
 namespace gn {
 
 template <int K1, int K2, bool EXACT>
-__global__ __launch_bounds__(kInvBlock) void k_invert_first_hit(GnSurfDev d, InvArgs a,
+__global__ __launch_bounds__(kInvBlock) void k_invert_first_hit(GnSurfDev d_in, InvArgs a,
                                                                 const GnTreeNode *__restrict__ tree,
                                                                 const uint8_t *__restrict__ valid,
                                                                 long count) {
-    __shared__ double sk[EXACT ? kLdsKnotsInv : 1];
-    const double *ku = d.ku, *kv = d.kv;
+    __shared__ double sk[EXACT ? 2 * kLdsKnotsInv : 1];
+    const double *ku = d_in.ku, *kv = d_in.kv;
+    GnSurfDev d = d_in;
     if constexpr (EXACT) {
-        stage_knots_lds(d, sk);
+        d = stage_knots_lds(d_in, sk);
         ku = sk;
         kv = sk + d.nu + d.p + 1;
     }
