@@ -54,6 +54,7 @@ void gn_invert_opts_default(gn_invert_opts *o) {
     o->seed_mode = GN_SEED_GREEDY_LEAF;
     o->newton = 0;
     o->tol_quad = 0.0;
+    o->fallback = 0;
 }
 
 static void free_tree(gn_surface *s) {
@@ -377,6 +378,8 @@ int gn_invert_iters(const gn_surface *s, const double *x, const double *y, const
         return fail(GN_EINVAL, "bad seed_mode");
     if (o.seed_mode == GN_SEED_GLOBAL && o.precision != GN_FP64)
         return fail(GN_EINVAL, "global projection runs in FP64 (precision GN_FP64)");
+    if (o.fallback && !s->tree_levels)
+        return fail(GN_ESTATE, "the global fallback needs the tree (gn_build_tree)");
     if (o.max_iter < 0 || o.fp32_iters < 0 || o.fp64_tail < 0 || o.max_iter > 10000 ||
         o.fp32_iters > 10000 || o.fp64_tail > 10000)
         return fail(GN_EINVAL, "iteration counts must be 0..10000");

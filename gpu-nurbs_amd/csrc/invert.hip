@@ -13,18 +13,22 @@ int launch_invert(const GnSurfDev &d, const GnSearchDev &sd, const GnTreeNode *t
                   uint8_t *status, uint8_t *it32, uint8_t *it64, hipStream_t st) {
     if (n == 0) return GN_OK;
     InvArgs a{x, y, z, n, u, v, status, it32, it64, o};
+    if (o.seed_mode == GN_SEED_GLOBAL) return launch_project_global(d, sd, a, st);
+    int rc;
     if (o.seed_mode == GN_SEED_REF_FIRST_HIT) {
         long count = 0, g = 1;
         for (int l = 0; l < tree_levels; l++) { count += g * g; g *= 8; }
-        return launch_gn_first_hit(d, a, tree, valid, count, st);
-    }
-    if (o.seed_mode == GN_SEED_GLOBAL) return launch_project_global(d, sd, a, st);
-    if (o.seed_mode == GN_SEED_NEAREST_LEAF || o.seed_mode == GN_SEED_GREEDY_LEAF) {
-        int rc = launch_seed(sd, o.seed_mode == GN_SEED_NEAREST_LEAF ? 0 : 1, x, y, z, n, nullptr,
+        rc = launch_gn_first_hit(d, a, tree, valid, count, st);
+    } else {
+        if (o.seed_mode == GN_SEED_NEAREST_LEAF || o.seed_mode == GN_SEED_GREEDY_LEAF) {
+            rc = launch_seed(sd, o.seed_mode == GN_SEED_NEAREST_LEAF ? 0 : 1, x, y, z, n, nullptr,
                              u, v, st);
-        if (rc) return rc;
+            if (rc) return rc;
+        }
+        rc = o.precision == GN_MIXED ? launch_gn_mixed(d, a, st) : launch_gn_plain(d, a, st);
     }
-    return o.precision == GN_MIXED ? launch_gn_mixed(d, a, st) : launch_gn_plain(d, a, st);
+    if (rc || !o.fallback) return rc;
+    return launch_project_fallback(d, sd, a, st);
 }
 
 }  // namespace gn

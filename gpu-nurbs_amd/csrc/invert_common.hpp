@@ -343,5 +343,7 @@ int launch_gn_first_hit(const GnSurfDev &d, const InvArgs &a, const GnTreeNode *
                         const uint8_t *valid, long count, hipStream_t st);
 int launch_project_global(const GnSurfDev &d, const GnSearchDev &sd, const InvArgs &a,
                           hipStream_t st);  // project.hip
+int launch_project_fallback(const GnSurfDev &d, const GnSearchDev &sd, const InvArgs &a,
+                            hipStream_t st);  // project.hip
 
 }  // namespace gn

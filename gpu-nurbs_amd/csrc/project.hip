@@ -12,7 +12,8 @@
 // depth-3 tree (64 x 64 cells) are nearly flat, so the distance restricted to one is close to
 // convex.  Projected Newton (gn_step with ACTIVE) holds a coordinate on its bound while the
 // descent points outward, so minima on cell edges and on the surface boundary are reached too.
-// FP64, one lane per query, exact-Hessian steps.
+// FP64, one lane per query, exact-Hessian steps.  Also the fallback of gn_invert (opts.fallback):
+// the queries an inversion left unconverged are listed and solved again this way.
 #include "gn_search.hpp"
 #include "invert_common.hpp"
 
@@ -65,72 +66,110 @@ __device__ __noinline__ int solve_in(const GnSurfDev &d, const double *P, double
     return st;
 }
 
+// The branch and bound for query i.  Global mode (counts): also writes the number of leaves
+// solved to it32.  Fallback (!counts): the (u, v) already in place is kept when it is nearer.
 template <int K1, int K2, bool EXACT>
-__global__ __launch_bounds__(256) void k_project_global(GnSurfDev d, GnSearchDev sd, InvArgs a) {
+__device__ __forceinline__ void project_point(const GnSurfDev &d, const GnSearchDev &sd,
+                                              const InvArgs &a, size_t i, bool counts) {
     const Domain full = domain_of(d);
     const int D = sd.depth;
     const double cell = 1.0 / (double)(1u << D);  // leaf cells of [0,1]^2 (the tree's uv layout)
     const int max_iter = a.o.max_iter;
     const double tol_step = a.o.tol_step;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < a.n;
-         i += (size_t)gridDim.x * blockDim.x) {
-        const double P[3] = {a.x[i], a.y[i], a.z[i]};
-        const uint32_t c0 = nearest_leaf(sd, P);
-        double bu = sd.seed_uv[2 * c0], bv = sd.seed_uv[2 * c0 + 1], best;
-        int bst = solve_in<K1, K2, EXACT>(d, P, bu, bv, full, max_iter, tol_step, best);
-        int ncand = 1;
-        if (D > 0) {
-            const float p[3] = {(float)P[0], (float)P[1], (float)P[2]};
-            // stackless depth-first walk (Morton-code successor), pruned by the current best
-            int l = 1;
-            uint32_t c = 0;
-            while (true) {
-                const double lb = (double)box_d2(sd.box32 + (qoff(l) + c) * 6, p);
-                bool descend = false;
-                if (lb <= best) {
-                    if (l < D) {
-                        descend = true;
-                    } else if (c != c0) {
-                        const uint32_t ix = compact2(c >> 1), iy = compact2(c);
-                        const Domain cr{ix * cell, (ix + 1) * cell, iy * cell, (iy + 1) * cell};
-                        double cu = sd.seed_uv[2 * c], cv = sd.seed_uv[2 * c + 1], d2;
-                        const int s2 = solve_in<K1, K2, EXACT>(d, P, cu, cv, cr, kLeafIter,
-                                                               tol_step, d2);
-                        ncand++;
-                        if (d2 < best) {
-                            best = d2;
-                            bu = cu;
-                            bv = cv;
-                            bst = s2;
-                        }
+    const double P[3] = {a.x[i], a.y[i], a.z[i]};
+    const uint32_t c0 = nearest_leaf(sd, P);
+    double bu = sd.seed_uv[2 * c0], bv = sd.seed_uv[2 * c0 + 1], best;
+    int bst = solve_in<K1, K2, EXACT>(d, P, bu, bv, full, max_iter, tol_step, best);
+    int ncand = 1;
+    if (D > 0) {
+        const float p[3] = {(float)P[0], (float)P[1], (float)P[2]};
+        // stackless depth-first walk (Morton-code successor), pruned by the current best
+        int l = 1;
+        uint32_t c = 0;
+        while (true) {
+            const double lb = (double)box_d2(sd.box32 + (qoff(l) + c) * 6, p);
+            bool descend = false;
+            if (lb <= best) {
+                if (l < D) {
+                    descend = true;
+                } else if (c != c0) {
+                    const uint32_t ix = compact2(c >> 1), iy = compact2(c);
+                    const Domain cr{ix * cell, (ix + 1) * cell, iy * cell, (iy + 1) * cell};
+                    double cu = sd.seed_uv[2 * c], cv = sd.seed_uv[2 * c + 1], d2;
+                    const int s2 = solve_in<K1, K2, EXACT>(d, P, cu, cv, cr, kLeafIter, tol_step,
+                                                           d2);
+                    ncand++;
+                    if (d2 < best) {
+                        best = d2;
+                        bu = cu;
+                        bv = cv;
+                        bst = s2;
                     }
                 }
-                if (descend) {
-                    c <<= 2;
-                    l++;
-                    continue;
-                }
-                while (l > 1 && (c & 3u) == 3u) {
-                    c >>= 2;
-                    l--;
-                }
-                if (l == 1 && (c & 3u) == 3u) break;
-                c++;
             }
+            if (descend) {
+                c <<= 2;
+                l++;
+                continue;
+            }
+            while (l > 1 && (c & 3u) == 3u) {
+                c >>= 2;
+                l--;
+            }
+            if (l == 1 && (c & 3u) == 3u) break;
+            c++;
         }
-        double pu = bu, pv = bv, pd2;
-        const int pst = solve_in<K1, K2, EXACT>(d, P, pu, pv, full, max_iter, tol_step, pd2);
-        if (pd2 <= best * (1.0 + 1e-9)) {  // not farther beyond rounding: the polished point
-            bu = pu;
-            bv = pv;
-            bst = pst;
-        }
-        a.u[i] = bu;
-        a.v[i] = bv;
-        a.status[i] = (uint8_t)bst;
+    }
+    double pu = bu, pv = bv, pd2;
+    const int pst = solve_in<K1, K2, EXACT>(d, P, pu, pv, full, max_iter, tol_step, pd2);
+    if (pd2 <= best * (1.0 + 1e-9)) {  // not farther beyond rounding: the polished point
+        bu = pu;
+        bv = pv;
+        bst = pst;
+        best = pd2;
+    }
+    if (!counts) {  // fallback: the earlier result stays if it is nearer
+        Cart<double> o;
+        eval_point<K1, K2, 0, EXACT, double>(d, d.ku, d.kv, a.u[i], a.v[i], o);
+        const double dx = o.S[0] - P[0], dy = o.S[1] - P[1], dz = o.S[2] - P[2];
+        if (fma(dz, dz, fma(dy, dy, dx * dx)) < best) return;
+    }
+    a.u[i] = bu;
+    a.v[i] = bv;
+    a.status[i] = (uint8_t)bst;
+    if (counts) {
         if (a.it32) a.it32[i] = sat8(ncand);
         if (a.it64) a.it64[i] = 0;
     }
+}
+
+template <int K1, int K2, bool EXACT>
+__global__ __launch_bounds__(256) void k_project_global(GnSurfDev d, GnSearchDev sd, InvArgs a) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < a.n;
+         i += (size_t)gridDim.x * blockDim.x)
+        project_point<K1, K2, EXACT>(d, sd, a, i, true);
+}
+
+// The fallback: the queries listed in list[0 .. *count) (iteration counts left as they were).
+template <int K1, int K2, bool EXACT>
+__global__ __launch_bounds__(256) void k_project_list(GnSurfDev d, GnSearchDev sd, InvArgs a,
+                                                      const uint32_t *__restrict__ list,
+                                                      const uint32_t *__restrict__ count) {
+    const size_t cnt = *count;
+    for (size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x; j < cnt;
+         j += (size_t)gridDim.x * blockDim.x)
+        project_point<K1, K2, EXACT>(d, sd, a, list[j], false);
+}
+
+__global__ void k_fb_zero(uint32_t *count) { *count = 0u; }
+
+// indices of the queries whose status is not converged
+__global__ void k_fb_compact(const uint8_t *__restrict__ status, size_t n, uint32_t *__restrict__ list,
+                             uint32_t *__restrict__ count) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        if (status[i] != GN_PT_CONVERGED && status[i] != GN_PT_CONVERGED_DIST)
+            list[atomicAdd(count, 1u)] = (uint32_t)i;
 }
 
 struct ProjectLaunch {
@@ -150,6 +189,38 @@ int launch_project_global(const GnSurfDev &d, const GnSearchDev &sd, const InvAr
                           hipStream_t st) {
     if (a.n == 0) return GN_OK;
     return dispatch_degrees(d.p, d.q, ProjectLaunch{d, sd, a, st});
+}
+
+struct ListLaunch {
+    const GnSurfDev &d;
+    const GnSearchDev &sd;
+    const InvArgs &a;
+    const uint32_t *list, *count;
+    hipStream_t st;
+    template <int K1, int K2, bool EXACT>
+    int operator()() {
+        k_project_list<K1, K2, EXACT><<<grid_for(a.n, 256, 256 * 16), 256, 0, st>>>(d, sd, a, list,
+                                                                                 count);
+        GN_HIP(hipGetLastError());
+        return GN_OK;
+    }
+};
+
+// After an inversion on st: the unconverged queries are listed on the device (stream-ordered
+// scratch of n + 1 words) and solved again by the global projection.
+int launch_project_fallback(const GnSurfDev &d, const GnSearchDev &sd, const InvArgs &a,
+                            hipStream_t st) {
+    if (a.n == 0) return GN_OK;
+    if (a.n > 0xffffffffull) return fail(GN_EINVAL, "the fallback lists at most 2^32 - 1 queries");
+    uint32_t *buf = nullptr;
+    GN_HIP(hipMallocAsync((void **)&buf, sizeof(uint32_t) * (a.n + 1), st));
+    uint32_t *count = buf, *list = buf + 1;
+    k_fb_zero<<<1, 1, 0, st>>>(count);
+    k_fb_compact<<<grid_for(a.n, 256, 256 * 16), 256, 0, st>>>(a.status, a.n, list, count);
+    int rc = hipGetLastError() == hipSuccess ? GN_OK : fail(GN_EDEVICE, "fallback compaction launch");
+    if (!rc) rc = dispatch_degrees(d.p, d.q, ListLaunch{d, sd, a, list, count, st});
+    GN_HIP(hipFreeAsync(buf, st));
+    return rc;
 }
 
 }  // namespace gn

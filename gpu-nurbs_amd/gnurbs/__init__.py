@@ -49,6 +49,7 @@ class InvertOpts(ctypes.Structure):
         ("seed_mode", _c_int),
         ("newton", _c_int),
         ("tol_quad", _c_double),
+        ("fallback", _c_int),
     ]
 
 

@@ -89,6 +89,11 @@ typedef struct {
     double tol_quad;    /* FP64 exact-Hessian steps also stop (converged) once the step is     */
                         /* below tol_quad: the error left after such a step is O(step^2);      */
                         /* <= 0 off                                                            */
+    int fallback;       /* gn_invert: points that end unconverged are solved again by the      */
+                        /* global projection (GN_SEED_GLOBAL, FP64, same max_iter / tol_step), */
+                        /* keeping the nearer of the two results -- the role of the legacy     */
+                        /* subdivision inversion's robust seeding (src/kernel/kernel.cu:4-149, */
+                        /* commented out at src/main.cu:270); needs the tree.  0: off          */
 } gn_invert_opts;
 
 enum { GN_NEWTON_FP32 = 1, /* FP32 bulk iterations of GN_MIXED                                */

@@ -95,3 +95,43 @@ def test_global_argument_errors():
         g.NewInverse(z, z, z, G.default_opts(**dict(GLOBAL, precision=G.GN_MIXED)))
     assert e.value.code == G.GN_EINVAL
     g.close()
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_fallback_resolves_unconverged_points(bumpy, prec):
+    """opts.fallback: the points an inversion leaves unconverged are solved again by the global
+    projection; converged points are untouched, each of the others is the nearer of its first
+    result and the global projection run on it alone with the same options."""
+    sd, g, os_, nodes = bumpy
+    _, _, P = _queries(os_, 3000, 0.2, 4)
+    if prec == "fp64":  # few iterations: many points end unconverged
+        base = dict(precision=G.GN_FP64, newton=G.NEWTON_FP64, max_iter=3, tol_step=1e-12)
+    else:
+        base = dict(precision=G.GN_MIXED, newton=G.NEWTON_FP32 | G.NEWTON_FP64, fp32_iters=3, fp64_tail=1,
+                    tol_step32=1e-4, tol_step=1e-12, tol_quad=1e-6, max_iter=3)
+    u0, v0, s0 = (_np(t) for t in g.NewInverse(P[0], P[1], P[2], G.default_opts(**base)))
+    u1, v1, s1 = (_np(t) for t in g.NewInverse(P[0], P[1], P[2], G.default_opts(fallback=1, **base)))
+    bad = (s0 != G.PT_CONVERGED) & (s0 != G.PT_CONVERGED_DIST)
+    assert bad.sum() > 50
+    assert np.array_equal(u1[~bad], u0[~bad]) and np.array_equal(v1[~bad], v0[~bad])
+    assert np.array_equal(s1[~bad], s0[~bad])
+    glob = dict(base, precision=G.GN_FP64, seed_mode=G.SEED_GLOBAL)
+    Pb = P[:, bad]
+    ug, vg, sg = (_np(t) for t in g.NewInverse(Pb[0].copy(), Pb[1].copy(), Pb[2].copy(), G.default_opts(**glob)))
+    first = (u1[bad] == u0[bad]) & (v1[bad] == v0[bad]) & (s1[bad] == s0[bad])
+    glob_ = (u1[bad] == ug) & (v1[bad] == vg) & (s1[bad] == sg)
+    assert np.all(first | glob_)
+    d0, d1, dg = _dist(os_, P, u0, v0), _dist(os_, P, u1, v1), _dist(os_, Pb, ug, vg)
+    assert np.all(d1 <= d0 + 1e-12)
+    assert np.all(d1[bad] <= dg + 1e-12)
+    assert np.sum(d1[bad] < d0[bad] - 1e-6) > 10  # the fallback found nearer points
+
+
+def test_fallback_needs_the_tree():
+    sd = synth.surface_G(8, 8, 3, 3, seed=1)
+    g = G.Surface(sd.p, sd.q, sd.ku, sd.kv, sd.cp, sd.w)
+    z = np.zeros(4)
+    with pytest.raises(G.GnError) as e:
+        g.NewInverse(z, z, z, G.default_opts(fallback=1), seeds=(z, z))
+    assert e.value.code == G.GN_ESTATE
+    g.close()
