@@ -120,7 +120,8 @@ __device__ __forceinline__ int gn_step(const GnSurfDev &d, const double *__restr
     T c = o.Sv[0] * o.Sv[0] + o.Sv[1] * o.Sv[1] + o.Sv[2] * o.Sv[2];
     const T gu = o.Su[0] * r0 + o.Su[1] * r1 + o.Su[2] * r2;
     const T gv = o.Sv[0] * r0 + o.Sv[1] * r1 + o.Sv[2] * r2;
-    T a1 = a, c1 = c;  // one-dimensional curvatures for a held coordinate (ACTIVE)
+    T a1 = a, c1 = c;    // one-dimensional curvatures for a held coordinate (ACTIVE)
+    bool hexact = false;  // the step uses the exact Hessian (quadratic convergence)
     if (NEWTON) {
         const T a2 = a + r0 * o.Suu[0] + r1 * o.Suu[1] + r2 * o.Suu[2];
         const T b2 = b + r0 * o.Suv[0] + r1 * o.Suv[1] + r2 * o.Suv[2];
@@ -129,7 +130,12 @@ __device__ __forceinline__ int gn_step(const GnSurfDev &d, const double *__restr
             if (a2 > T(0)) a1 = a2;
             if (c2 > T(0)) c1 = c2;
         }
-        if (a2 > T(0) && a2 * c2 - b2 * b2 > T(0)) { a = a2; b = b2; c = c2; }
+        if (a2 > T(0) && a2 * c2 - b2 * b2 > T(0)) {
+            a = a2;
+            b = b2;
+            c = c2;
+            hexact = true;
+        }
     }
     const T det = a * c - b * b;
     if (!(det == det)) return GN_PT_NONFINITE;
@@ -156,7 +162,10 @@ __device__ __forceinline__ int gn_step(const GnSurfDev &d, const double *__restr
     }
     if (!(cu == cu) || !(cv == cv)) return GN_PT_NONFINITE;
     if (tol_step > 0 && fabs(du) < T(tol_step) && fabs(dv) < T(tol_step)) return GN_PT_CONVERGED;
-    if (NEWTON && tol_quad > 0 && fabs(du) < T(tol_quad) && fabs(dv) < T(tol_quad))
+    // the O(step^2) certificate holds only for an exact-Hessian step; where the Hessian is not
+    // positive definite (a saddle of the distance: the query is farther than the curvature radius)
+    // the Gauss-Newton step converges linearly and only tol_step stops it
+    if (NEWTON && hexact && tol_quad > 0 && fabs(du) < T(tol_quad) && fabs(dv) < T(tol_quad))
         return GN_PT_CONVERGED;
     return kStepContinue;
 }

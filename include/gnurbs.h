@@ -87,8 +87,8 @@ typedef struct {
                         /* term sum_k r_k d2S_k to J^T J (exact Hessian of |S-P|^2/2, falling  */
                         /* back to J^T J where that is not positive definite)                  */
     double tol_quad;    /* FP64 exact-Hessian steps also stop (converged) once the step is     */
-                        /* below tol_quad: the error left after such a step is O(step^2);      */
-                        /* <= 0 off                                                            */
+                        /* below tol_quad: the error left after such a step is O(step^2)      */
+                        /* (not applied to a step that fell back to J^T J); <= 0 off           */
     int fallback;       /* gn_invert: points that end unconverged are solved again by the      */
                         /* global projection (GN_SEED_GLOBAL, FP64, same max_iter / tol_step), */
                         /* keeping the nearer of the two results -- the role of the legacy     */
