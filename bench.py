@@ -237,7 +237,8 @@ def bench_serve(G, synth, torch, duration=1.0, loads=(0.5, 0.8), budget=32 << 30
     helix curves (64 control points) resident; requests of log-uniform size on [64, 100000] points
     for a uniformly drawn object, Poisson arrivals at 50% and 80% of the capacity measured by a
     saturating burst; continuous batching (gn_server, 2 step slots, 32 GB HBM budget).  Latency is
-    from the scheduled arrival to the server's completion time."""
+    from the scheduled arrival to the server's completion time, over one measured second at each
+    load after 0.3 s of warm-up at the same rate."""
     import ctypes
     import sys
 
@@ -354,6 +355,10 @@ def bench_serve(G, synth, torch, duration=1.0, loads=(0.5, 0.8), budget=32 << 30
                 rate = load * cap / mean_size
                 dur = min(duration, 0.6 * arena / (17.0 * load * cap))
                 n_req = max(200, int(rate * dur))
+                # steady state: 0.3 s at the same rate first (discarded); the first loaded window
+                # after the idle gap has shown one stall of all GPU work of several ms
+                n_w = max(100, int(rate * 0.3 * dur / duration))
+                run(srv, *trace(n_w, rate, int(2004 + 100 * load)))
                 sizes, objs, offs, arr = trace(n_req, rate, int(1004 + 100 * load))
                 st0 = srv.stats()
                 lat, lag, span, steps = run(srv, sizes, objs, offs, arr)
