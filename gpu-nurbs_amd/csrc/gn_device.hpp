@@ -361,8 +361,9 @@ struct SpanCache {
 };
 
 // Patch sources of eval_point_cached: per-lane gathers, the register-cached patch, or LDS rows
-// staged by the whole wave (wave-uniform call required).
-enum { kPatchGlobal = 0, kPatchRegs = 1, kPatchWave = 2 };
+// staged by the whole wave (wave-uniform call required), the latter also with the next row
+// prefetched.
+enum { kPatchGlobal = 0, kPatchRegs = 1, kPatchWave = 2, kPatchWavePipe = 3 };
 
 template <int K1, int K2, int ORDER, bool EXACT, typename T, int PSRC = kPatchGlobal>
 __device__ __forceinline__ void eval_point_cached(const GnSurfDev &d, const double *__restrict__ ku,
@@ -411,6 +412,11 @@ __device__ __forceinline__ void eval_point_cached(const GnSurfDev &d, const doub
         static_assert(EXACT, "LDS row staging needs exact sizes");
         (void)moved;
         WaveRows<K2, T> rows(patch, wave_lds);
+        contract_rows<K1, K2, ORDER, true, T>(rows, k1, k2, t, s, h);
+    } else if constexpr (PSRC == kPatchWavePipe) {
+        static_assert(EXACT, "LDS row staging needs exact sizes");
+        (void)moved;
+        WaveRowsPipe<K1, K2, T> rows(patch, wave_lds);
         contract_rows<K1, K2, ORDER, true, T>(rows, k1, k2, t, s, h);
     } else {
         (void)moved;

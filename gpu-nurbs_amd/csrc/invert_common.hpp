@@ -242,12 +242,14 @@ __global__ __launch_bounds__(kInvBlock, (EXACT && sizeof(T) == 4 && K1 * K2 <= 1
     int it = 0, lim = max_iter;
     SpanCache sc;
     // FP32 exact-degree kernels keep the lane's span patch in registers between iterations; FP64
-    // exact-degree kernels (one or few iterations per point) stage patch rows through LDS
+    // exact-degree kernels stage patch rows through LDS with the next row in flight (per-lane
+    // gathers: plain FP64 974 us, rows one at a time 717 us, prefetched 531 us at 1M points)
     constexpr int PSRC = !EXACT ? kPatchGlobal
                        : sizeof(T) == 4 ? (K1 * K2 <= 16 ? kPatchRegs : kPatchGlobal)
-                                        : kPatchWave;
+                                        : kPatchWavePipe;
     PatchRegs<T, K1 * K2> pr;
-    constexpr int kWaveVec = PSRC == kPatchWave ? WaveRows<K2, T>::STRIDE * 64 : 1;
+    constexpr bool kWave = PSRC == kPatchWave || PSRC == kPatchWavePipe;
+    constexpr int kWaveVec = kWave ? WaveRows<K2, T>::STRIDE * 64 : 1;
     __shared__ float4 wrows[kInvBlock / 64][kWaveVec];
     float4 *wave_lds = wrows[threadIdx.x >> 6];
     while (true) {
@@ -290,7 +292,7 @@ __global__ __launch_bounds__(kInvBlock, (EXACT && sizeof(T) == 4 && K1 * K2 <= 1
         // discard the result)
         int st = kStepContinue;
         const bool act = idx >= 0 && it < lim;
-        if (PSRC == kPatchWave || act) {
+        if (kWave || act) {
             double tu = act ? cu : dom.u0, tv = act ? cv : dom.v0;
             const int s2 = gn_step<K1, K2, EXACT, T, NEWTON, PSRC>(
                 d, ku, kv, Px, Py, Pz, tu, tv, tol_dist, tol_step, tol_quad, a.o.clamp, dom, sc,
