@@ -16,6 +16,14 @@ constexpr int kCounterStride = 16;  // 16 x 8 bytes = one 128-byte line per coun
 // that uses them must end every wave with retire_wave, which leaves them zeroed again.
 int work_counters(hipStream_t st, unsigned long long **out);
 constexpr int kExitWord = 8;  // exit counter: word 8 of slice 0's line (slices use word 0)
+// The continuation list paired with a counter slot, for a streamed pass that hands the points it
+// has not finished to a lane-refill pass (kCloseList): kMaxSlices segments of kListSegCap point
+// indices, segment s filled through the count in word kListWord of slice s's line (appends to
+// one address would serialise), which retire_wave clears with the claim counters.
+constexpr int kListWord = 9;
+constexpr size_t kListSegCap = 2048;
+constexpr size_t kListCap = kListSegCap * kMaxSlices;
+int work_list(const unsigned long long *counters, uint32_t **list);
 int cu_count();
 
 struct InvArgs {
@@ -25,26 +33,42 @@ struct InvArgs {
     uint8_t *status;
     uint8_t *it32, *it64;  // optional per-point iteration counts (FP32 bulk, FP64), or null
     gn_invert_opts o;
+    const uint32_t *list = nullptr;  // kCloseList: the points to continue (kListCap segments)
 };
 
 // Lane 0 of every wave calls this once, after the wave's last claim.  A wave has consumed the
 // value returned by each of its claim atomics before it gets here, so its claims are performed
-// before it counts itself out; the count itself needs no release ordering (an agent-scope
-// release would write back the XCD's L2 once per wave).  The last wave of the grid to arrive
-// takes an acquire fence and zeroes the claim counters and the exit count, which readies the
-// slot for its next launch without a separate clearing kernel (and keeps captured graphs
-// replayable).
+// before it counts itself out; the counts themselves need no release ordering (an agent-scope
+// release would write back the XCD's L2 once per wave).  Exits are counted in two levels: wave w
+// counts into the exit word of slice w % slices (its own 128-byte line), and the last wave of a
+// slice counts the slice into kExitWord -- one address taking an add from every wave serialises
+// at ~12 ns each, which a grid of thousands of waves retiring together would wait out.  The last
+// slice to finish takes an acquire fence and zeroes the claim counters, the list counts and the
+// exit counts, which readies the slot for its next launch without a separate clearing kernel (and
+// keeps captured graphs replayable).  Needs slices <= waves (every slice has a wave).
+constexpr int kSliceExitWord = 10;
 __device__ __forceinline__ void retire_wave(unsigned long long *counters, int slices) {
     const unsigned long long waves = (unsigned long long)gridDim.x * (blockDim.x / 64);
-    const unsigned long long prev = __hip_atomic_fetch_add(counters + kExitWord, 1ull, __ATOMIC_RELAXED,
+    const unsigned long long w = ((unsigned long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const unsigned long long s = w % (unsigned long long)slices;
+    const unsigned long long in_slice = waves / slices + (s < waves % slices ? 1ull : 0ull);
+    const unsigned long long prev = __hip_atomic_fetch_add(
+        counters + s * kCounterStride + kSliceExitWord, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (prev + 1 != in_slice) return;
+    const unsigned long long done = __hip_atomic_fetch_add(counters + kExitWord, 1ull, __ATOMIC_RELAXED,
                                                            __HIP_MEMORY_SCOPE_AGENT);
-    if (prev + 1 == waves) {
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-        for (int s = 0; s < slices; s++)
-            __hip_atomic_store(counters + (size_t)s * kCounterStride, 0ull, __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(counters + kExitWord, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (done + 1 != (unsigned long long)slices) return;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    for (int k = 0; k < slices; k++) {
+        __hip_atomic_store(counters + (size_t)k * kCounterStride, 0ull, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(counters + (size_t)k * kCounterStride + kSliceExitWord, 0ull,
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
+    for (int k = 0; k < kMaxSlices; k++)
+        __hip_atomic_store(counters + (size_t)k * kCounterStride + kListWord, 0ull, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(counters + kExitWord, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 __device__ __forceinline__ uint8_t sat8(int x) { return (uint8_t)(x > 255 ? 255 : x); }
@@ -210,7 +234,10 @@ __device__ __forceinline__ unsigned long long bcast64(unsigned long long x, int 
 //                 restarts from it.
 //   MODE kClose : FP64 closing pass of the mixed mode; a point gets max_iter steps from the bulk
 //                 result, or restart_iter steps when its bulk status is a breakdown.
-enum { kPlain = 0, kBulk = 1, kClose = 2 };
+//   MODE kCloseList : the rest of a closing pass whose first step was streamed: the points of
+//                 a.list (slice s takes list segment s; launched with kMaxSlices slices), one
+//                 step done, resume from their (u, v); their status is still the bulk's.
+enum { kPlain = 0, kBulk = 1, kClose = 2, kCloseList = 3 };
 
 template <int K1, int K2, bool EXACT, typename T, bool NEWTON, int MODE>
 __global__ __launch_bounds__(kInvBlock, (EXACT && sizeof(T) == 4 && K1 * K2 <= 16) ? 3 : 1) void k_gn_refill(GnSurfDev d_in, InvArgs a,
@@ -231,8 +258,15 @@ __global__ __launch_bounds__(kInvBlock, (EXACT && sizeof(T) == 4 && K1 * K2 <= 1
     const unsigned long long lt = (1ull << lane) - 1ull;
     const int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
     const int slice = wave % slices;
-    const unsigned long long s_lo = (unsigned long long)(a.n * (size_t)slice / slices);
-    const unsigned long long s_hi = (unsigned long long)(a.n * (size_t)(slice + 1) / slices);
+    unsigned long long s_lo = (unsigned long long)(a.n * (size_t)slice / slices);
+    unsigned long long s_hi = (unsigned long long)(a.n * (size_t)(slice + 1) / slices);
+    if (MODE == kCloseList) {
+        const unsigned long long listed = __hip_atomic_load(
+            counters + (size_t)slice * kCounterStride + kListWord, __ATOMIC_RELAXED,
+            __HIP_MEMORY_SCOPE_AGENT);
+        s_lo = (unsigned long long)slice * kListSegCap;
+        s_hi = s_lo + (listed < kListSegCap ? listed : kListSegCap);
+    }
     unsigned long long *counter = counters + (size_t)slice * kCounterStride;
     unsigned long long q_lo = 0, q_hi = 0;  // wave-uniform chunk
     bool exhausted = false;                 // wave-uniform
@@ -268,15 +302,15 @@ __global__ __launch_bounds__(kInvBlock, (EXACT && sizeof(T) == 4 && K1 * K2 <= 1
             if (idx < 0) {
                 const unsigned long long rank = (unsigned long long)__popcll(idle & lt);
                 if (rank < avail) {
-                    idx = (long long)(q_lo + rank);
+                    idx = MODE == kCloseList ? (long long)a.list[q_lo + rank] : (long long)(q_lo + rank);
                     Px = T(a.x[idx]);
                     Py = T(a.y[idx]);
                     Pz = T(a.z[idx]);
                     su = cu = a.u[idx];
                     sv = cv = a.v[idx];
-                    it = 0;
+                    it = MODE == kCloseList ? 1 : 0;
                     lim = max_iter;
-                    if (MODE == kClose) {
+                    if (MODE == kClose || MODE == kCloseList) {
                         const int sb = a.status[idx];
                         if (sb == GN_PT_SINGULAR || sb == GN_PT_NONFINITE) lim = restart_iter;
                     }

@@ -5,6 +5,8 @@
 //     zeroed (the last wave to retire clears it), so a launch needs no clearing kernel or memset
 //     and a captured hipGraph can be replayed.  Two launches share a slot only if more than
 //     kCounterSlots launches are in flight at once.
+//   - beside each slot, a continuation list of kListCap point indices (128 MB per device for the
+//     ring, allocated on the first mixed-precision inversion).
 //   - the compute-unit count used to size persistent grids.
 #include <atomic>
 #include <mutex>
@@ -21,6 +23,7 @@ constexpr size_t kSlotWords = (size_t)kMaxSlices * kCounterStride;
 namespace {
 std::mutex g_mu;
 unsigned long long *g_counters[kMaxDevices] = {};
+uint32_t *g_lists[kMaxDevices] = {};
 int g_cus[kMaxDevices] = {};
 std::atomic<unsigned> g_next{0};
 }  // namespace
@@ -48,6 +51,24 @@ int work_counters(hipStream_t st, unsigned long long **out) {
     const unsigned slot = g_next.fetch_add(1, std::memory_order_relaxed) % kCounterSlots;
     (void)st;
     *out = g_counters[dev] + (size_t)slot * kSlotWords;
+    return GN_OK;
+}
+
+int work_list(const unsigned long long *counters, uint32_t **list) {
+    int dev = 0;
+    GN_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(GN_EDEVICE, "device index out of range");
+    std::lock_guard<std::mutex> lk(g_mu);
+    const unsigned long long *base = g_counters[dev];
+    if (!base || counters < base || counters >= base + (size_t)kCounterSlots * kSlotWords)
+        return fail(GN_EDEVICE, "counter slot of another device");
+    if (!g_lists[dev]) {
+        void *p = nullptr;
+        GN_HIP(hipMalloc(&p, sizeof(uint32_t) * kCounterSlots * kListCap));  // 128 MB
+        g_lists[dev] = (uint32_t *)p;
+    }
+    const size_t slot = (size_t)(counters - base) / kSlotWords;
+    *list = g_lists[dev] + slot * kListCap;
     return GN_OK;
 }
 
