@@ -6,7 +6,7 @@
 //     and a captured hipGraph can be replayed.  Two launches share a slot only if more than
 //     kCounterSlots launches are in flight at once.
 //   - beside each slot, a continuation list of kListCap point indices (128 MB per device for the
-//     ring, allocated on the first mixed-precision inversion).
+//     ring, allocated with the counters).
 //   - the compute-unit count used to size persistent grids.
 #include <atomic>
 #include <mutex>
@@ -40,6 +40,11 @@ int ensure_runtime() {
         GN_HIP(hipDeviceSynchronize());
         g_counters[dev] = (unsigned long long *)p;
     }
+    if (!g_lists[dev]) {  // with the counters, so one inversion before a graph capture readies both
+        void *p = nullptr;
+        GN_HIP(hipMalloc(&p, sizeof(uint32_t) * kCounterSlots * kListCap));  // 128 MB
+        g_lists[dev] = (uint32_t *)p;
+    }
     return GN_OK;
 }
 
@@ -60,13 +65,9 @@ int work_list(const unsigned long long *counters, uint32_t **list) {
     if (dev < 0 || dev >= kMaxDevices) return fail(GN_EDEVICE, "device index out of range");
     std::lock_guard<std::mutex> lk(g_mu);
     const unsigned long long *base = g_counters[dev];
-    if (!base || counters < base || counters >= base + (size_t)kCounterSlots * kSlotWords)
+    if (!base || !g_lists[dev] || counters < base ||
+        counters >= base + (size_t)kCounterSlots * kSlotWords)
         return fail(GN_EDEVICE, "counter slot of another device");
-    if (!g_lists[dev]) {
-        void *p = nullptr;
-        GN_HIP(hipMalloc(&p, sizeof(uint32_t) * kCounterSlots * kListCap));  // 128 MB
-        g_lists[dev] = (uint32_t *)p;
-    }
     const size_t slot = (size_t)(counters - base) / kSlotWords;
     *list = g_lists[dev] + slot * kListCap;
     return GN_OK;
