@@ -128,7 +128,7 @@ void k_batch(BatchArgs a, unsigned long long *counters, int slices, int max_iter
     // waves per SIMD to one and group 2 would spill; group 1 runs one wave per SIMD either way
     // (uneven batch of 1,024 surfaces: plain FP64 8.09 -> 7.37 ms, mixed 2.52 -> 2.44 ms).
     constexpr bool WAVE = sizeof(T) == 8 && GRP == 1;
-    __shared__ float4 wrows[kInvBlock / 64][WAVE ? WaveRows<kExactMaxK, double>::STRIDE * 64 : 1];
+    __shared__ float4 wrows[kInvBlock / 64][WAVE ? WaveRowsPipe<kExactMaxK, kExactMaxK, double>::STRIDE * 64 : 1];
     float4 *wave_lds = wrows[threadIdx.x >> 6];
     while (true) {
         while (true) {

@@ -155,53 +155,15 @@ __device__ __forceinline__ void contract(const T *__restrict__ patch, int k1, in
 // Wave-cooperative staging of patch rows through LDS: row r of the 64 lanes' patches is copied
 // with contiguous 16-byte loads (one load instruction covers whole rows of several lanes, i.e.
 // a few cache lines, instead of one line per lane), then each lane reads its own row from LDS.
-// Every lane of the wave must take part (wave-uniform control flow); an idle lane passes any
-// valid patch.  lds: the wave's buffer of 64 * STRIDE 16-byte vectors.
-template <int K2, typename T>
-struct WaveRows {
+// The global loads of rows r+1..r+DEPTH are in flight while row r goes through LDS and is
+// contracted, so up to DEPTH L2 round trips overlap.  Rows must be requested in order 0, 1, ...,
+// K1-1 (contract_rows does).  Every lane of the wave must take part (wave-uniform control flow);
+// an idle lane passes any valid patch.  lds: the wave's buffer of 64 * STRIDE 16-byte vectors.
+template <int K1, int K2, typename T, int DEPTH = 1>
+struct WaveRowsPipe {
     static constexpr int VEC = K2 * 4 * (int)sizeof(T) / 16;  // 16-byte vectors per row
     static constexpr int STRIDE = VEC + 1;                     // padded row stride
     static_assert(K2 * 4 * sizeof(T) % 16 == 0, "row size");
-    const char *src[VEC];  // source of vector j*64 + lane of row 0
-    float4 *lds;
-    int lane;
-    __device__ __forceinline__ WaveRows(const T *patch, float4 *buf) : lds(buf) {
-        lane = threadIdx.x & 63;
-        const unsigned long long b = (unsigned long long)patch;
-        const unsigned lo = (unsigned)(b & 0xffffffffull), hi = (unsigned)(b >> 32);
-#pragma unroll
-        for (int j = 0; j < VEC; j++) {
-            const int vec = j * 64 + lane, pp = vec / VEC, o = vec - pp * VEC;
-            const unsigned long long pb = (unsigned long long)(unsigned)__shfl((int)lo, pp, 64) |
-                                          ((unsigned long long)(unsigned)__shfl((int)hi, pp, 64) << 32);
-            src[j] = reinterpret_cast<const char *>(pb) + o * 16;
-        }
-    }
-    __device__ __forceinline__ void operator()(int r, Hom<T> (&m)[K2]) {
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");  // earlier reads of the buffer
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-#pragma unroll
-        for (int j = 0; j < VEC; j++) {
-            const int vec = j * 64 + lane, pp = vec / VEC, o = vec - pp * VEC;
-            lds[pp * STRIDE + o] = *reinterpret_cast<const float4 *>(src[j] + (size_t)r * VEC * 16);
-        }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        const T *mine = reinterpret_cast<const T *>(lds + lane * STRIDE);
-#pragma unroll
-        for (int c = 0; c < K2; c++) m[c] = load_hom<T>(mine + c * 4);
-    }
-};
-
-// WaveRows with the next DEPTH rows prefetched: the global loads of rows r+1..r+DEPTH are in
-// flight while row r goes through LDS and is contracted, so up to DEPTH L2 round trips overlap.
-// Rows must be requested in order 0, 1, ..., K1-1 (contract_rows does), wave-uniformly.
-template <int K1, int K2, typename T, int DEPTH = 1>
-struct WaveRowsPipe {
-    static constexpr int VEC = WaveRows<K2, T>::VEC;
-    static constexpr int STRIDE = VEC + 1;
     const char *src[VEC];
     float4 *lds;
     float4 nxt[DEPTH][VEC];
@@ -361,9 +323,8 @@ struct SpanCache {
 };
 
 // Patch sources of eval_point_cached: per-lane gathers, the register-cached patch, or LDS rows
-// staged by the whole wave (wave-uniform call required), the latter also with the next row
-// prefetched.
-enum { kPatchGlobal = 0, kPatchRegs = 1, kPatchWave = 2, kPatchWavePipe = 3 };
+// staged by the whole wave with the next row prefetched (wave-uniform call required).
+enum { kPatchGlobal = 0, kPatchRegs = 1, kPatchWavePipe = 2 };
 
 template <int K1, int K2, int ORDER, bool EXACT, typename T, int PSRC = kPatchGlobal>
 __device__ __forceinline__ void eval_point_cached(const GnSurfDev &d, const double *__restrict__ ku,
@@ -408,11 +369,6 @@ __device__ __forceinline__ void eval_point_cached(const GnSurfDev &d, const doub
                 for (int c = 0; c < K2; c++) m[c] = pr[r * K2 + c];
             },
             k1, k2, t, s, h);
-    } else if constexpr (PSRC == kPatchWave) {
-        static_assert(EXACT, "LDS row staging needs exact sizes");
-        (void)moved;
-        WaveRows<K2, T> rows(patch, wave_lds);
-        contract_rows<K1, K2, ORDER, true, T>(rows, k1, k2, t, s, h);
     } else if constexpr (PSRC == kPatchWavePipe) {
         static_assert(EXACT, "LDS row staging needs exact sizes");
         (void)moved;

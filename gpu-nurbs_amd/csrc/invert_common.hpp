@@ -282,8 +282,8 @@ __global__ __launch_bounds__(kInvBlock, (EXACT && sizeof(T) == 4 && K1 * K2 <= 1
                        : sizeof(T) == 4 ? (K1 * K2 <= 16 ? kPatchRegs : kPatchGlobal)
                                         : kPatchWavePipe;
     PatchRegs<T, K1 * K2> pr;
-    constexpr bool kWave = PSRC == kPatchWave || PSRC == kPatchWavePipe;
-    constexpr int kWaveVec = kWave ? WaveRows<K2, T>::STRIDE * 64 : 1;
+    constexpr bool kWave = PSRC == kPatchWavePipe;
+    constexpr int kWaveVec = kWave ? WaveRowsPipe<K1, K2, T>::STRIDE * 64 : 1;
     __shared__ float4 wrows[kInvBlock / 64][kWaveVec];
     float4 *wave_lds = wrows[threadIdx.x >> 6];
     while (true) {
