@@ -532,7 +532,8 @@ def main():
     roof = {"bound": "mfma", "achieved": (f32 + f64) / t / 1e12, "peak": peak_eff / 1e12, "unit": "TFLOP/s",
             "frac": ideal / t, "traffic": traffic,
             "algorithmic_bytes": N * (40 + 17) + N * (41 + 17),
-            "kernel": "k_gn_refill FP32 bulk + FP64 closing pass (Gauss-Newton from given seeds)",
+            "kernel": "FP32 bulk (k_gn_refill) + FP64 closing pass (k_close_first, then k_gn_refill over the "
+                      "listed rest): Gauss-Newton from given seeds",
             "roof_note": "compute roof: FP32 flops at 157.3 TFLOP/s plus FP64 flops at 78.6 TFLOP/s "
                          "(published MI355X peaks); VALU arithmetic",
             "kernel_ms": gn_ms, "flops_fp32": f32, "flops_fp64": f64,
