@@ -26,8 +26,8 @@ PEAK_F64 = 78.6e12
 
 
 def flops_per_iter(k1, k2, newton):
-    """Algorithmic flops of one iteration (measurement): contraction FMAs x 2 plus
-    quotient and 2x2 solve."""
+    """Algorithmic flops of one iteration: contraction FMAs x 2 (point and first derivatives; with
+    Newton also the second derivatives) plus the rational quotient and the 2x2 solve."""
     fma1 = 4 * (k1 * k2 + (k1 - 1) * k2 + k2 + k2 + (k2 - 1))
     f = 2 * fma1 + 26 + 45
     if newton:
@@ -405,7 +405,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (inversion is "
-              "independent per point: shard points across GPUs, see DESIGN.md)", file=sys.stderr)
+              "independent per point, so points would be sharded across GPUs with no communication)",
+              file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
