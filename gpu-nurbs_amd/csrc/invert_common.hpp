@@ -48,12 +48,13 @@ struct InvArgs {
 // keeps captured graphs replayable).  Needs slices <= waves (every slice has a wave).
 constexpr int kSliceExitWord = 10;
 __device__ __forceinline__ void retire_wave(unsigned long long *counters, int slices) {
-    const unsigned long long waves = (unsigned long long)gridDim.x * (blockDim.x / 64);
-    const unsigned long long w = ((unsigned long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const unsigned long long s = w % (unsigned long long)slices;
-    const unsigned long long in_slice = waves / slices + (s < waves % slices ? 1ull : 0ull);
+    const unsigned waves = gridDim.x * (blockDim.x / 64);  // 32-bit: a grid is far below 2^32 waves
+    const unsigned w = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const unsigned ns = (unsigned)slices, s = w % ns;
+    const unsigned long long in_slice = waves / ns + (s < waves % ns ? 1u : 0u);
     const unsigned long long prev = __hip_atomic_fetch_add(
-        counters + s * kCounterStride + kSliceExitWord, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        counters + (size_t)s * kCounterStride + kSliceExitWord, 1ull, __ATOMIC_RELAXED,
+        __HIP_MEMORY_SCOPE_AGENT);
     if (prev + 1 != in_slice) return;
     const unsigned long long done = __hip_atomic_fetch_add(counters + kExitWord, 1ull, __ATOMIC_RELAXED,
                                                            __HIP_MEMORY_SCOPE_AGENT);
