@@ -224,6 +224,10 @@ int batch_waves(int bpc_cap);
 int launch_curve_batch(const GnCurveDev *curves, const GnTile *tiles, int ntiles, const GnReq *reqs,
                        int max_samples, const gn_invert_opts &o, hipStream_t st);
 int ensure_runtime();  // allocates the per-device claim counters (before any graph capture)
+// Copies `bytes` from pinned host memory (src: its device-side address) to device memory with an
+// ordinary kernel on st; both buffers 16-byte aligned and readable/writable to bytes rounded up
+// to 16.
+int launch_stage_copy(const void *src, void *dst, size_t bytes, hipStream_t st);
 int launch_invert(const GnSurfDev &d, const GnSearchDev &sd, const GnTreeNode *tree,
                   const uint8_t *valid, int tree_levels, const double *x, const double *y,
                   const double *z, size_t n, const gn_invert_opts &o, double *u, double *v,

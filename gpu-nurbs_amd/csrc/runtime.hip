@@ -7,6 +7,7 @@
 //     kCounterSlots launches are in flight at once.
 //   - beside each slot, a continuation list of kListCap point indices (128 MB per device for the
 //     ring, allocated with the counters).
+//   - a copy kernel for small host-to-device images (the server's step tables).
 //   - the compute-unit count used to size persistent grids.
 #include <atomic>
 #include <mutex>
@@ -70,6 +71,23 @@ int work_list(const unsigned long long *counters, uint32_t **list) {
         return fail(GN_EDEVICE, "counter slot of another device");
     const size_t slot = (size_t)(counters - base) / kSlotWords;
     *list = g_lists[dev] + slot * kListCap;
+    return GN_OK;
+}
+
+__global__ __launch_bounds__(256) void k_stage_copy(const uint4 *__restrict__ src,
+                                                    uint4 *__restrict__ dst, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+int launch_stage_copy(const void *src, void *dst, size_t bytes, hipStream_t st) {
+    const size_t n = (bytes + 15) / 16;
+    if (n == 0) return GN_OK;
+    const size_t blocks = (n + 255) / 256;
+    k_stage_copy<<<(unsigned)(blocks < 1024 ? blocks : 1024), 256, 0, st>>>(
+        (const uint4 *)src, (uint4 *)dst, n);
+    GN_HIP(hipGetLastError());
     return GN_OK;
 }
 

@@ -60,7 +60,8 @@ struct Slot {
     hipStream_t st = nullptr;
     hipEvent_t ev = nullptr;
     gn::SideStreams side;  // the step's class launches run concurrently on these
-    char *h_stage = nullptr;  // pinned
+    char *h_stage = nullptr;  // pinned, coherent
+    void *h_stage_dev = nullptr;  // h_stage's device-side address
     char *d_stage = nullptr;
     std::vector<Req *> reqs;  // the step's requests (set by the dispatcher under the lock)
     bool busy = false;        // assigned and not finished (under the lock)
@@ -200,7 +201,10 @@ int gn_server::launch_step(Slot &sl) {
             tl[p++] = GnTile{obj, (int32_t)std::min(T, r->n - off), (int64_t)off, k, 0};
     }
     const size_t bytes = toff + sizeof(GnTile) * (ns + nc);
-    GN_HIP(hipMemcpyAsync(sl.d_stage, sl.h_stage, bytes, hipMemcpyHostToDevice, sl.st));
+    // an ordinary kernel carries the image: hipMemcpyAsync of it has held the calling thread (and
+    // the other slot's launches) for 12-15 ms now and then under load
+    int crc = gn::launch_stage_copy(sl.h_stage_dev, sl.d_stage, bytes, sl.st);
+    if (crc) return crc;
     const GnReq *d_rq = (const GnReq *)sl.d_stage;
     const GnTile *d_tl = (const GnTile *)(sl.d_stage + toff);
     // surface groups (seeds and precision phases) on the slot's side streams, curves on the
@@ -383,7 +387,8 @@ int gn_server_create(gn_batch *batch, gn_curve *const *curves, int ncurves,
     for (Slot &sl : s->slots) {
         if (e == hipSuccess) e = hipStreamCreateWithFlags(&sl.st, hipStreamNonBlocking);
         if (e == hipSuccess) e = hipEventCreateWithFlags(&sl.ev, hipEventDisableTiming);
-        if (e == hipSuccess) e = hipHostMalloc((void **)&sl.h_stage, kStageBytes, hipHostMallocDefault);
+        if (e == hipSuccess) e = hipHostMalloc((void **)&sl.h_stage, kStageBytes, hipHostMallocCoherent);
+        if (e == hipSuccess) e = hipHostGetDevicePointer(&sl.h_stage_dev, sl.h_stage, 0);
         if (e == hipSuccess) e = hipMalloc((void **)&sl.d_stage, kStageBytes);
         if (e == hipSuccess && gn::side_streams_create(&sl.side) != GN_OK) e = hipErrorOutOfMemory;
     }
