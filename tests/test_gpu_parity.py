@@ -243,6 +243,28 @@ def test_invert_mixed_reaches_1e9():
     assert agree[both].mean() > 0.995
 
 
+@pytest.mark.parametrize("newton", [0, G.NEWTON_FP32, G.NEWTON_FP64, G.NEWTON_FP32 | G.NEWTON_FP64])
+def test_invert_mixed_newton_flags_match_oracle(newton):
+    """Every combination of exact-Hessian steps in the FP32 bulk and the FP64 closing pass (the
+    streamed first closing step and its listed rest in both forms) reaches the oracle's foot points
+    from the same greedy-leaf seeds."""
+    sd, gs, os_, u0, v0, P = _projection_case(20000, 1e-2)
+    gs.BuildTree(3)
+    opts = G.default_opts(precision=G.GN_MIXED, newton=newton, fp32_iters=20, fp64_tail=20,
+                          tol_step32=1e-4, tol_step=1e-12,
+                          tol_quad=1e-6 if newton & G.NEWTON_FP64 else 0.0)
+    u, v, st = (_np(t) for t in gs.NewInverse(P[0], P[1], P[2], opts))
+    nodes, _, _ = gs.GetTree()
+    _, ssu, ssv = O.seed_greedy_leaf(nodes, 3, P[0], P[1], P[2])
+    ru, rv, rst, _ = os_.gauss_newton(P[0], P[1], P[2], ssu, ssv, tol_step=1e-12, max_iter=40, newton=1)
+    both = (st == G.PT_CONVERGED) & (rst == O.CONV_STEP)
+    assert both.mean() > (0.95 if newton & G.NEWTON_FP64 else 0.6)
+    d = np.maximum(np.abs(u - ru), np.abs(v - rv))[both]
+    same = d < 1e-6  # the few others reached another local minimum of the distance from the seed
+    assert same.mean() > 0.99
+    assert np.mean(d[same] <= 1e-9) > 0.995
+
+
 def test_invert_plain_nearest_leaf_on_surface_round_trip():
     sd = synth.surface_G(32, 32, 3, 3, seed=1002)
     gs, os_ = _pair(sd)
