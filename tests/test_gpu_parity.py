@@ -343,6 +343,25 @@ def test_result_independent_of_batch_and_order(precision, newton):
         assert np.array_equal(a[sel], b)
 
 
+@pytest.mark.parametrize("precision,newton", [(G.GN_FP64, G.NEWTON_FP64), (G.GN_MIXED, G.NEWTON_FP32 | G.NEWTON_FP64)])
+def test_ragged_sizes_match_the_full_run(precision, newton):
+    """Calls of 1, 63, 64, 65, 257 and 4,097 points (partial 64-point tiles, a grid of one
+    workgroup, one wave holding a few points) give each point the bits it gets in a call of
+    30,000; an empty call is a no-op."""
+    sd, gs, os_, u0, v0, P = _projection_case(30000, 1e-2)
+    gs.BuildTree(3)
+    opts = G.default_opts(precision=precision, newton=newton, tol_quad=1e-6)
+    big = [_np(t) for t in gs.NewInverse(P[0], P[1], P[2], opts, iters=True)]
+    rng = np.random.default_rng(21)
+    for n in (1, 63, 64, 65, 257, 4097):
+        sel = rng.choice(30000, n, replace=False)
+        small = [_np(t) for t in gs.NewInverse(P[0][sel], P[1][sel], P[2][sel], opts, iters=True)]
+        for a, b in zip(big, small):
+            assert np.array_equal(a[sel], b), n
+    e = gs.NewInverse(np.zeros(0), np.zeros(0), np.zeros(0), opts)
+    assert all(t.numel() == 0 for t in e)
+
+
 def test_long_knot_vectors_take_the_global_knot_paths():
     """Knot vectors too long for LDS (603 + 1003 knots): evaluation falls back from the
     LDS-staged kernel and inversion from the exact-degree kernels to the generic kernels that
