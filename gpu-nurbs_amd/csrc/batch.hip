@@ -38,22 +38,37 @@ struct BatchArgs {
 
 // Seeds of every point of the tiles into its request's u, v.  Workgroup b takes the points
 // (b % per_tile) * kSeedBlock .. + kSeedBlock - 1 of tile b / per_tile, one point per lane, so
-// every point has a lane of its own (the seed search is a chain of dependent loads).
+// every point has a lane of its own (the seed search is a chain of dependent loads).  All lanes
+// of a workgroup search one surface, so the greedy form first stages the FP32 boxes of its tree's
+// levels 1..kBatchSeedLdsLevels (340 boxes, 8 KB) in LDS and descends those levels there.
 constexpr int kSeedBlock = 256;
+constexpr int kBatchSeedLdsLevels = 4;
 __global__ __launch_bounds__(kSeedBlock) void k_batch_seed(const GnBatchObj *__restrict__ objs,
                                                            const GnTile *__restrict__ tiles,
                                                            long long nblocks, int per_tile,
                                                            const GnReq *__restrict__ reqs,
                                                            int mode) {
-    for (long long b = blockIdx.x; b < nblocks; b += gridDim.x) {
+    __shared__ __align__(16) float lbox[(((1 << (2 * (kBatchSeedLdsLevels + 1))) - 1) / 3 - 1) * 6];
+    for (long long b = blockIdx.x; b < nblocks; b += gridDim.x) {  // workgroup-uniform
         const GnTile tl = tiles[b / per_tile];
+        const GnSearchDev &sd = objs[tl.obj].sd;
+        const int ll = mode == GN_SEED_NEAREST_LEAF ? 0
+                       : (sd.depth < kBatchSeedLdsLevels ? sd.depth : kBatchSeedLdsLevels);
+        __syncthreads();  // the previous tile's descents are done with lbox
+        if (ll > 0) {
+            // boxes of levels 1..ll start one box (24 bytes) into box32: float2 copies
+            const int nf2 = (int)(qoff(ll + 1) - 1) * 3;
+            const float2 *src = reinterpret_cast<const float2 *>(sd.box32 + 6);
+            float2 *dst = reinterpret_cast<float2 *>(lbox);
+            for (int k = threadIdx.x; k < nf2; k += blockDim.x) dst[k] = src[k];
+        }
+        __syncthreads();
         const int j = (int)(b % per_tile) * kSeedBlock + threadIdx.x;
         if (j >= tl.count) continue;
         const GnReq &R = reqs[tl.req];
-        const GnSearchDev &sd = objs[tl.obj].sd;
         const int64_t i = tl.start + j;
         const double P[3] = {R.x[i], R.y[i], R.z[i]};
-        const uint32_t c = mode == GN_SEED_NEAREST_LEAF ? nearest_leaf(sd, P) : greedy_leaf(sd, P);
+        const uint32_t c = mode == GN_SEED_NEAREST_LEAF ? nearest_leaf(sd, P) : greedy_leaf(sd, P, lbox, ll);
         R.u[i] = sd.seed_uv[c * 2 + 0];
         R.v[i] = sd.seed_uv[c * 2 + 1];
     }
