@@ -6,7 +6,8 @@ same bits, because a point's iteration does not depend on which route or launch 
     points overflows every segment;
   * a 40,000-point subset of it, run alone, fits the list;
   * the subset's results must be bit-identical to its points in the full run, and converged points
-    must be stationary points of the distance (residual orthogonal to both tangents).
+    must be stationary points of the distance (residual orthogonal to both tangents);
+  * calls in flight at once on different streams keep their lists apart.
 """
 import numpy as np
 import pytest
@@ -19,6 +20,43 @@ pytestmark = pytest.mark.gpu
 
 OPTS = dict(precision=G.GN_MIXED, newton=G.NEWTON_FP32 | G.NEWTON_FP64, fp32_iters=1, fp64_tail=10,
             tol_step32=1e-4, tol_step=1e-12, tol_quad=1e-6)
+
+
+def test_concurrent_mixed_calls_on_two_streams():
+    """Two mixed inversions in flight at once on different streams (the same surface and a second
+    one) take separate counter slots and continuation lists: each result is bit-identical to the
+    same call run alone."""
+    n = 200000
+    outs = []
+    surfs, queries = [], []
+    for k, seed in enumerate((1002, 1005)):
+        sd = synth.surface_G(32, 32, 3, 3, seed=seed)
+        gs = G.Surface(sd.p, sd.q, sd.ku, sd.kv, sd.cp, sd.w)
+        gs.BuildTree(3)
+        u0, v0 = synth.uniform_uv(n, seed=seed + 1)
+        ev = gs.Evaluate(u0, v0, 1)
+        nrm = torch.linalg.cross(ev[3:6].T, ev[6:9].T).T
+        nrm = nrm / torch.linalg.norm(nrm, dim=0, keepdim=True)
+        delta = torch.from_numpy(np.random.default_rng(seed + 2).normal(0.0, 1e-2, n)).cuda()
+        P = (ev[0:3] + delta * nrm).contiguous()
+        surfs.append(gs)
+        queries.append(tuple(P[i].contiguous() for i in range(3)))
+    opts = G.default_opts(**dict(OPTS, fp32_iters=20, fp64_tail=3))
+    alone = [surfs[k].NewInverse(*queries[k], opts) for k in (0, 1, 0)]
+    torch.cuda.synchronize()
+    streams = [torch.cuda.Stream() for _ in range(3)]
+    res = []
+    for k, s in zip((0, 1, 0), streams):
+        u = torch.empty(n, dtype=torch.float64, device="cuda")
+        v = torch.empty_like(u)
+        st = torch.empty(n, dtype=torch.uint8, device="cuda")
+        surfs[k].NewInverse(*queries[k], opts, out=(u, v, st), stream=s)
+        res.append((u, v, st))
+    torch.cuda.synchronize()
+    for a, b in zip(alone, res):
+        assert all(torch.equal(x, y) for x, y in zip(a, b))
+    for s in surfs:
+        s.close()
 
 
 def test_list_overflow_matches_listed_path():
