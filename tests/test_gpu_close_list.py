@@ -27,7 +27,6 @@ def test_concurrent_mixed_calls_on_two_streams():
     one) take separate counter slots and continuation lists: each result is bit-identical to the
     same call run alone."""
     n = 200000
-    outs = []
     surfs, queries = [], []
     for k, seed in enumerate((1002, 1005)):
         sd = synth.surface_G(32, 32, 3, 3, seed=seed)
