@@ -85,6 +85,8 @@ __global__ __launch_bounds__(kRowBlock) void k_eval_rows(GnSurfDev d, const TP *
     for (int i = threadIdx.x; i < nsv; i += blockDim.x) sk[nku + nkv + nsu + i] = d.ihv[i];
     __syncthreads();
     const double *ku = sk, *kv = sk + nku, *ihus = sk + nku + nkv, *ihvs = ihus + nsu;
+    const double inv_u = (double)(d.nu - d.p) / (ku[d.nu] - ku[d.p]);
+    const double inv_v = (double)(d.nv - d.q) / (kv[d.nv] - kv[d.q]);
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     float4 *my = stage + (size_t)wv * 64 * Rows::STRIDE;
     const T *tab = sizeof(T) == 8 ? reinterpret_cast<const T *>(d.tab64)
@@ -100,8 +102,8 @@ __global__ __launch_bounds__(kRowBlock) void k_eval_rows(GnSurfDev d, const TP *
         const double uu = live ? (double)pu : 0.0, vv = live ? (double)pv : 0.0;
         const size_t inext = base + waves * 64 + lane;
         if (inext < n) { pu = u[inext]; pv = v[inext]; }
-        const int m = find_span(ku, d.p, d.nu, uu);
-        const int nn = find_span(kv, d.q, d.nv, vv);
+        const int m = find_span_guess(ku, d.p, d.nu, uu, inv_u);
+        const int nn = find_span_guess(kv, d.q, d.nv, vv, inv_v);
         const int im = m - d.p, in = nn - d.q;
         const T *patch = tab + (size_t)(im * d.sv + in) * (K1 * K2 * 4);
         Rows rows(patch, my);

@@ -21,6 +21,26 @@ __device__ __forceinline__ int find_span(const double *__restrict__ k, int p, in
     return lo;
 }
 
+// find_span with a first guess: the span that u would fall in if the knots were uniform over the
+// domain (inv = (cp - p) / (k[cp] - k[p])).  The guess is taken only when it brackets u, and the
+// bracketing span is unique, so the result is find_span's in every case; on uniform knots the
+// bisection is skipped.
+__device__ __forceinline__ int find_span_guess(const double *__restrict__ k, int p, int cp,
+                                               double u, double inv) {
+    if (u >= k[cp]) return cp - 1;
+    if (u <= k[p]) return p;
+    if (u != u) return (p + cp) >> 1;
+    int m = p + (int)((u - k[p]) * inv);
+    m = m < p ? p : (m > cp - 1 ? cp - 1 : m);
+    if (k[m] <= u && u < k[m + 1]) return m;
+    int lo = p, hi = cp;
+    while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        if (u < k[mid]) hi = mid; else lo = mid;
+    }
+    return lo;
+}
+
 template <typename T>
 struct Hom {  // homogeneous 4-vector
     T x, y, z, w;
