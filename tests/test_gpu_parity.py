@@ -104,6 +104,35 @@ def test_eval_fp64(name, mk, order):
         assert ok, (name, key, err)
 
 
+def test_eval_repeated_interior_knots_at_and_beside_knots():
+    """Interior knots of multiplicity 2 and 3 (empty spans between the copies) and non-uniform
+    spacing: at every knot and one ulp either side, every field to order 3 matches the oracle,
+    whose span search is the reference's (the kernel's search starts from a uniform-knot guess)."""
+    p = q = 3
+    ku = np.r_[[0.0] * 4, 0.1, 0.3, 0.3, 0.55, 0.55, 0.55, 0.8, [1.0] * 4]
+    kv = np.r_[[0.0] * 4, 0.25, 0.25, 0.6, 0.9, [1.0] * 4]
+    nu, nv = len(ku) - p - 1, len(kv) - q - 1
+    rng = np.random.default_rng(31)
+    X, Y = np.meshgrid(np.linspace(0, 1, nu), np.linspace(0, 1, nv), indexing="ij")
+    cp = np.stack([X, Y, 0.2 * rng.standard_normal((nu, nv))], axis=-1)
+    w = rng.uniform(0.5, 2.0, (nu, nv))
+    sd = synth.SurfaceDef(p, q, ku, kv, cp, w)
+    gs, os_ = _pair(sd)
+    pts = []
+    for k in np.unique(np.r_[ku, kv]):
+        pts += [np.nextafter(k, -np.inf), k, np.nextafter(k, np.inf)]
+    pts = np.array(pts)
+    U, V = np.meshgrid(pts, pts, indexing="ij")
+    u, v = U.ravel(), V.ravel()
+    got = blocks(_np(gs.Evaluate(u, v, 3)))
+    ref = blocks(os_.eval(u, v, 3))
+    floor = 1e-2 * np.max(np.abs(ref["S"]))
+    for key in FIELDS:
+        ok, err = field_close(got[key], ref[key], 1e-12, floor)
+        assert ok, (key, err)
+    gs.close()
+
+
 @pytest.mark.parametrize("order", [2, 3])
 @pytest.mark.parametrize("name,mk", SURFS)
 def test_eval_fp32(name, mk, order):
