@@ -453,3 +453,25 @@ def test_zero_iteration_limit_returns_the_start_point(prec):
         assert np.percentile(err, 99) < 1e-3 and np.median(err) > 1e-10
         assert np.median(ferr) < 1e-12
         assert np.mean(st == G.PT_MAXITER) > 0.99  # points whose FP32 bulk broke down restart
+
+
+def test_seed_greedy_leaf_independent_of_batch_size():
+    """A query's greedy-leaf seed does not depend on the other queries of the call: prefixes of
+    ragged lengths (where threads carry an odd number of points, or none past n) give the
+    prefix of the full call's leaves and seeds, bit for bit."""
+    import torch
+    sd = synth.surface_G(32, 32, 3, 3, seed=1002)
+    gs = G.Surface(sd.p, sd.q, sd.ku, sd.kv, sd.cp, sd.w)
+    gs.BuildTree(3)  # depth 6: levels past the ones staged in LDS are read from the tree
+    N = 600001
+    u0, v0 = synth.uniform_uv(N, seed=33)
+    ev = gs.Evaluate(u0, v0, 1)
+    nrm = torch.linalg.cross(ev[3:6].T, ev[6:9].T).T
+    nrm = nrm / torch.linalg.norm(nrm, dim=0, keepdim=True)
+    delta = torch.from_numpy(np.random.default_rng(34).normal(0.0, 1e-2, N)).cuda()
+    P = (ev[0:3] + delta * nrm).contiguous()
+    full = gs.Seed(P[0].contiguous(), P[1].contiguous(), P[2].contiguous(), exact=False)
+    for m in (1, 3, 255, 257, 4097, 262143, 262145, 524289, 524290):
+        part = gs.Seed(P[0, :m].contiguous(), P[1, :m].contiguous(), P[2, :m].contiguous(), exact=False)
+        assert all(torch.equal(a, b[:m]) for a, b in zip(part, full)), m
+    gs.close()
