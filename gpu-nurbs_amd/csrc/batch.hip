@@ -377,7 +377,8 @@ static int run_range(const GnBatchObj *objs, const GnTile *tiles, const GroupRan
 static int launch_ranges(const GnBatchObj *objs, const GnTile *tiles,
                          const std::vector<GroupRange> &ranges, const GnReq *reqs,
                          const gn_invert_opts &o, int bpc_cap, hipStream_t st,
-                         const SideStreams &ss, const std::function<int(hipStream_t)> &beside) {
+                         const SideStreams &ss, const std::function<int(hipStream_t)> &beside,
+                         std::vector<std::pair<int, hipEvent_t>> *group_done) {
     if (ranges.size() == 1 && !beside)  // one group, nothing beside it: no fork
         return run_range(objs, tiles, ranges[0], reqs, o, bpc_cap, st);
     if (ranges.size() > (size_t)kSideStreams)
@@ -390,6 +391,7 @@ static int launch_ranges(const GnBatchObj *objs, const GnTile *tiles,
         const int rc = run_range(objs, tiles, gr, reqs, o, bpc_cap, ss.s[k]);
         if (rc) return rc;
         GN_HIP(hipEventRecord(ss.join[k], ss.s[k]));
+        if (group_done) group_done->emplace_back(gr.grp, ss.join[k]);
         k++;
     }
     if (beside) {
@@ -418,16 +420,18 @@ int batch_prepare() {
 int launch_batch_invert(const GnBatchObj *objs, const GnTile *tiles,
                         const std::vector<GroupRange> &ranges, const GnReq *reqs,
                         const gn_invert_opts &o, int bpc_cap, hipStream_t st,
-                        const SideStreams *side, const std::function<int(hipStream_t)> &beside) {
+                        const SideStreams *side, const std::function<int(hipStream_t)> &beside,
+                        std::vector<std::pair<int, hipEvent_t>> *group_done) {
     if (ranges.empty()) return beside ? beside(st) : GN_OK;
-    if (side) return launch_ranges(objs, tiles, ranges, reqs, o, bpc_cap, st, *side, beside);
+    if (side)
+        return launch_ranges(objs, tiles, ranges, reqs, o, bpc_cap, st, *side, beside, group_done);
     int dev = 0;
     GN_HIP(hipGetDevice(&dev));
     if (dev < 0 || dev >= 64) return fail(GN_EDEVICE, "device index out of range");
     std::lock_guard<std::mutex> lk(g_side_mu);  // one user of the shared set at a time
     const int rc = shared_side_locked(dev);
     if (rc) return rc;
-    return launch_ranges(objs, tiles, ranges, reqs, o, bpc_cap, st, g_side[dev], beside);
+    return launch_ranges(objs, tiles, ranges, reqs, o, bpc_cap, st, g_side[dev], beside, group_done);
 }
 
 }  // namespace gn

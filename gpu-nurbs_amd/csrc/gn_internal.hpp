@@ -6,6 +6,7 @@
 #include <cstdint>
 #include <functional>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "gnurbs.h"
@@ -212,12 +213,16 @@ void side_streams_destroy(SideStreams *ss);
 // Per group range: seeds (unless GN_SEED_GIVEN), then one persistent launch per precision
 // phase.  The groups run concurrently on the side streams (forked from and joined back into st);
 // `beside`, if given, enqueues further work on st that runs alongside them.  With side ==
-// nullptr the device's shared set is used.
+// nullptr the device's shared set is used.  group_done, if given, receives (group, event) for
+// every group that ran on a side stream: the event completes when that group's work has (it is
+// the side set's join event, valid until the set's next use); a group run on st itself is not
+// listed.
 int launch_batch_invert(const GnBatchObj *objs, const GnTile *tiles,
                         const std::vector<GroupRange> &ranges, const GnReq *reqs,
                         const gn_invert_opts &o, int bpc_cap, hipStream_t st,
                         const SideStreams *side = nullptr,
-                        const std::function<int(hipStream_t)> &beside = {});
+                        const std::function<int(hipStream_t)> &beside = {},
+                        std::vector<std::pair<int, hipEvent_t>> *group_done = nullptr);
 // resident waves of one persistent batch launch (sizes tiles for short serving steps)
 int batch_waves(int bpc_cap);
 // tiles of curve objects (obj indexes `curves`); max_samples: largest seed sample count

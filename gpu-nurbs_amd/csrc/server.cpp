@@ -5,7 +5,8 @@
 // request that is not yet running to the next free slot.  The slot's thread builds the step (its
 // request table and tile lists, written in class order straight into pinned staging), uploads it
 // with one copy, launches the class groups and curves on the slot's streams, and polls the
-// step's event; up to `streams` steps are in flight at once, so requests arriving while a step
+// groups' events and the step's event (a class group's requests complete when that group's
+// kernels have, curves with the step); up to `streams` steps are in flight at once, so requests arriving while a step
 // runs are gathered into the next one as soon as a slot frees, and one slot's launch work never
 // delays another slot's completion.  Every point is computed by the same per-point iteration
 // whatever else shares its step, so a request's results are the same alone and in a mix.
@@ -63,7 +64,8 @@ struct Slot {
     char *h_stage = nullptr;  // pinned, coherent
     void *h_stage_dev = nullptr;  // h_stage's device-side address
     char *d_stage = nullptr;
-    std::vector<Req *> reqs;  // the step's requests (set by the dispatcher under the lock)
+    std::vector<Req *> reqs;  // the step's unfinished requests (set by the dispatcher under the lock)
+    std::vector<std::pair<int, hipEvent_t>> group_done;  // class groups of the step on side streams
     bool busy = false;        // assigned and not finished (under the lock)
     bool go = false;          // a new assignment for the slot's thread (under the lock)
     std::condition_variable cv;
@@ -109,6 +111,7 @@ struct gn_server {
     void run();
     void run_slot(Slot &sl);
     int launch_step(Slot &sl);
+    void finish_group(Slot &sl, int grp);
     void finish(Slot &sl, bool ok);
 };
 
@@ -215,14 +218,38 @@ int gn_server::launch_step(Slot &sl) {
             return gn::launch_curve_batch(d_curves, d_tl + ns, (int)nc, d_rq, max_samples, o, cs);
         };
     int rc = GN_OK;
+    sl.group_done.clear();
     if (ns)
         rc = gn::launch_batch_invert(d_objs, d_tl, gn::group_ranges(tl, ns, cls), d_rq, o, bpc,
-                                     sl.st, &sl.side, curves);
+                                     sl.st, &sl.side, curves, &sl.group_done);
     else if (nc)
         rc = curves(sl.st);
     if (rc) return rc;
     GN_HIP(hipEventRecord(sl.ev, sl.st));
     return GN_OK;
+}
+
+// Completes the step's surface requests of class group grp, whose work has finished while the
+// rest of the step runs on: they leave sl.reqs (a client may release them from now on; the
+// step's remaining kernels touch only the other requests' buffers).
+void gn_server::finish_group(Slot &sl, int grp) {
+    const int64_t now = gn_now_ns_impl();
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        size_t keep = 0;
+        for (Req *r : sl.reqs) {
+            if (r->curve || gn::class_group(cls[r->obj]) != grp) {
+                sl.reqs[keep++] = r;
+                continue;
+            }
+            r->done_ns = now;
+            r->state = kDone;
+            stats.completed++;
+            stats.points += (int64_t)r->n;
+        }
+        sl.reqs.resize(keep);
+    }
+    cv_done.notify_all();
 }
 
 void gn_server::finish(Slot &sl, bool ok) {
@@ -258,8 +285,25 @@ void gn_server::run_slot(Slot &sl) {
         if (!ok) {
             (void)hipStreamSynchronize(sl.st);
         } else {
+            // each class group's requests complete when that group's work has; the rest (and
+            // everything after an error) with the step's event
             hipError_t q;
-            while ((q = hipEventQuery(sl.ev)) == hipErrorNotReady) std::this_thread::yield();
+            size_t open = sl.group_done.size();
+            while ((q = hipEventQuery(sl.ev)) == hipErrorNotReady) {
+                for (auto &g : sl.group_done) {
+                    if (!g.second) continue;
+                    const hipError_t e = hipEventQuery(g.second);
+                    if (e == hipErrorNotReady) continue;
+                    if (e == hipSuccess) finish_group(sl, g.first);
+                    g.second = nullptr;
+                    open--;
+                }
+                if (!open) {
+                    while ((q = hipEventQuery(sl.ev)) == hipErrorNotReady) std::this_thread::yield();
+                    break;
+                }
+                std::this_thread::yield();
+            }
             ok = q == hipSuccess;
         }
         finish(sl, ok);
