@@ -131,8 +131,9 @@ def test_admission_within_budget(world):
         for k, rid in enumerate(ids):
             res, info = fetch(srv, rid)  # releasing lets the next ones in
             assert info.admit_ns >= info.submit_ns and info.done_ns >= info.launch_ns >= info.admit_ns
-            if k in (0, 29):
-                assert same(res, direct(world, k % 12, Ps[k], opts))
+            # a request completes with its class group, so it may be released (and its arena range
+            # reused) while the rest of its step still runs: every result must stay intact
+            assert same(res, direct(world, k % 12, Ps[k], opts)), k
             done += 1
         st = srv.stats()
         assert done == 30 and st.completed == 30
