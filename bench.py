@@ -191,7 +191,7 @@ def bench_batch(G, synth, torch, steps, count=1024):
     v = torch.empty_like(u)
     st = torch.empty(n, dtype=torch.uint8, device="cuda")
     mixed = dict(precision=G.GN_MIXED, newton=G.NEWTON_FP32 | G.NEWTON_FP64, fp32_iters=20, fp64_tail=3,
-                 tol_step32=1e-4, tol_step=1e-10, tol_quad=1e-6)
+                 tol_step32=1e-4, tol_step=1e-10, tol_quad=1e-10)
     shape = b.tune(segs, x, y, z, G.default_opts(**mixed), out=(u, v, st))
     stream = torch.cuda.Stream()
     res = {"workload": f"{count} surfaces (degrees 2..5, nets 4..48, jittered knots), {n} points "
@@ -337,7 +337,7 @@ def bench_serve(G, synth, torch, duration=1.0, loads=(0.5, 0.8), budget=32 << 30
                 "in_step_points_per_s": float(1.0 / b) if b > 0 else None}
 
     mixed = dict(precision=G.GN_MIXED, newton=G.NEWTON_FP32 | G.NEWTON_FP64, fp32_iters=20, fp64_tail=3,
-                 tol_step32=1e-4, tol_step=1e-10, tol_quad=1e-6)
+                 tol_step32=1e-4, tol_step=1e-10, tol_quad=1e-10)
     plain = dict(precision=G.GN_FP64, max_iter=20, tol_step=1e-10)
     try:
         for name, opts in (("mixed", mixed), ("plain_fp64", plain)):
@@ -449,14 +449,14 @@ def main():
     # mixed: FP32 exact-Hessian bulk to a 1e-4 step, then FP64 exact-Hessian closing steps (<= 3)
     # stopping at a 1e-6 step (error after the step O(step^2)) or 1e-10
     mixed_opts = dict(precision=G.GN_MIXED, newton=G.NEWTON_FP32 | G.NEWTON_FP64, fp32_iters=20,
-                      fp64_tail=3, tol_step32=1e-4, tol_step=1e-10, tol_quad=1e-6)
+                      fp64_tail=3, tol_step32=1e-4, tol_step=1e-10, tol_quad=1e-10)
     variants = {
         "mixed": G.default_opts(**mixed_opts),
         # the reference iteration (J^T J Gauss-Newton, src/kernel/kernel.cu:151-249) in FP64
         "plain_fp64": G.default_opts(precision=G.GN_FP64, newton=0, max_iter=20, tol_step=1e-10),
         # the mixed mode's algorithm run entirely in FP64, to separate algorithm from precision
         "fp64_newton": G.default_opts(precision=G.GN_FP64, newton=G.NEWTON_FP64, max_iter=20, tol_step=1e-10,
-                                      tol_quad=1e-6),
+                                      tol_quad=1e-10),
     }
 
     def run(opts):

@@ -53,7 +53,7 @@ void gn_invert_opts_default(gn_invert_opts *o) {
     o->tol_step32 = 1e-6;
     o->seed_mode = GN_SEED_GREEDY_LEAF;
     o->newton = 0;
-    o->tol_quad = 0.0;
+    o->tol_quad = 1e-10;
     o->fallback = 0;
 }
 

@@ -45,6 +45,7 @@ __global__ void k_curve_table(GnCurveDev c, const double *__restrict__ xyz,
 template <typename T>
 struct CurvePt {
     T C[3], C1[3], C2[3];
+    T ih;  // 1 / length of the span evaluated
 };
 
 template <int ORDER, typename T>
@@ -75,6 +76,7 @@ __device__ __forceinline__ void curve_eval(const GnCurveDev &c, double t, CurveP
         R.x = fma(R.x, s, a.x); R.y = fma(R.y, s, a.y); R.z = fma(R.z, s, a.z); R.w = fma(R.w, s, a.w);
     }
     const T iw = recip(R.w), h1 = T(ih);
+    o.ih = h1;
     const T A[3] = {R.x, R.y, R.z}, A1[3] = {R1.x, R1.y, R1.z}, A2[3] = {R2.x, R2.y, R2.z};
 #pragma unroll
     for (int q = 0; q < 3; q++) o.C[q] = A[q] * iw;
@@ -150,10 +152,15 @@ __device__ int curve_loop(const GnCurveDev &c, const double P[3], double &t, int
         if (!(rr == rr)) { st = GN_PT_NONFINITE; break; }
         if (tol_dist > 0 && sqrt(rr) < T(tol_dist)) { st = GN_PT_CONVERGED_DIST; break; }
         const T f = o.C1[0] * r0 + o.C1[1] * r1 + o.C1[2] * r2;
-        T fp = o.C1[0] * o.C1[0] + o.C1[1] * o.C1[1] + o.C1[2] * o.C1[2];
+        const T c1 = o.C1[0] * o.C1[0] + o.C1[1] * o.C1[1] + o.C1[2] * o.C1[2];
+        T fp = c1;
+        bool exact = false;  // Newton step (exact second derivative of |C-P|^2/2)
         if (NEWTON) {
             const T f2 = fp + o.C2[0] * r0 + o.C2[1] * r1 + o.C2[2] * r2;
-            if (f2 > T(0)) fp = f2;
+            if (f2 > T(0)) {
+                fp = f2;
+                exact = true;
+            }
         }
         if (!(fp == fp)) { st = GN_PT_NONFINITE; break; }
         if (fp == T(0)) { st = GN_PT_SINGULAR; break; }
@@ -163,7 +170,17 @@ __device__ int curve_loop(const GnCurveDev &c, const double P[3], double &t, int
         it++;
         if (!(t == t)) { st = GN_PT_NONFINITE; break; }
         if (tol_step > 0 && fabs(dt) < T(tol_step)) { st = GN_PT_CONVERGED; break; }
-        if (NEWTON && tol_quad > 0 && fabs(dt) < T(tol_quad)) { st = GN_PT_CONVERGED; break; }
+        // certificate of a Newton step dt: the error left is at most L dt^2 / (2 f''), with
+        // L = 3 |C'| |C''| + |r| |C'''| bounding f''' and |C'''| taken as |C''| (p+1)/h (one more
+        // derivative of the span's polynomial); squared, with L^2 <= 2 |C''|^2 (9 |C'|^2 + |r|^2 m^2)
+        if (NEWTON && exact && tol_quad > 0) {
+            const T c2 = o.C2[0] * o.C2[0] + o.C2[1] * o.C2[1] + o.C2[2] * o.C2[2];
+            const T m = T(c.p + 1) * o.ih, t2 = dt * dt, tq = T(tol_quad);
+            if (T(0.5) * c2 * (T(9) * c1 + rr * m * m) * (t2 * t2) <= tq * tq * fp * fp) {
+                st = GN_PT_CONVERGED;
+                break;
+            }
+        }
     }
     iters = it;
     return st;

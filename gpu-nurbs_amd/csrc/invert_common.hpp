@@ -143,6 +143,7 @@ __device__ __forceinline__ int gn_step(const GnSurfDev &d, const double *__restr
     T a = o.Su[0] * o.Su[0] + o.Su[1] * o.Su[1] + o.Su[2] * o.Su[2];
     T b = o.Su[0] * o.Sv[0] + o.Su[1] * o.Sv[1] + o.Su[2] * o.Sv[2];
     T c = o.Sv[0] * o.Sv[0] + o.Sv[1] * o.Sv[1] + o.Sv[2] * o.Sv[2];
+    const T j2 = a + c;  // |J|_F^2
     const T gu = o.Su[0] * r0 + o.Su[1] * r1 + o.Su[2] * r2;
     const T gv = o.Sv[0] * r0 + o.Sv[1] * r1 + o.Sv[2] * r2;
     T a1 = a, c1 = c;    // one-dimensional curvatures for a held coordinate (ACTIVE)
@@ -187,11 +188,26 @@ __device__ __forceinline__ int gn_step(const GnSurfDev &d, const double *__restr
     }
     if (!(cu == cu) || !(cv == cv)) return GN_PT_NONFINITE;
     if (tol_step > 0 && fabs(du) < T(tol_step) && fabs(dv) < T(tol_step)) return GN_PT_CONVERGED;
-    // the O(step^2) certificate holds only for an exact-Hessian step; where the Hessian is not
-    // positive definite (a saddle of the distance: the query is farther than the curvature radius)
-    // the Gauss-Newton step converges linearly and only tol_step stops it
-    if (NEWTON && hexact && tol_quad > 0 && fabs(du) < T(tol_quad) && fabs(dv) < T(tol_quad))
-        return GN_PT_CONVERGED;
+    // Certificate of an exact-Hessian (Newton) step: the step d leaves an error of at most
+    // |H^-1| L |d|^2 / 2, where L bounds the third derivative of f = |S-P|^2/2,
+    // |f'''| <= 3 |J| |S''| + |r| |S'''|, and |S'''| is taken as |S''| (k1/h_u + k2/h_v) (one more
+    // derivative of the span's polynomial pieces).  |H^-1| = 1/lambda_min >= (a+c)/det.  The point
+    // is certified when that bound is below tol_quad; squared so that no square root is needed:
+    //   (|S''|^2 (3|J| + |r| m)^2 <= 2 |S''|^2 (9 |J|^2 + |r|^2 m^2))
+    //   0.5 |S''|^2 (9 |J|^2 + |r|^2 m^2) |d|^4 (a+c)^2 <= tol_quad^2 det^2.
+    // A flat minimum (small lambda_min) therefore needs a smaller step than a well-conditioned one.
+    // Where the Hessian is not positive definite (a saddle of the distance: the query is farther
+    // than the curvature radius) the Gauss-Newton step converges linearly and only tol_step stops it.
+    if (NEWTON && hexact && tol_quad > 0) {
+        const T h2 = o.Suu[0] * o.Suu[0] + o.Suu[1] * o.Suu[1] + o.Suu[2] * o.Suu[2] +
+                     T(2) * (o.Suv[0] * o.Suv[0] + o.Suv[1] * o.Suv[1] + o.Suv[2] * o.Suv[2]) +
+                     o.Svv[0] * o.Svv[0] + o.Svv[1] * o.Svv[1] + o.Svv[2] * o.Svv[2];
+        const T m = T((d.p + 1) * sc.ihu + (d.q + 1) * sc.ihv);
+        const T d2 = du * du + dv * dv, tr = a + c;
+        const T lhs = T(0.5) * h2 * (T(9) * j2 + rr * m * m) * (d2 * d2) * (tr * tr);
+        const T tq = T(tol_quad);
+        if (lhs <= tq * tq * det * det) return GN_PT_CONVERGED;
+    }
     return kStepContinue;
 }
 

@@ -86,9 +86,13 @@ typedef struct {
     int newton;         /* bitmask of GN_NEWTON_*: iterations that add the residual-curvature  */
                         /* term sum_k r_k d2S_k to J^T J (exact Hessian of |S-P|^2/2, falling  */
                         /* back to J^T J where that is not positive definite)                  */
-    double tol_quad;    /* FP64 exact-Hessian steps also stop (converged) once the step is     */
-                        /* below tol_quad: the error left after such a step is O(step^2)      */
-                        /* (not applied to a step that fell back to J^T J); <= 0 off           */
+    double tol_quad;    /* FP64 exact-Hessian (Newton) steps also stop (converged) once a      */
+                        /* bound on the (u,v) error the step leaves is below tol_quad: the     */
+                        /* bound is |H^-1| L |step|^2 / 2 with L bounding the third derivative */
+                        /* of |S-P|^2/2 from S', S'' and the span length, so a flat minimum    */
+                        /* (small eigenvalue of H) needs a smaller step; points that never     */
+                        /* meet it or tol_step end GN_PT_MAXITER.  Not applied to a step that   */
+                        /* fell back to J^T J.  Default 1e-10; <= 0 off                        */
     int fallback;       /* gn_invert: points that end unconverged are solved again by the      */
                         /* global projection (GN_SEED_GLOBAL, FP64, same max_iter / tol_step), */
                         /* keeping the nearer of the two results -- the role of the legacy     */
@@ -101,8 +105,8 @@ enum { GN_NEWTON_FP32 = 1, /* FP32 bulk iterations of GN_MIXED                  
 };
 
 /* Defaults: the reference iteration in plain FP64 (tol_dist 0, tol_step 1e-10, max_iter 20,
- * clamp 1, greedy-leaf seeds, newton 0); mixed: <= 20 FP32 iterations to a 1e-6 step, then
- * <= 3 FP64 closing iterations. */
+ * clamp 1, greedy-leaf seeds, newton 0, tol_quad 1e-10 for Newton steps); mixed: <= 20 FP32
+ * iterations to a 1e-6 step, then <= 3 FP64 closing iterations. */
 void gn_invert_opts_default(gn_invert_opts *opts);
 
 const char *gn_last_error(void);       /* message of the last failure on this thread */

@@ -431,6 +431,11 @@ typedef struct {
                         (exact curvature where positive); a step after which the squared
                         distance grew by more than 1e-6 relative is halved (up to 8 times in a
                         row); the nearest accepted point is kept */
+    double tol_quad; /* certificate of an exact-Hessian step (the library's rule, not in the
+                        reference): stop when |H^-1| L |d|^2 / 2 <= tol_quad, L = |S''|
+                        (3 |J| + |r| (k1/h_u + k2/h_v)), evaluated in the squared form
+                        0.5 |S''|^2 (9 |J|^2 + |r|^2 m^2) |d|^4 (a+c)^2 <= tol_quad^2 det^2
+                        (<= 0: off) */
 } or_gn_opts;
 
 /* Returns status; (u,v) is updated in place; *iters gets the number of iterations begun. */
@@ -473,6 +478,8 @@ static int gn_one_in(const or_surf *sf, const double P[3], const or_gn_opts *op,
             gv += o.Sv[k] * d[k];
         }
         double a1 = a, c1 = c; /* one-dimensional curvatures for a held coordinate */
+        double j2 = a + c;
+        int hexact = 0;
         if (op->newton) {
             double a2 = a, b2 = b, c2 = c;
             for (int k = 0; k < 3; k++) {
@@ -482,7 +489,7 @@ static int gn_one_in(const or_surf *sf, const double P[3], const or_gn_opts *op,
             }
             if (a2 > 0) a1 = a2;
             if (c2 > 0) c1 = c2;
-            if (a2 > 0 && a2 * c2 - b2 * b2 > 0) { a = a2; b = b2; c = c2; }
+            if (a2 > 0 && a2 * c2 - b2 * b2 > 0) { a = a2; b = b2; c = c2; hexact = 1; }
         }
         double det = a * c - b * b;
         if (!(det == det)) { st = OR_NONFINITE; break; }
@@ -507,6 +514,20 @@ static int gn_one_in(const or_surf *sf, const double P[3], const or_gn_opts *op,
             it++;
             break;
         }
+        if (op->newton && hexact && op->tol_quad > 0) {
+            double h2 = 0;
+            for (int k = 0; k < 3; k++)
+                h2 += o.Suu[k] * o.Suu[k] + 2 * o.Suv[k] * o.Suv[k] + o.Svv[k] * o.Svv[k];
+            double m = (sf->p + 1) / (sf->ku[o.m + 1] - sf->ku[o.m]) +
+                       (sf->q + 1) / (sf->kv[o.n + 1] - sf->kv[o.n]);
+            double d2 = du * du + dv * dv, tr = a + c;
+            if (0.5 * h2 * (9 * j2 + rr * m * m) * d2 * d2 * tr * tr <=
+                op->tol_quad * op->tol_quad * det * det) {
+                st = OR_CONV_STEP;
+                it++;
+                break;
+            }
+        }
     }
     if (op->active) { /* the final step did not help: keep the last accepted point */
         or_pt o;
@@ -530,11 +551,11 @@ static int gn_one(const or_surf *sf, const double P[3], const or_gn_opts *op, do
 void or_gauss_newton(int p, int q, int nu, int nv, const double *ku, const double *kv,
                      const double *MatrixN, const double *x, const double *y, const double *z,
                      long n, double tol_dist, double tol_step, int max_iter, int clamp, int newton,
-                     const double *seed_u, const double *seed_v, double *u_out, double *v_out,
-                     unsigned char *status, int *iters, int nthreads)
+                     double tol_quad, const double *seed_u, const double *seed_v, double *u_out,
+                     double *v_out, unsigned char *status, int *iters, int nthreads)
 {
     or_surf sf = {p, q, nu, nv, ku, kv, MatrixN};
-    or_gn_opts op = {tol_dist, tol_step, max_iter, clamp, newton, 0};
+    or_gn_opts op = {tol_dist, tol_step, max_iter, clamp, newton, 0, tol_quad};
 #ifdef _OPENMP
     if (nthreads <= 0) nthreads = omp_get_max_threads();
 #pragma omp parallel for schedule(dynamic, 256) num_threads(nthreads)
@@ -672,7 +693,7 @@ void or_invert_first_hit(int p, int q, int nu, int nv, const double *ku, const d
                          int nthreads)
 {
     or_surf sf = {p, q, nu, nv, ku, kv, MatrixN};
-    or_gn_opts op = {tol_dist, tol_step, max_iter, clamp, 0, 0};
+    or_gn_opts op = {tol_dist, tol_step, max_iter, clamp, 0, 0, 0.0};
 #ifdef _OPENMP
     if (nthreads <= 0) nthreads = omp_get_max_threads();
 #pragma omp parallel for schedule(dynamic, 64) num_threads(nthreads)
@@ -863,7 +884,8 @@ void or_project_global(int p, int q, int nu, int nv, const double *ku, const dou
 {
     or_surf sf = {p, q, nu, nv, ku, kv, MatrixN};
     long base = or_tree_base(levels - 1), side = 1L << (3 * (levels - 1));
-    or_gn_opts full = {0.0, tol_step, max_iter, 1, 1, 1}, leaf = {0.0, tol_step, leaf_iter, 1, 1, 1};
+    or_gn_opts full = {0.0, tol_step, max_iter, 1, 1, 1, 0.0},
+               leaf = {0.0, tol_step, leaf_iter, 1, 1, 1, 0.0};
 #ifdef _OPENMP
     if (nthreads <= 0) nthreads = omp_get_max_threads();
 #pragma omp parallel for schedule(dynamic, 16) num_threads(nthreads)

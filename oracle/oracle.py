@@ -41,7 +41,7 @@ def lib():
         L.or_eval.argtypes = [_I, _I, _I, _I, _P, _P, _P, _P, _P, _L, _I, _P, _I]
         L.or_eval_homogeneous.argtypes = [_I, _I, _I, _I, _P, _P, _P, _P, _P, _L, _P]
         L.or_gauss_newton.argtypes = [_I, _I, _I, _I, _P, _P, _P, _P, _P, _P, _L, _D, _D, _I, _I, _I,
-                                      _P, _P, _P, _P, _P, _P, _I]
+                                      _D, _P, _P, _P, _P, _P, _P, _I]
         L.or_tree_base.restype = _L
         L.or_tree_base.argtypes = [_I]
         L.or_need_divide.restype = _I
@@ -125,14 +125,14 @@ class OSurf:
         return out
 
     def gauss_newton(self, x, y, z, seed_u, seed_v, tol_dist=0.0, tol_step=1e-10, max_iter=20, clamp=1,
-                     newton=0, nthreads=0):
+                     newton=0, nthreads=0, tol_quad=0.0):
         x, y, z, su, sv = (_a(a) for a in (x, y, z, seed_u, seed_v))
         n = x.size
         u, v = np.zeros(n), np.zeros(n)
         st = np.zeros(n, dtype=np.uint8)
         it = np.zeros(n, dtype=np.int32)
         lib().or_gauss_newton(*self._args(), x.ctypes.data, y.ctypes.data, z.ctypes.data, n, tol_dist, tol_step,
-                              max_iter, clamp, newton, su.ctypes.data, sv.ctypes.data, u.ctypes.data,
+                              max_iter, clamp, newton, tol_quad, su.ctypes.data, sv.ctypes.data, u.ctypes.data,
                               v.ctypes.data, st.ctypes.data, it.ctypes.data, nthreads)
         return u, v, st, it
 

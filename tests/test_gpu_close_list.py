@@ -19,7 +19,7 @@ from gnurbs import synth
 pytestmark = pytest.mark.gpu
 
 OPTS = dict(precision=G.GN_MIXED, newton=G.NEWTON_FP32 | G.NEWTON_FP64, fp32_iters=1, fp64_tail=10,
-            tol_step32=1e-4, tol_step=1e-12, tol_quad=1e-6)
+            tol_step32=1e-4, tol_step=1e-12, tol_quad=1e-10)
 
 
 def test_concurrent_mixed_calls_on_two_streams():
@@ -86,10 +86,10 @@ def test_list_overflow_matches_listed_path():
     e = gs.Evaluate(u, v, 1)
     r = e[0:3] - P
     inner = conv & (u > 0) & (u < 1) & (v > 0) & (v < 1)
-    # converged through the 1e-6 quadratic certificate: within 1e-9 but for a few 1e-9 at flat
-    # minima (an unconverged point would be 1e-6 or more off)
+    # converged through the Newton-step certificate, which bounds the error left by the
+    # conditioning of the Hessian: stationary within 1e-9 everywhere, flat minima included (an
+    # unconverged point would be 1e-6 or more off)
     for t in (e[3:6], e[6:9]):
         off = (t * r).sum(0).abs() / (t * t).sum(0)
-        assert (off[inner] < 1e-9).float().mean().item() > 0.9999
-        assert off[inner].max().item() < 1e-8
+        assert off[inner].max().item() < 1e-9
     gs.close()

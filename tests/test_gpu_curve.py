@@ -94,10 +94,13 @@ def test_mixed_round_trip(newton):
     rng = np.random.default_rng(3)
     ts = rng.uniform(0, 1, 100000)
     P = o.eval(ts)
-    opts = G.default_opts(precision=G.GN_MIXED, newton=newton, tol_quad=1e-6 if newton else 0.0)
+    opts = G.default_opts(precision=G.GN_MIXED, newton=newton, tol_quad=1e-10 if newton else 0.0)
     t, st = g.Invert(*P, opts)
     err = np.abs(t.cpu().numpy() - ts)
-    assert (err <= 1e-9).mean() > 0.99
+    conv = st.cpu().numpy() == G.PT_CONVERGED
+    same = conv & (err < 1e-6)  # converged to the sampled parameter, not another local minimum
+    assert same.mean() > 0.99
+    assert np.max(err[same]) <= 1e-9
 
 
 def test_circle_foot_points():

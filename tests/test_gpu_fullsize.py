@@ -5,9 +5,10 @@ random subset; the rest is consistency the whole batch must satisfy):
     full batch; FP32 within 1e-5 of FP64 at the same parameters over all points;
   * projection, 1,048,576 queries, bicubic 32x32: converged mixed results inside the domain are
     stationary points (residual orthogonal to both tangents, within 1e-9 in parameter units), all
-    agree with the same algorithm run in FP64 within 1e-9, and a random subset agrees with the
-    oracle's Gauss-Newton from the same seeds within 1e-9 where both reach the same foot point
-    (a few per mille reach another local minimum of the distance);
+    agree with the same algorithm run in FP64 and with the constructed foot points within 1e-9
+    where they reach the same foot point, and a random subset agrees with the oracle's
+    Gauss-Newton from the same seeds within 1e-9 where both reach the same foot point (a few per
+    mille reach another local minimum of the distance);
   * uneven batch, 1,024 surfaces (~4.95 M points): every sampled surface's results in the full
     batch are bit-identical to a single-surface inversion of its points alone.
 """
@@ -24,7 +25,7 @@ pytestmark = pytest.mark.gpu
 
 FIELDS = ("S", "Su", "Sv", "Suu", "Suv", "Svv")
 MIXED = dict(precision=G.GN_MIXED, newton=G.NEWTON_FP32 | G.NEWTON_FP64, fp32_iters=20, fp64_tail=3,
-             tol_step32=1e-4, tol_step=1e-10, tol_quad=1e-6)
+             tol_step32=1e-4, tol_step=1e-10, tol_quad=1e-10)
 
 
 def test_evaluation_16M_points():
@@ -89,7 +90,14 @@ def test_projection_1M_queries():
     fu, fv, fst = gs.NewInverse(x, y, z, G.default_opts(**f64))
     both = conv & (fst == G.PT_CONVERGED)
     d = torch.maximum((u - fu).abs(), (v - fv).abs())[both]
-    assert (d <= 1e-9).float().mean().item() > 0.999
+    same = d < 1e-6
+    assert same.float().mean().item() > 0.999
+    assert d[same].max().item() <= 1e-9
+    # every converged point that reached the constructed foot point is within 1e-9 of it (about
+    # 5% of the offset queries have a nearer local minimum elsewhere, which they converge to)
+    dt = torch.maximum((u - torch.from_numpy(u0).cuda()).abs(), (v - torch.from_numpy(v0).cuda()).abs())[conv]
+    assert (dt < 1e-6).float().mean().item() > 0.9
+    assert dt[dt < 1e-6].max().item() <= 1e-9
     # a random subset against the oracle's Gauss-Newton from the same seeds
     _, su, sv = gs.Seed(x, y, z, exact=False)
     rng = np.random.default_rng(9)
@@ -106,10 +114,9 @@ def test_projection_1M_queries():
     # distance (both results stationary, checked above): a few per mille, 1e-2 apart or more
     same = dd < 1e-6
     assert same.mean() > 0.995
-    # the same foot point: within 1e-9 but for very flat minima (small Hessian eigenvalue), where
-    # the 1e-6 quadratic certificate leaves a few 1e-9
-    assert np.mean(dd[same] <= 1e-9) > 0.999
-    assert dd[same].max() < 1e-7
+    # the same foot point: within 1e-9 everywhere, flat minima included (the certificate scales
+    # the step it accepts with the Hessian's conditioning)
+    assert dd[same].max() <= 1e-9
     gs.close()
 
 

@@ -268,8 +268,10 @@ def test_invert_mixed_reaches_1e9():
     _, ssu, ssv = O.seed_greedy_leaf(nodes, 3, P[0], P[1], P[2])
     ru, rv, rst, _ = os_.gauss_newton(P[0], P[1], P[2], ssu, ssv, tol_step=1e-12, max_iter=40, newton=1)
     both = (st == G.PT_CONVERGED) & (rst == O.CONV_STEP)
-    agree = np.maximum(np.abs(u - ru), np.abs(v - rv)) <= 1e-9
-    assert agree[both].mean() > 0.995
+    d = np.maximum(np.abs(u - ru), np.abs(v - rv))[both]
+    same = d < 1e-6  # the few others reached another local minimum of the distance from the seed
+    assert both.mean() > 0.95 and same.mean() > 0.99
+    assert np.max(d[same]) <= 1e-9
 
 
 @pytest.mark.parametrize("newton", [0, G.NEWTON_FP32, G.NEWTON_FP64, G.NEWTON_FP32 | G.NEWTON_FP64])
@@ -281,7 +283,7 @@ def test_invert_mixed_newton_flags_match_oracle(newton):
     gs.BuildTree(3)
     opts = G.default_opts(precision=G.GN_MIXED, newton=newton, fp32_iters=20, fp64_tail=20,
                           tol_step32=1e-4, tol_step=1e-12,
-                          tol_quad=1e-6 if newton & G.NEWTON_FP64 else 0.0)
+                          tol_quad=1e-10 if newton & G.NEWTON_FP64 else 0.0)
     u, v, st = (_np(t) for t in gs.NewInverse(P[0], P[1], P[2], opts))
     nodes, _, _ = gs.GetTree()
     _, ssu, ssv = O.seed_greedy_leaf(nodes, 3, P[0], P[1], P[2])
@@ -291,7 +293,7 @@ def test_invert_mixed_newton_flags_match_oracle(newton):
     d = np.maximum(np.abs(u - ru), np.abs(v - rv))[both]
     same = d < 1e-6  # the few others reached another local minimum of the distance from the seed
     assert same.mean() > 0.99
-    assert np.mean(d[same] <= 1e-9) > 0.995
+    assert np.max(d[same]) <= 1e-9
 
 
 def test_invert_plain_nearest_leaf_on_surface_round_trip():
@@ -379,7 +381,7 @@ def test_ragged_sizes_match_the_full_run(precision, newton):
     30,000; an empty call is a no-op."""
     sd, gs, os_, u0, v0, P = _projection_case(30000, 1e-2)
     gs.BuildTree(3)
-    opts = G.default_opts(precision=precision, newton=newton, tol_quad=1e-6)
+    opts = G.default_opts(precision=precision, newton=newton, tol_quad=1e-10)
     big = [_np(t) for t in gs.NewInverse(P[0], P[1], P[2], opts, iters=True)]
     rng = np.random.default_rng(21)
     for n in (1, 63, 64, 65, 257, 4097):
@@ -415,7 +417,7 @@ def test_long_knot_vectors_take_the_global_knot_paths():
     P = o.eval(u[:5000], v[:5000], 0)
     g.BuildTree(3)
     for prec in (G.GN_FP64, G.GN_MIXED):
-        opts = G.default_opts(precision=prec, newton=G.NEWTON_FP32 | G.NEWTON_FP64, tol_quad=1e-6,
+        opts = G.default_opts(precision=prec, newton=G.NEWTON_FP32 | G.NEWTON_FP64, tol_quad=1e-10,
                               tol_step32=1e-4)
         gu, gv, st = (_np(t) for t in g.NewInverse(P[0], P[1], P[2], opts))
         err = np.maximum(np.abs(gu - u[:5000]), np.abs(gv - v[:5000]))
@@ -444,7 +446,7 @@ def test_zero_iteration_limit_returns_the_start_point(prec):
                   tol_step32=1e-4)
         g32 = G.default_opts(fp64_tail=0, **kw)
         gu, gv, st = (_np(t) for t in g.NewInverse(P[0], P[1], P[2], g32, seeds=(su, sv)))
-        full = G.default_opts(fp64_tail=3, tol_quad=1e-6, **kw)
+        full = G.default_opts(fp64_tail=3, tol_quad=1e-10, **kw)
         fu, fv, _ = (_np(t) for t in g.NewInverse(P[0], P[1], P[2], full, seeds=(su, sv)))
         err = np.maximum(np.abs(gu - u0), np.abs(gv - v0))
         ferr = np.maximum(np.abs(fu - u0), np.abs(fv - v0))
@@ -475,3 +477,34 @@ def test_seed_greedy_leaf_independent_of_batch_size():
         part = gs.Seed(P[0, :m].contiguous(), P[1, :m].contiguous(), P[2, :m].contiguous(), exact=False)
         assert all(torch.equal(a, b[:m]) for a, b in zip(part, full)), m
     gs.close()
+
+
+@pytest.mark.parametrize("precision", [G.GN_FP64, G.GN_MIXED])
+def test_reference_cpu_case_4096_on_surface_inversions(precision):
+    """The reference's CPU-runnable case: bicubic rational 8x8 net (generator G, seed 1000),
+    4,096 points sampled exactly on the surface at uniform (u*, v*), inverted from greedy-leaf
+    seeds on a depth-6 tree.  Plain FP64 (the reference iteration) equals the oracle's same
+    iteration from the oracle's seeds within 1e-9 with the same status; every converged point of
+    either mode is within 1e-9 of (u*, v*)."""
+    sd = synth.surface_G(8, 8, 3, 3, seed=1000)
+    gs, os_ = _pair(sd)
+    gs.BuildTree(3)
+    u0, v0 = synth.uniform_uv(4096, seed=1001)
+    S = os_.eval(u0, v0, 0)
+    if precision == G.GN_FP64:
+        opts = G.default_opts(precision=G.GN_FP64, newton=0, max_iter=20, tol_step=1e-10)
+    else:
+        opts = G.default_opts(precision=G.GN_MIXED, newton=G.NEWTON_FP32 | G.NEWTON_FP64, tol_step32=1e-4)
+    u, v, st = (_np(t) for t in gs.NewInverse(S[0], S[1], S[2], opts))
+    conv = st == G.PT_CONVERGED
+    assert conv.mean() > 0.99
+    err = np.maximum(np.abs(u - u0), np.abs(v - v0))
+    assert np.max(err[conv]) <= 1e-9
+    if precision == G.GN_FP64:
+        nodes, _, _ = gs.GetTree()
+        leaf, ssu, ssv = O.seed_greedy_leaf(nodes, 3, S[0], S[1], S[2])
+        gleaf, gsu, gsv = (_np(t) for t in gs.Seed(S[0], S[1], S[2], exact=False))
+        assert np.array_equal(leaf, gleaf) and np.array_equal(ssu, gsu) and np.array_equal(ssv, gsv)
+        ru, rv, rst, _ = os_.gauss_newton(S[0], S[1], S[2], ssu, ssv, tol_step=1e-10, max_iter=20)
+        assert np.array_equal(st, rst)
+        assert np.max(np.maximum(np.abs(u - ru), np.abs(v - rv))[conv]) <= 1e-9

@@ -47,7 +47,7 @@ def greville(kn, d, ncp):
 OPTS = {
     "fp64": dict(precision=G.GN_FP64, max_iter=20, tol_step=1e-12),
     "mixed": dict(precision=G.GN_MIXED, newton=G.NEWTON_FP32 | G.NEWTON_FP64, fp32_iters=20, fp64_tail=3,
-                  tol_step32=1e-4, tol_step=1e-12, tol_quad=1e-6),
+                  tol_step32=1e-4, tol_step=1e-12, tol_quad=1e-10),
 }
 
 

@@ -108,7 +108,7 @@ def test_fallback_resolves_unconverged_points(bumpy, prec):
         base = dict(precision=G.GN_FP64, newton=G.NEWTON_FP64, max_iter=3, tol_step=1e-12)
     else:
         base = dict(precision=G.GN_MIXED, newton=G.NEWTON_FP32 | G.NEWTON_FP64, fp32_iters=3, fp64_tail=1,
-                    tol_step32=1e-4, tol_step=1e-12, tol_quad=1e-6, max_iter=3)
+                    tol_step32=1e-4, tol_step=1e-12, tol_quad=1e-10, max_iter=3)
     u0, v0, s0 = (_np(t) for t in g.NewInverse(P[0], P[1], P[2], G.default_opts(**base)))
     u1, v1, s1 = (_np(t) for t in g.NewInverse(P[0], P[1], P[2], G.default_opts(fallback=1, **base)))
     bad = (s0 != G.PT_CONVERGED) & (s0 != G.PT_CONVERGED_DIST)

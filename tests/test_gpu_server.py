@@ -16,7 +16,7 @@ from helpers import blocks
 
 pytestmark = pytest.mark.gpu
 
-MIXED = dict(precision=G.GN_MIXED, newton=G.NEWTON_FP32 | G.NEWTON_FP64, tol_step32=1e-4, tol_quad=1e-6)
+MIXED = dict(precision=G.GN_MIXED, newton=G.NEWTON_FP32 | G.NEWTON_FP64, tol_step32=1e-4, tol_quad=1e-10)
 
 
 @pytest.fixture(scope="module")
