@@ -25,6 +25,35 @@ PEAK_F32 = 157.3e12
 PEAK_F64 = 78.6e12
 
 
+def contract_flops_per_iter(k1, k2):
+    """The per-iteration count the workload definition states: 2 (8 k1 k2 + 12 k1) + 40 flops
+    (392 for bicubic): the order-1 contraction and the 2x2 Gauss-Newton solve."""
+    return 2 * (8 * k1 * k2 + 12 * k1) + 40
+
+
+def cpu_info():
+    """Cores this process may use, the machine's logical CPU count and the CPU model."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    omp = os.environ.get("OMP_NUM_THREADS")
+    threads = usable
+    if omp and omp.isdigit() and int(omp) > 0:
+        threads = min(threads, int(omp))  # the host's CPU share for this GPU, where it is set
+    return {"threads": threads, "affinity": usable, "nproc": os.cpu_count(), "omp_num_threads": omp,
+            "model": model}
+
+
 def flops_per_iter(k1, k2, newton):
     """Algorithmic flops of one iteration: contraction FMAs x 2 (point and first derivatives; with
     Newton also the second derivatives) plus the rational quotient and the 2x2 solve."""
@@ -191,7 +220,7 @@ def bench_batch(G, synth, torch, steps, count=1024):
     v = torch.empty_like(u)
     st = torch.empty(n, dtype=torch.uint8, device="cuda")
     mixed = dict(precision=G.GN_MIXED, newton=G.NEWTON_FP32 | G.NEWTON_FP64, fp32_iters=20, fp64_tail=3,
-                 tol_step32=1e-4, tol_step=1e-10, tol_quad=1e-10)
+                 tol_step32=1e-4, tol_step=1e-10, tol_quad=1e-10, seed_mode=G.SEED_GREEDY_STEP)
     shape = b.tune(segs, x, y, z, G.default_opts(**mixed), out=(u, v, st))
     stream = torch.cuda.Stream()
     res = {"workload": f"{count} surfaces (degrees 2..5, nets 4..48, jittered knots), {n} points "
@@ -337,7 +366,7 @@ def bench_serve(G, synth, torch, duration=1.0, loads=(0.5, 0.8), budget=32 << 30
                 "in_step_points_per_s": float(1.0 / b) if b > 0 else None}
 
     mixed = dict(precision=G.GN_MIXED, newton=G.NEWTON_FP32 | G.NEWTON_FP64, fp32_iters=20, fp64_tail=3,
-                 tol_step32=1e-4, tol_step=1e-10, tol_quad=1e-10)
+                 tol_step32=1e-4, tol_step=1e-10, tol_quad=1e-10, seed_mode=G.SEED_GREEDY_STEP)
     plain = dict(precision=G.GN_FP64, max_iter=20, tol_step=1e-10)
     try:
         for name, opts in (("mixed", mixed), ("plain_fp64", plain)):
@@ -446,17 +475,24 @@ def main():
     st = torch.empty(N, dtype=torch.uint8, device=dev)
     torch.cuda.synchronize()
 
-    # mixed: FP32 exact-Hessian bulk to a 1e-4 step, then FP64 exact-Hessian closing steps (<= 3)
-    # stopping at a 1e-6 step (error after the step O(step^2)) or 1e-10
+    # mixed: greedy leaf + one Gauss-Newton step with the leaf's stored Jacobian as the seed, FP32
+    # exact-Hessian bulk to a 1e-4 step, then FP64 exact-Hessian closing steps (<= 3) until the
+    # Newton certificate bounds the (u,v) error by 1e-10, or a 1e-10 step
     mixed_opts = dict(precision=G.GN_MIXED, newton=G.NEWTON_FP32 | G.NEWTON_FP64, fp32_iters=20,
-                      fp64_tail=3, tol_step32=1e-4, tol_step=1e-10, tol_quad=1e-10)
+                      fp64_tail=3, tol_step32=1e-4, tol_step=1e-10, tol_quad=1e-10,
+                      seed_mode=G.SEED_GREEDY_STEP)
     variants = {
         "mixed": G.default_opts(**mixed_opts),
-        # the reference iteration (J^T J Gauss-Newton, src/kernel/kernel.cu:151-249) in FP64
+        # the reference iteration (J^T J Gauss-Newton, src/kernel/kernel.cu:151-249) in FP64 from the
+        # reference's node-centre seeds (greedy leaf)
         "plain_fp64": G.default_opts(precision=G.GN_FP64, newton=0, max_iter=20, tol_step=1e-10),
-        # the mixed mode's algorithm run entirely in FP64, to separate algorithm from precision
+        # the same reference iteration from the mixed mode's stepped seeds, to separate the seed's
+        # share of the gain
+        "plain_fp64_step_seeds": G.default_opts(precision=G.GN_FP64, newton=0, max_iter=20, tol_step=1e-10,
+                                                seed_mode=G.SEED_GREEDY_STEP),
+        # the mixed mode's algorithm and seeds run entirely in FP64, to separate algorithm from precision
         "fp64_newton": G.default_opts(precision=G.GN_FP64, newton=G.NEWTON_FP64, max_iter=20, tol_step=1e-10,
-                                      tol_quad=1e-10),
+                                      tol_quad=1e-10, seed_mode=G.SEED_GREEDY_STEP),
     }
 
     def run(opts):
@@ -486,7 +522,7 @@ def main():
 
     # per-kernel timing of the dominant kernel (mixed Gauss-Newton from given seeds) and the seed
     # search, both with HIP events on the launch stream
-    leaf, su_t, sv_t = gs.Seed(x, y, z, exact=False)
+    leaf, su_t, sv_t = gs.Seed(x, y, z, exact="step")
     torch.cuda.synchronize()
     mixed_given = G.default_opts(**mixed_opts)
     reps = max(3, args.steps)
@@ -512,16 +548,21 @@ def main():
 
     copy_ms = timed(lambda: (u.copy_(su_t), v.copy_(sv_t)))
     gn_ms = timed(gn_only) - copy_ms
-    seed_ms = timed(lambda: gs.Seed(x, y, z, exact=False))
+    seed_ms = timed(lambda: gs.Seed(x, y, z, exact="step"))
     _, _, _, it32, it64 = gs.NewInverse(x, y, z, G.default_opts(**{k: getattr(mixed_given, k) for k, _ in
                                                                    G.InvertOpts._fields_}),
                                         seeds=(su_t, sv_t), iters=True)
     it32 = it32.cpu().numpy().astype(np.int64)
     it64 = it64.cpu().numpy().astype(np.int64)
-    f32 = float(it32.sum()) * flops_per_iter(4, 4, True)
-    f64 = float(it64.sum()) * flops_per_iter(4, 4, True)
+    # contract count: 392 flops per iteration (bicubic); the Newton iteration as built does more
+    # (second derivatives and the Hessian term: flops_per_iter), reported beside it
+    f32 = float(it32.sum()) * contract_flops_per_iter(4, 4)
+    f64 = float(it64.sum()) * contract_flops_per_iter(4, 4)
+    f32n = float(it32.sum()) * flops_per_iter(4, 4, True)
+    f64n = float(it64.sum()) * flops_per_iter(4, 4, True)
     t = gn_ms / 1e3
     ideal = f32 / PEAK_F32 + f64 / PEAK_F64
+    ideal_n = f32n / PEAK_F32 + f64n / PEAK_F64
     peak_eff = (f32 + f64) / ideal
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "inversion_projection_1M_traffic.json")
@@ -530,13 +571,17 @@ def main():
         # FETCH_SIZE / WRITE_SIZE passes (scaled by the copy-kernel calibration)
         with open(tpath) as f:
             traffic = json.load(f)["gn_step_traffic_bytes"]
-    roof = {"bound": "mfma", "achieved": (f32 + f64) / t / 1e12, "peak": peak_eff / 1e12, "unit": "TFLOP/s",
+    roof = {"bound": "valu", "achieved": (f32 + f64) / t / 1e12, "peak": peak_eff / 1e12, "unit": "TFLOP/s",
             "frac": ideal / t, "traffic": traffic,
+            "flops_per_iter": contract_flops_per_iter(4, 4),
+            "frac_newton_flops": ideal_n / t, "newton_flops_per_iter": flops_per_iter(4, 4, True),
             "algorithmic_bytes": N * (40 + 17) + N * (41 + 17),
             "kernel": "FP32 bulk (k_gn_refill) + FP64 closing pass (k_close_first, then k_gn_refill over the "
                       "listed rest): Gauss-Newton from given seeds",
-            "roof_note": "compute roof: FP32 flops at 157.3 TFLOP/s plus FP64 flops at 78.6 TFLOP/s "
-                         "(published MI355X peaks); VALU arithmetic",
+            "roof_note": "VALU arithmetic (no MFMA): compute roof from FP32 flops at 157.3 TFLOP/s plus FP64 "
+                         "flops at 78.6 TFLOP/s (published MI355X vector peaks); frac = time at those peaks / "
+                         "measured time, counting 392 flops per iteration (2(8k1k2+12k1)+40, bicubic); "
+                         "frac_newton_flops counts the Newton iteration as built",
             "kernel_ms": gn_ms, "flops_fp32": f32, "flops_fp64": f64,
             "iters_fp32_mean": float(it32.mean()), "iters_fp64_mean": float(it64.mean())}
 
@@ -560,7 +605,8 @@ def main():
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle as O
 
-        threads = min(16, os.cpu_count() or 1)
+        ci = cpu_info()
+        threads = ci["threads"]
         os_ = O.OSurf.from_def(sd)
         nodes, _, _ = os_.build_tree(3)  # set-up, like the GPU tree (not timed on either side)
         xs, ys, zs = (a.cpu().numpy() for a in (x, y, z))
@@ -577,7 +623,10 @@ def main():
                "sample": f"all {N} queries x {reps} repetitions ({dt:.1f} s wall): greedy-leaf seed "
                          f"search + plain FP64 Gauss-Newton (oracle restatement of "
                          f"src/kernel/kernel.cu:151-249, <=20 iterations, 1e-10 step)",
-               "ok_frac": cok / N}
+               "ok_frac": cok / N, "nproc": ci["nproc"], "affinity_cpus": ci["affinity"],
+               "omp_num_threads": ci["omp_num_threads"], "cpu_model": ci["model"],
+               "threads_note": "every CPU this process may use (its affinity mask), capped by OMP_NUM_THREADS "
+                               "where the host sets the GPU's CPU share"}
         cpu_case = bench_cpu_case(G, synth, torch, O, args.steps, threads)
 
     eval_res = None if args.no_eval else bench_eval(G, synth, torch, max(3, args.steps))
@@ -600,13 +649,15 @@ def main():
         "data": "synthetic",
         "config": {"workload": "projection: bicubic rational surface, 32x32 net (generator G, seed 1002), "
                                f"{N} queries offset N(0,1e-2) along the normal, depth-6 tree (64x64 leaves) "
-                               "greedy-leaf seeds, <=20 FP32 + <=3 FP64 iterations",
+                               "greedy-leaf seeds + one Gauss-Newton step (leaf Jacobian), <=20 FP32 + <=3 FP64 "
+                               "iterations",
                    "points": N, "surface": "p=q=3, 32x32", "tree_depth": 6},
         "roofline": roof,
         "cpu_baseline": cpu,
         "mixed": mixed,
         "plain_fp64": res["plain_fp64"],
         "fp64_newton": res["fp64_newton"],
+        "plain_fp64_step_seeds": res["plain_fp64_step_seeds"],
         "speedup_vs_plain_fp64": mixed["points_per_s"] / max(res["plain_fp64"]["points_per_s"], 1e-30),
         "seed_ms": seed_ms,
         "global_projection": global_res,

@@ -67,7 +67,9 @@ static void free_tree(gn_surface *s) {
     (void)hipFree(s->d_sbox32);
     (void)hipFree(s->d_sref);
     (void)hipFree(s->d_sleaf32);
+    (void)hipFree(s->d_sjac);
     s->d_sleaf32 = nullptr;
+    s->d_sjac = nullptr;
     s->d_tree = nullptr;
     s->d_valid = s->d_divided = nullptr;
     s->d_sbox = s->d_scentre = s->d_sseed = nullptr;
@@ -298,6 +300,15 @@ int gn_evaluate_grid_f32(const gn_surface *s, const float *us, int nu_pts, const
                                   (hipStream_t)stream);
 }
 
+int gn_evaluate_sorted_f32(const gn_surface *s, const float *u, const float *v, size_t n,
+                           int max_order, float *out, uint32_t *perm, gn_stream stream) {
+    if (!s) return fail(GN_EINVAL, "null surface");
+    if (max_order < 0 || max_order > 2) return fail(GN_EINVAL, "max_order must be 0..2");
+    if (n > (size_t)UINT32_MAX) return fail(GN_EINVAL, "at most 2^32-1 points per call");
+    if (n && (!u || !v || !out || !perm)) return fail(GN_EINVAL, "null buffer");
+    return gn::launch_eval_sorted32(s->dev(), u, v, n, max_order, out, perm, (hipStream_t)stream);
+}
+
 int gn_build_tree(gn_surface *s, int levels, int flags, gn_stream stream) {
     if (!s) return fail(GN_EINVAL, "null surface");
     if (levels < 1 || levels > 4) return fail(GN_EINVAL, "levels must be 1..4");
@@ -318,7 +329,8 @@ int gn_build_tree(gn_surface *s, int levels, int flags, gn_stream stream) {
         (rc = dalloc(&s->d_scentre, (size_t)leaves * 3, acc)) ||
         (rc = dalloc(&s->d_sseed, (size_t)leaves * 2, acc)) ||
         (rc = dalloc(&s->d_sref, (size_t)leaves, acc)) ||
-        (rc = dalloc(&s->d_sleaf32, (size_t)leaves, acc))) {
+        (rc = dalloc(&s->d_sleaf32, (size_t)leaves, acc)) ||
+        (rc = dalloc(&s->d_sjac, (size_t)leaves * 6, acc))) {
         free_tree(s);
         return rc;
     }
@@ -328,7 +340,7 @@ int gn_build_tree(gn_surface *s, int levels, int flags, gn_stream stream) {
         if (rc) { free_tree(s); return rc; }
     }
     rc = gn::launch_search_build(d, levels, s->d_tree, s->d_sbox, s->d_sbox32, s->d_scentre,
-                                 s->d_sseed, s->d_sref, s->d_sleaf32, st);
+                                 s->d_sseed, s->d_sref, s->d_sleaf32, s->d_sjac, st);
     if (rc) { free_tree(s); return rc; }
     s->tree_levels = levels;
     s->device_bytes += acc;
@@ -354,8 +366,9 @@ int gn_seed(const gn_surface *s, int exact, const double *x, const double *y, co
     if (!s) return fail(GN_EINVAL, "null surface");
     if (!s->tree_levels) return fail(GN_ESTATE, "tree not built");
     if (n && (!x || !y || !z || !su || !sv)) return fail(GN_EINVAL, "null buffer");
-    return gn::launch_seed(s->search(), exact ? 0 : 1, x, y, z, n, leaf, su, sv,
-                           (hipStream_t)stream);
+    // exact: 0 greedy leaf, 2 greedy leaf + Gauss-Newton step, any other value the exact search
+    return gn::launch_seed(s->search(), exact == 0 ? 1 : exact == 2 ? 2 : 0, x, y, z, n, leaf, su,
+                           sv, (hipStream_t)stream);
 }
 
 int gn_invert(const gn_surface *s, const double *x, const double *y, const double *z, size_t n,
@@ -367,14 +380,26 @@ int gn_invert(const gn_surface *s, const double *x, const double *y, const doubl
 int gn_invert_iters(const gn_surface *s, const double *x, const double *y, const double *z,
                     size_t n, const gn_invert_opts *opts, double *u, double *v, uint8_t *status,
                     uint8_t *iters_fp32, uint8_t *iters_fp64, gn_stream stream) {
-    if (!s) return fail(GN_EINVAL, "null surface");
     gn_invert_opts o;
     if (opts) o = *opts;
     else gn_invert_opts_default(&o);
+    const int rc = gn::check_invert(s, x, y, z, n, o, u, v, status);
+    if (rc) return rc;
+    return gn::launch_invert(s->dev(), s->search(), s->d_tree, s->d_valid, s->tree_levels, x, y,
+                             z, n, o, u, v, status, iters_fp32, iters_fp64, (hipStream_t)stream);
+}
+
+}  // extern "C"
+
+namespace gn {
+
+int check_invert(const gn_surface *s, const double *x, const double *y, const double *z, size_t n,
+                 const gn_invert_opts &o, const double *u, const double *v, const uint8_t *status) {
+    if (!s) return fail(GN_EINVAL, "null surface");
     if (n && (!x || !y || !z || !u || !v || !status)) return fail(GN_EINVAL, "null buffer");
     if (o.precision != GN_FP64 && o.precision != GN_MIXED)
         return fail(GN_EINVAL, "precision must be GN_FP64 or GN_MIXED");
-    if (o.seed_mode < GN_SEED_NEAREST_LEAF || o.seed_mode > GN_SEED_GLOBAL)
+    if (o.seed_mode < GN_SEED_NEAREST_LEAF || o.seed_mode > GN_SEED_GREEDY_STEP)
         return fail(GN_EINVAL, "bad seed_mode");
     if (o.seed_mode == GN_SEED_GLOBAL && o.precision != GN_FP64)
         return fail(GN_EINVAL, "global projection runs in FP64 (precision GN_FP64)");
@@ -387,8 +412,7 @@ int gn_invert_iters(const gn_surface *s, const double *x, const double *y, const
         return fail(GN_ESTATE, "tree not built (gn_build_tree) and no seeds given");
     if (o.seed_mode == GN_SEED_REF_FIRST_HIT && o.precision != GN_FP64)
         return fail(GN_EINVAL, "first-hit seeding is the plain FP64 reference path");
-    return gn::launch_invert(s->dev(), s->search(), s->d_tree, s->d_valid, s->tree_levels, x, y,
-                             z, n, o, u, v, status, iters_fp32, iters_fp64, (hipStream_t)stream);
+    return GN_OK;
 }
 
-}  // extern "C"
+}  // namespace gn

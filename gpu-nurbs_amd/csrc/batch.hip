@@ -69,8 +69,15 @@ __global__ __launch_bounds__(kSeedBlock) void k_batch_seed(const GnBatchObj *__r
         const int64_t i = tl.start + j;
         const double P[3] = {R.x[i], R.y[i], R.z[i]};
         const uint32_t c = mode == GN_SEED_NEAREST_LEAF ? nearest_leaf(sd, P) : greedy_leaf(sd, P, lbox, ll);
-        R.u[i] = sd.seed_uv[c * 2 + 0];
-        R.v[i] = sd.seed_uv[c * 2 + 1];
+        if (mode == GN_SEED_GREEDY_STEP) {
+            double u, v;
+            step_seed(sd, c, P, u, v);
+            R.u[i] = u;
+            R.v[i] = v;
+        } else {
+            R.u[i] = sd.seed_uv[c * 2 + 0];
+            R.v[i] = sd.seed_uv[c * 2 + 1];
+        }
     }
 }
 
@@ -384,22 +391,37 @@ static int launch_ranges(const GnBatchObj *objs, const GnTile *tiles,
     if (ranges.size() > (size_t)kSideStreams)
         return fail(GN_EDEVICE, "more group ranges than side streams (tiles not sorted by class)");
     GN_HIP(hipEventRecord(ss.fork, st));
-    int k = 0;
+    // st joins every side stream that was forked, also when a later launch fails: the caller
+    // may then synchronise st alone (or free buffers in its order) before it reuses what the
+    // kernels already enqueued still read or write
+    int k = 0, rc = GN_OK;
     for (const GroupRange &gr : ranges) {
         if (gr.points == 0) continue;
-        GN_HIP(hipStreamWaitEvent(ss.s[k], ss.fork, 0));
-        const int rc = run_range(objs, tiles, gr, reqs, o, bpc_cap, ss.s[k]);
-        if (rc) return rc;
-        GN_HIP(hipEventRecord(ss.join[k], ss.s[k]));
-        if (group_done) group_done->emplace_back(gr.grp, ss.join[k]);
+        const hipError_t e = hipStreamWaitEvent(ss.s[k], ss.fork, 0);
+        if (e != hipSuccess) {
+            rc = hip_fail(e, "hipStreamWaitEvent(fork)");
+            break;
+        }
+        rc = run_range(objs, tiles, gr, reqs, o, bpc_cap, ss.s[k]);
+        const hipError_t ej = hipEventRecord(ss.join[k], ss.s[k]);
+        if (ej != hipSuccess) {
+            if (!rc) rc = hip_fail(ej, "hipEventRecord(join)");
+            (void)hipStreamSynchronize(ss.s[k]);  // no join event: wait for it here
+            break;
+        }
         k++;
+        if (rc) break;
+        if (group_done) group_done->emplace_back(gr.grp, ss.join[k - 1]);
     }
-    if (beside) {
-        const int rc = beside(st);
-        if (rc) return rc;
+    if (!rc && beside) rc = beside(st);
+    for (int j = 0; j < k; j++) {
+        const hipError_t e = hipStreamWaitEvent(st, ss.join[j], 0);
+        if (e != hipSuccess) {
+            (void)hipStreamSynchronize(ss.s[j]);
+            if (!rc) rc = hip_fail(e, "hipStreamWaitEvent(join)");
+        }
     }
-    for (int j = 0; j < k; j++) GN_HIP(hipStreamWaitEvent(st, ss.join[j], 0));
-    return GN_OK;
+    return rc;
 }
 
 static int shared_side_locked(int dev) {

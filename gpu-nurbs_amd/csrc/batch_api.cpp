@@ -30,6 +30,9 @@ struct gn_graph {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     char *d_buf = nullptr;  // tile list + request table of the captured call
+    // the claim-counter slots baked into the graph: its own, so no eager launch is ever given
+    // them (each replay leaves them zeroed for the next)
+    unsigned long long *d_ctr = nullptr;
 };
 
 namespace {
@@ -134,7 +137,9 @@ int check_opts(const gn_invert_opts &o) {
         return fail(GN_EINVAL, "precision must be GN_FP64 or GN_MIXED");
     if (o.seed_mode == GN_SEED_REF_FIRST_HIT)
         return fail(GN_EINVAL, "first-hit seeding is a single-surface reference mode");
-    if (o.seed_mode < GN_SEED_NEAREST_LEAF || o.seed_mode > GN_SEED_GREEDY_LEAF)
+    if (o.seed_mode == GN_SEED_GLOBAL)
+        return fail(GN_EINVAL, "the global projection is a single-surface mode");
+    if (o.seed_mode < GN_SEED_NEAREST_LEAF || o.seed_mode > GN_SEED_GREEDY_STEP)
         return fail(GN_EINVAL, "bad seed_mode");
     if (o.max_iter < 0 || o.fp32_iters < 0 || o.fp64_tail < 0 || o.max_iter > 10000 ||
         o.fp32_iters > 10000 || o.fp64_tail > 10000)
@@ -201,6 +206,7 @@ int gn_batch_create(gn_surface *const *surfaces, int count, gn_stream stream, gn
             {s->d_sseed, sizeof(double) * leaves * 2, 0},
             {s->d_sref, sizeof(int32_t) * leaves, 0},
             {s->d_sleaf32, sizeof(float4) * leaves, 0},
+            {s->d_sjac, sizeof(double) * leaves * 6, 0},
         };
         for (Part p : ps) {
             p.off = total;
@@ -245,6 +251,7 @@ int gn_batch_create(gn_surface *const *surfaces, int count, gn_stream stream, gn
         ob.sd.seed_uv = (const double *)(base + parts[i][8].off);
         ob.sd.ref_index = (const int32_t *)(base + parts[i][9].off);
         ob.sd.leaf32 = (const float4 *)(base + parts[i][10].off);
+        ob.sd.jac = (const double *)(base + parts[i][11].off);
         ob.cls = degree_class(s->p, s->q);
         ob.pad_ = 0;
         b->cls[i] = ob.cls;
@@ -332,6 +339,15 @@ int gn_batch_tune(gn_batch *b, const gn_segment *segs, int nseg, const double *x
         }
     }
     hipStream_t st = (hipStream_t)stream;
+    // given seeds live in u and v, which every run overwrites: keep a copy and put it back
+    // before each run (outside the timed region)
+    double *seeds = nullptr;
+    const bool given = o.seed_mode == GN_SEED_GIVEN && n > 0;
+    if (given) {
+        GN_HIP(hipMallocAsync((void **)&seeds, sizeof(double) * 2 * n, st));
+        GN_HIP(hipMemcpyAsync(seeds, u, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+        GN_HIP(hipMemcpyAsync(seeds + n, v, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+    }
     hipEvent_t e0, e1;
     GN_HIP(hipEventCreate(&e0));
     GN_HIP(hipEventCreate(&e1));
@@ -343,7 +359,10 @@ int gn_batch_tune(gn_batch *b, const gn_segment *segs, int nseg, const double *x
         for (int c : caps) {
             float sum = 0;
             for (int r = 0; r < 3 && !rc; r++) {  // first run warms up, not counted
-                if (o.seed_mode == GN_SEED_GIVEN) break;  // seeds would be overwritten
+                if (given) {
+                    GN_HIP(hipMemcpyAsync(u, seeds, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+                    GN_HIP(hipMemcpyAsync(v, seeds + n, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+                }
                 GN_HIP(hipEventRecord(e0, st));
                 rc = run_batch(b, segs, nseg, x, y, z, n, o, u, v, status, nullptr, nullptr, t, c, st);
                 GN_HIP(hipEventRecord(e1, st));
@@ -362,6 +381,7 @@ int gn_batch_tune(gn_batch *b, const gn_segment *segs, int nseg, const double *x
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    if (given) GN_HIP(hipFreeAsync(seeds, st));
     if (rc) return rc;
     b->tile = bt;
     b->bpc = bc;
@@ -392,27 +412,41 @@ int gn_batch_graph_create(gn_batch *b, const gn_segment *segs, int nseg, const d
     const CallImage img = call_image(b, build_tiles(b, segs, nseg, b->tile), req);
     gn_graph *g = new gn_graph();
     hipStream_t st = (hipStream_t)stream;
+    // one slot per persistent launch: one per class group and precision phase
+    gn::SlotPool pool;
+    pool.nslots = (int)img.ranges.size() * (o.precision == GN_MIXED ? 2 : 1);
+    const size_t ctr_bytes = sizeof(unsigned long long) * gn::slot_words() * (pool.nslots ? pool.nslots : 1);
     hipError_t e = hipMalloc((void **)&g->d_buf, img.bytes.size());
     if (e == hipSuccess)
         e = hipMemcpy(g->d_buf, img.bytes.data(), img.bytes.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMalloc((void **)&g->d_ctr, ctr_bytes);
+    if (e == hipSuccess) e = hipMemset(g->d_ctr, 0, ctr_bytes);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
     if (e != hipSuccess) {
         (void)hipFree(g->d_buf);
+        (void)hipFree(g->d_ctr);
         delete g;
-        return gn::hip_fail(e, "graph tile list");
+        return gn::hip_fail(e, "graph tile list / counter slots");
     }
+    pool.ctr = g->d_ctr;
     e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
     if (e != hipSuccess) {
         (void)hipFree(g->d_buf);
+        (void)hipFree(g->d_ctr);
         delete g;
         return gn::hip_fail(e, "hipStreamBeginCapture");
     }
-    rc = gn::launch_batch_invert(b->d_objs, (const GnTile *)g->d_buf, img.ranges,
-                                 (const GnReq *)(g->d_buf + img.req_off), o, b->bpc, st);
+    {
+        gn::PoolScope scope(&pool);
+        rc = gn::launch_batch_invert(b->d_objs, (const GnTile *)g->d_buf, img.ranges,
+                                     (const GnReq *)(g->d_buf + img.req_off), o, b->bpc, st);
+    }
     e = hipStreamEndCapture(st, &g->graph);
     if (!rc && e == hipSuccess) e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
     if (rc || e != hipSuccess) {
         if (g->graph) (void)hipGraphDestroy(g->graph);
         (void)hipFree(g->d_buf);
+        (void)hipFree(g->d_ctr);
         delete g;
         return rc ? rc : gn::hip_fail(e, "graph capture / instantiate");
     }
@@ -432,6 +466,7 @@ int gn_graph_destroy(gn_graph *g) {
     if (g->exec) (void)hipGraphExecDestroy(g->exec);
     if (g->graph) (void)hipGraphDestroy(g->graph);
     (void)hipFree(g->d_buf);
+    (void)hipFree(g->d_ctr);
     delete g;
     return GN_OK;
 }

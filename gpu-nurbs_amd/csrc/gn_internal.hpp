@@ -43,6 +43,8 @@ struct GnSearchDev {
     const int32_t *ref_index; // per leaf (Morton order) reference tree index
     const float4 *leaf32;     // per leaf (Morton order) centre in FP32 and the reference index
                               // (bits of w), one 16-byte load for the greedy 3 x 3 search
+    const double *jac;        // per leaf (Morton order) S_u, S_v at the leaf's seed (u, v)
+    double u0, u1, v0, v1;    // the knot domain the leaves tile
 };
 
 // One object of a batch: the surface and its seed tree (pointers into the batch's pack) and its
@@ -116,6 +118,7 @@ struct gn_surface {
     float *d_sbox32 = nullptr;
     int32_t *d_sref = nullptr;
     float4 *d_sleaf32 = nullptr;
+    double *d_sjac = nullptr;
     size_t device_bytes = 0;
 
     GnSurfDev dev() const {
@@ -132,6 +135,11 @@ struct gn_surface {
         s.box = d_sbox; s.box32 = d_sbox32; s.centre = d_scentre; s.seed_uv = d_sseed;
         s.ref_index = d_sref;
         s.leaf32 = d_sleaf32;
+        s.jac = d_sjac;
+        s.u0 = ku.empty() ? 0.0 : ku[p];
+        s.u1 = ku.empty() ? 1.0 : ku[nu];
+        s.v0 = kv.empty() ? 0.0 : kv[q];
+        s.v1 = kv.empty() ? 1.0 : kv[nv];
         return s;
     }
 };
@@ -168,12 +176,16 @@ int launch_eval32(const GnSurfDev &d, const float *u, const float *v, size_t n, 
                   float *out, hipStream_t st);
 int launch_eval_grid32(const GnSurfDev &d, const float *us, int nu_pts, const float *vs,
                        int nv_pts, int order, float *out, hipStream_t st);
+// span-order FP32 evaluation (span_gemm.hip): out in sorted order, perm[k] = input index
+int launch_eval_sorted32(const GnSurfDev &d, const float *u, const float *v, size_t n, int order,
+                         float *out, uint32_t *perm, hipStream_t st);
 int launch_tree_level(const GnSurfDev &d, int level, int flags, GnTreeNode *tree,
                       uint8_t *valid, uint8_t *divided, hipStream_t st);
 int launch_search_build(const GnSurfDev &d, int levels, const GnTreeNode *tree, double *sbox,
                         float *sbox32, double *scentre, double *sseed, int32_t *sref,
-                        float4 *sleaf32, hipStream_t st);
-// mode 0: exact nearest leaf centre; mode 1: greedy descent + 3x3 leaf neighbourhood
+                        float4 *sleaf32, double *sjac, hipStream_t st);
+// mode 0: exact nearest leaf centre; mode 1: greedy descent + 3x3 leaf neighbourhood;
+// mode 2: mode 1, then one Gauss-Newton step from the leaf's seed with its stored Jacobian
 int launch_seed(const GnSearchDev &sd, int mode, const double *x, const double *y,
                 const double *z, size_t n, int32_t *leaf, double *su, double *sv,
                 hipStream_t st);
@@ -229,10 +241,38 @@ int batch_waves(int bpc_cap);
 int launch_curve_batch(const GnCurveDev *curves, const GnTile *tiles, int ntiles, const GnReq *reqs,
                        int max_samples, const gn_invert_opts &o, hipStream_t st);
 int ensure_runtime();  // allocates the per-device claim counters (before any graph capture)
+// Claim-counter slots (and continuation lists) owned by one client instead of the device's shared
+// ring: a captured graph keeps the slots it was captured with for every replay, so they must never
+// be handed to another launch; the no-allocation inversion uses a fixed device array.  While a
+// pool is set for the calling thread, every launch that thread enqueues takes its slots from the
+// pool in order (an error once it is used up).  The counters must start zeroed; every launch
+// leaves its slot zeroed again.
+struct SlotPool {
+    unsigned long long *ctr = nullptr;  // nslots slots of slot_words() counters
+    uint32_t *list = nullptr;           // nlists continuation lists of list_entries() indices
+    int nslots = 0, nlists = 0;
+    int used = 0, lists_used = 0;       // taken so far
+};
+size_t slot_words();    // 64-bit words per counter slot
+size_t list_entries();  // 32-bit indices per continuation list
+// Routes the calling thread's launches to `pool` for the lifetime of the object.
+class PoolScope {
+  public:
+    explicit PoolScope(SlotPool *pool);
+    ~PoolScope();
+    PoolScope(const PoolScope &) = delete;
+    PoolScope &operator=(const PoolScope &) = delete;
+
+  private:
+    SlotPool *prev_;
+};
 // Copies `bytes` from pinned host memory (src: its device-side address) to device memory with an
 // ordinary kernel on st; both buffers 16-byte aligned and readable/writable to bytes rounded up
 // to 16.
 int launch_stage_copy(const void *src, void *dst, size_t bytes, hipStream_t st);
+// Argument checks of gn_invert (a failing check sets the thread's error message).
+int check_invert(const gn_surface *s, const double *x, const double *y, const double *z, size_t n,
+                 const gn_invert_opts &o, const double *u, const double *v, const uint8_t *status);
 int launch_invert(const GnSurfDev &d, const GnSearchDev &sd, const GnTreeNode *tree,
                   const uint8_t *valid, int tree_levels, const double *x, const double *y,
                   const double *z, size_t n, const gn_invert_opts &o, double *u, double *v,

@@ -161,4 +161,35 @@ __device__ inline uint32_t greedy_leaf(const GnSearchDev &sd, const double P[3],
     return best_c;
 }
 
+// Seed refinement of GN_SEED_GREEDY_STEP: one Gauss-Newton step from leaf c's seed (u, v) with
+// the Jacobian J = [S_u S_v] stored for the leaf (evaluated at its seed, whose surface point is
+// the leaf's centre C), so no surface evaluation is needed:
+//   (du, dv) = (J^T J)^-1 J^T (C - P),
+// scaled down uniformly to at most one leaf cell in each direction and clamped to the domain.  A
+// singular J^T J keeps the leaf seed.  Every multiply-add is an explicit fma (the oracle does the
+// same), so the seed does not depend on the compiler's contraction.
+__device__ __forceinline__ void step_seed(const GnSearchDev &sd, uint32_t c, const double P[3],
+                                          double &su, double &sv) {
+    su = sd.seed_uv[c * 2 + 0];
+    sv = sd.seed_uv[c * 2 + 1];
+    const double *C = sd.centre + (size_t)c * 3;
+    const double *J = sd.jac + (size_t)c * 6;
+    const double r0 = C[0] - P[0], r1 = C[1] - P[1], r2 = C[2] - P[2];
+    const double a = fma(J[2], J[2], fma(J[1], J[1], J[0] * J[0]));
+    const double b = fma(J[2], J[5], fma(J[1], J[4], J[0] * J[3]));
+    const double e = fma(J[5], J[5], fma(J[4], J[4], J[3] * J[3]));
+    const double gu = fma(J[2], r2, fma(J[1], r1, J[0] * r0));
+    const double gv = fma(J[5], r2, fma(J[4], r1, J[3] * r0));
+    const double det = fma(a, e, -(b * b));
+    if (!(det > 0.0)) return;
+    const double du = fma(e, gu, -(b * gv)) / det;
+    const double dv = fma(a, gv, -(b * gu)) / det;
+    const double cells = (double)(1u << sd.depth);
+    const double hu = (sd.u1 - sd.u0) / cells, hv = (sd.v1 - sd.v0) / cells;
+    const double sc = fmax(1.0, fmax(fabs(du) / hu, fabs(dv) / hv));
+    if (!(sc < INFINITY)) return;
+    su = fmin(fmax(su - du / sc, sd.u0), sd.u1);
+    sv = fmin(fmax(sv - dv / sc, sd.v0), sd.v1);
+}
+
 }  // namespace gn

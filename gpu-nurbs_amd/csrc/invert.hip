@@ -20,9 +20,10 @@ int launch_invert(const GnSurfDev &d, const GnSearchDev &sd, const GnTreeNode *t
         for (int l = 0; l < tree_levels; l++) { count += g * g; g *= 8; }
         rc = launch_gn_first_hit(d, a, tree, valid, count, st);
     } else {
-        if (o.seed_mode == GN_SEED_NEAREST_LEAF || o.seed_mode == GN_SEED_GREEDY_LEAF) {
-            rc = launch_seed(sd, o.seed_mode == GN_SEED_NEAREST_LEAF ? 0 : 1, x, y, z, n, nullptr,
-                             u, v, st);
+        if (o.seed_mode != GN_SEED_GIVEN) {
+            const int mode = o.seed_mode == GN_SEED_NEAREST_LEAF ? 0
+                           : o.seed_mode == GN_SEED_GREEDY_STEP  ? 2 : 1;
+            rc = launch_seed(sd, mode, x, y, z, n, nullptr, u, v, st);
             if (rc) return rc;
         }
         rc = o.precision == GN_MIXED ? launch_gn_mixed(d, a, st) : launch_gn_plain(d, a, st);

@@ -77,7 +77,10 @@ __device__ __forceinline__ void project_point(const GnSurfDev &d, const GnSearch
                                               const InvArgs &a, size_t i, bool counts) {
     const Domain full = domain_of(d);
     const int D = sd.depth;
-    const double cell = 1.0 / (double)(1u << D);  // leaf cells of [0,1]^2 (the tree's uv layout)
+    // leaf cells: the unit square's 2^D x 2^D cells mapped onto the knot domain, exactly as the
+    // tree lays out its nodes' uv rectangles
+    const double cell = 1.0 / (double)(1u << D);
+    const double wu = full.u1 - full.u0, wv = full.v1 - full.v0;
     const int max_iter = a.o.max_iter;
     const double tol_step = a.o.tol_step;
     const double P[3] = {a.x[i], a.y[i], a.z[i]};
@@ -98,7 +101,8 @@ __device__ __forceinline__ void project_point(const GnSurfDev &d, const GnSearch
                     descend = true;
                 } else if (c != c0) {
                     const uint32_t ix = compact2(c >> 1), iy = compact2(c);
-                    const Domain cr{ix * cell, (ix + 1) * cell, iy * cell, (iy + 1) * cell};
+                    const Domain cr{full.u0 + wu * (ix * cell), full.u0 + wu * ((ix + 1) * cell),
+                                    full.v0 + wv * (iy * cell), full.v0 + wv * ((iy + 1) * cell)};
                     double cu = sd.seed_uv[2 * c], cv = sd.seed_uv[2 * c + 1], d2;
                     const int s2 = solve_in<K1, K2, EXACT>(d, P, cu, cv, cr, kLeafIter, tol_step,
                                                            d2);

@@ -3,8 +3,9 @@
 //     slot kMaxSlices 64-bit counters on their own 128-byte lines plus an exit counter; every
 //     launch takes the next slot.  Slots start zeroed and every kernel that uses one leaves it
 //     zeroed (the last wave to retire clears it), so a launch needs no clearing kernel or memset
-//     and a captured hipGraph can be replayed.  Two launches share a slot only if more than
-//     kCounterSlots launches are in flight at once.
+//     and a captured hipGraph can be replayed.  Two eager launches share a slot only if more than
+//     kCounterSlots launches are in flight at once.  Captured graphs and the no-allocation entry
+//     point take their slots from pools of their own (SlotPool), never from the ring.
 //   - beside each slot, a continuation list of kListCap point indices (128 MB per device for the
 //     ring, allocated with the counters).
 //   - a copy kernel for small host-to-device images (the server's step tables).
@@ -27,7 +28,14 @@ unsigned long long *g_counters[kMaxDevices] = {};
 uint32_t *g_lists[kMaxDevices] = {};
 int g_cus[kMaxDevices] = {};
 std::atomic<unsigned> g_next{0};
+thread_local SlotPool *t_pool = nullptr;
 }  // namespace
+
+size_t slot_words() { return kSlotWords; }
+size_t list_entries() { return kListCap; }
+
+PoolScope::PoolScope(SlotPool *pool) : prev_(t_pool) { t_pool = pool; }
+PoolScope::~PoolScope() { t_pool = prev_; }
 
 int ensure_runtime() {
     int dev = 0;
@@ -50,6 +58,11 @@ int ensure_runtime() {
 }
 
 int work_counters(hipStream_t st, unsigned long long **out) {
+    if (SlotPool *p = t_pool) {
+        if (p->used >= p->nslots) return fail(GN_ESTATE, "counter slot pool used up");
+        *out = p->ctr + (size_t)p->used++ * kSlotWords;
+        return GN_OK;
+    }
     int rc = ensure_runtime();
     if (rc) return rc;
     int dev = 0;
@@ -61,6 +74,13 @@ int work_counters(hipStream_t st, unsigned long long **out) {
 }
 
 int work_list(const unsigned long long *counters, uint32_t **list) {
+    if (SlotPool *p = t_pool) {
+        if (counters < p->ctr || counters >= p->ctr + (size_t)p->nslots * kSlotWords)
+            return fail(GN_EDEVICE, "counter slot outside the thread's pool");
+        if (p->lists_used >= p->nlists) return fail(GN_ESTATE, "continuation list pool used up");
+        *list = p->list + (size_t)p->lists_used++ * kListCap;
+        return GN_OK;
+    }
     int dev = 0;
     GN_HIP(hipGetDevice(&dev));
     if (dev < 0 || dev >= kMaxDevices) return fail(GN_EDEVICE, "device index out of range");

@@ -283,7 +283,11 @@ void gn_server::run_slot(Slot &sl) {
         lk.unlock();
         bool ok = launch_step(sl) == GN_OK;
         if (!ok) {
+            // the launch joins every side stream it forked into sl.st even when it fails; the
+            // side streams are waited for as well, so no kernel of the failed step still writes
+            // into result ranges that may be released and handed to new requests
             (void)hipStreamSynchronize(sl.st);
+            for (int k = 0; k < gn::kSideStreams; k++) (void)hipStreamSynchronize(sl.side.s[k]);
         } else {
             // each class group's requests complete when that group's work has; the rest (and
             // everything after an error) with the step's event
@@ -380,8 +384,10 @@ int gn_server_create(gn_batch *batch, gn_curve *const *curves, int ncurves,
     else gn_invert_opts_default(&o);
     if (o.precision != GN_FP64 && o.precision != GN_MIXED)
         return fail(GN_EINVAL, "precision must be GN_FP64 or GN_MIXED");
-    if (o.seed_mode != GN_SEED_GREEDY_LEAF && o.seed_mode != GN_SEED_NEAREST_LEAF)
-        return fail(GN_EINVAL, "server seeds: GN_SEED_GREEDY_LEAF or GN_SEED_NEAREST_LEAF");
+    if (o.seed_mode != GN_SEED_GREEDY_LEAF && o.seed_mode != GN_SEED_NEAREST_LEAF &&
+        o.seed_mode != GN_SEED_GREEDY_STEP)
+        return fail(GN_EINVAL,
+                    "server seeds: GN_SEED_GREEDY_LEAF, GN_SEED_GREEDY_STEP or GN_SEED_NEAREST_LEAF");
     if (o.max_iter < 0 || o.fp32_iters < 0 || o.fp64_tail < 0 || o.max_iter > 10000 ||
         o.fp32_iters > 10000 || o.fp64_tail > 10000)
         return fail(GN_EINVAL, "iteration counts must be 0..10000");

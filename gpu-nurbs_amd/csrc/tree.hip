@@ -5,7 +5,12 @@
 // node (bx, by) of level L (g = 8^L per side) covers [bx/g, (bx+1)/g] x [by/g, (by+1)/g], is
 // sampled on a 9 x 9 grid, and gets the AABB of the samples enlarged by
 // K = (M1 + 2 M2 + M3) sqrt(3) / (8 (8g)^2), M* = max over samples of |S_uu|inf, |S_uv|inf,
-// |S_vv|inf, then is clamped to its parent's box.  The reference runs five launches per level
+// |S_vv|inf, then is clamped to its parent's box.  The reference lays the tree over [0,1]^2
+// (src/kernel/kernel.cu:383-387, 406-407), which is the parameter domain only when the knots
+// run from 0 to 1; here the unit square is mapped onto the knot domain [u0,u1] x [v0,v1]
+// (u = u0 + (u1-u0) x), and K takes the cell sizes (u1-u0)/(8g), (v1-v0)/(8g):
+// K = (M1 Wu^2 + 2 M2 Wu Wv + M3 Wv^2) sqrt(3) / (8 (8g)^2).  On [0,1] knots every mapped value
+// and K are bit-identical to the reference's (0 + 1 x = x, M 1 1 = M).  The reference runs five launches per level
 // (samples, three derivative passes, two LDS reductions) with a device-wide sync after each; here
 // one wave per node evaluates its 81 samples (lanes 0..63, then lanes 0..16 again) and reduces
 // them with cross-lane shuffles in a single launch per level.
@@ -54,11 +59,13 @@ __global__ __launch_bounds__(64) void k_tree_level(GnSurfDev d, int level, int f
     const long bx = idx / g, by = idx % g;
     const long ti = base + idx;
     const double base_unit = 1.0 / g, unit = base_unit / 8;
+    const double u0 = d.ku[d.p], wu = d.ku[d.nu] - u0, v0 = d.kv[d.q], wv = d.kv[d.nv] - v0;
     double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
     double M1 = 0, M2 = 0, M3 = 0;
     for (int sidx = threadIdx.x; sidx < 81; sidx += 64) {
         const int tx = sidx / 9, ty = sidx % 9;
-        const double su = base_unit * bx + tx * unit, sv = base_unit * by + ty * unit;
+        const double su = u0 + wu * (base_unit * bx + tx * unit),
+                     sv = v0 + wv * (base_unit * by + ty * unit);
         Cart<double> o;
         eval_point<K1, K2, 2, EXACT, double>(d, d.ku, d.kv, su, sv, o);
 #pragma unroll
@@ -79,13 +86,14 @@ __global__ __launch_bounds__(64) void k_tree_level(GnSurfDev d, int level, int f
     M2 = wave_max(M2);
     M3 = wave_max(M3);
     if (threadIdx.x != 0) return;
-    const double K = (M1 + 2 * M2 + M3) * sqrt(3.0) / (8.0 * 8.0 * (double)g * 8.0 * (double)g);
+    const double K = (M1 * wu * wu + 2 * M2 * wu * wv + M3 * wv * wv) * sqrt(3.0) /
+                     (8.0 * 8.0 * (double)g * 8.0 * (double)g);
     GnTreeNode nd;
     const double unitg = 1.0 / g;
-    nd.uv_lb[0] = bx * unitg;
-    nd.uv_lb[1] = by * unitg;
-    nd.uv_rt[0] = (bx + 1) * unitg;
-    nd.uv_rt[1] = (by + 1) * unitg;
+    nd.uv_lb[0] = u0 + wu * (bx * unitg);
+    nd.uv_lb[1] = v0 + wv * (by * unitg);
+    nd.uv_rt[0] = u0 + wu * ((bx + 1) * unitg);
+    nd.uv_rt[1] = v0 + wv * ((by + 1) * unitg);
     for (int c = 0; c < 3; c++) {
         nd.lb[c] = lo[c] - K;
         nd.rt[c] = hi[c] + K;
@@ -203,14 +211,45 @@ __global__ void k_search_box32(long total, const double *__restrict__ sbox,
     }
 }
 
+// S_u, S_v at every leaf's seed (u, v), for GN_SEED_GREEDY_STEP: one thread per leaf.
+template <int K1, int K2, bool EXACT>
+__global__ void k_search_jac(GnSurfDev d, long leaves, const double *__restrict__ sseed,
+                             double *__restrict__ sjac) {
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= leaves) return;
+    Cart<double> o;
+    eval_point<K1, K2, 1, EXACT, double>(d, d.ku, d.kv, sseed[c * 2], sseed[c * 2 + 1], o);
+    for (int k = 0; k < 3; k++) {
+        sjac[c * 6 + k] = o.Su[k];
+        sjac[c * 6 + 3 + k] = o.Sv[k];
+    }
+}
+
+struct JacLaunch {
+    const GnSurfDev &d;
+    long leaves;
+    const double *sseed;
+    double *sjac;
+    hipStream_t st;
+    template <int K1, int K2, bool EXACT>
+    int operator()() {
+        k_search_jac<K1, K2, EXACT><<<grid_for(leaves, 256, 1u << 30), 256, 0, st>>>(d, leaves, sseed,
+                                                                                   sjac);
+        GN_HIP(hipGetLastError());
+        return GN_OK;
+    }
+};
+
 int launch_search_build(const GnSurfDev &d, int levels, const GnTreeNode *tree, double *sbox,
                         float *sbox32, double *scentre, double *sseed, int32_t *sref,
-                        float4 *sleaf32, hipStream_t st) {
-    (void)d;
+                        float4 *sleaf32, double *sjac, hipStream_t st) {
     const int D = 3 * (levels - 1);
     const long leaves = 1L << (2 * D);
     k_search_leaves<<<grid_for(leaves, 256, 1u << 30), 256, 0, st>>>(levels, tree, sbox, scentre,
                                                                        sseed, sref, sleaf32);
+    GN_HIP(hipGetLastError());
+    const int rc = dispatch_degrees(d.p, d.q, JacLaunch{d, leaves, sseed, sjac, st});
+    if (rc) return rc;
     for (int l = D - 1; l >= 0; l--)
         k_search_level<<<grid_for(1L << (2 * l), 256, 1u << 30), 256, 0, st>>>(l, sbox);
     const long total = qoff(D + 1);
@@ -219,7 +258,8 @@ int launch_search_build(const GnSurfDev &d, int levels, const GnTreeNode *tree, 
     return GN_OK;
 }
 
-// One lane per query; mode 0 exact nearest leaf, mode 1 greedy + 3x3.
+// One lane per query; mode 0 exact nearest leaf, mode 1 greedy + 3x3, mode 2 greedy + 3x3 and
+// the Gauss-Newton step of step_seed.
 // Levels of the seed quadtree whose FP32 boxes the greedy seed kernel stages in LDS: levels
 // 1..5 are 1364 boxes, 32 KB.
 constexpr int kSeedLdsLevels = 5;
@@ -231,7 +271,7 @@ __global__ __launch_bounds__(256) void k_seed(GnSearchDev sd, int mode,
                                               int32_t *__restrict__ leaf, double *__restrict__ su,
                                               double *__restrict__ sv) {
     __shared__ __align__(16) float lbox[(((1 << (2 * (kSeedLdsLevels + 1))) - 1) / 3 - 1) * 6];
-    const int ll = mode == 1 ? (sd.depth < kSeedLdsLevels ? sd.depth : kSeedLdsLevels) : 0;
+    const int ll = mode != 0 ? (sd.depth < kSeedLdsLevels ? sd.depth : kSeedLdsLevels) : 0;
     if (ll > 0) {
         // boxes of levels 1..ll start one box (24 bytes) into box32: float2 copies
         const int nf2 = (int)(qoff(ll + 1) - 1) * 3;
@@ -246,8 +286,15 @@ __global__ __launch_bounds__(256) void k_seed(GnSearchDev sd, int mode,
         const double P[3] = {x[i], y[i], z[i]};
         const uint32_t c = mode == 0 ? nearest_leaf(sd, P) : greedy_leaf(sd, P, lbox, ll);
         if (leaf) leaf[i] = sd.ref_index[c];
-        su[i] = sd.seed_uv[c * 2 + 0];
-        sv[i] = sd.seed_uv[c * 2 + 1];
+        if (mode == 2) {
+            double u, v;
+            step_seed(sd, c, P, u, v);
+            su[i] = u;
+            sv[i] = v;
+        } else {
+            su[i] = sd.seed_uv[c * 2 + 0];
+            sv[i] = sd.seed_uv[c * 2 + 1];
+        }
     }
 }
 

@@ -27,7 +27,7 @@ LIB_PATH = os.path.join(_HERE, "libgnurbs.so")
 GN_OK, GN_EINVAL, GN_ENOMEM, GN_EDEVICE, GN_EIO, GN_ESTATE, GN_EBUDGET, GN_ENOTFOUND, GN_EAGAIN = range(9)
 GN_FP64, GN_FP32, GN_MIXED = 0, 1, 2
 PT_CONVERGED, PT_CONVERGED_DIST, PT_MAXITER, PT_SINGULAR, PT_NONFINITE, PT_NOSEED = range(6)
-SEED_NEAREST_LEAF, SEED_REF_FIRST_HIT, SEED_GIVEN, SEED_GREEDY_LEAF, SEED_GLOBAL = 0, 1, 2, 3, 4
+SEED_NEAREST_LEAF, SEED_REF_FIRST_HIT, SEED_GIVEN, SEED_GREEDY_LEAF, SEED_GLOBAL, SEED_GREEDY_STEP = 0, 1, 2, 3, 4, 5
 TREE_REF_PARENT = 1
 NEWTON_FP32, NEWTON_FP64 = 1, 2
 
@@ -77,6 +77,7 @@ _SIGS = [
     ("gn_get_span_matrix", _c_int, [_c_void, _c_int, _c_int, _c_void]),
     ("gn_evaluate", _c_int, [_c_void, _c_void, _c_void, _c_size, _c_int, _c_void, _c_void]),
     ("gn_evaluate_f32", _c_int, [_c_void, _c_void, _c_void, _c_size, _c_int, _c_void, _c_void]),
+    ("gn_evaluate_sorted_f32", _c_int, [_c_void, _c_void, _c_void, _c_size, _c_int, _c_void, _c_void, _c_void]),
     ("gn_evaluate_grid_f32", _c_int, [_c_void, _c_void, _c_int, _c_void, _c_int, _c_int, _c_void, _c_void]),
     ("gn_build_tree", _c_int, [_c_void, _c_int, _c_int, _c_void]),
     ("gn_get_tree", _c_int, [_c_void, _c_void, _c_void, _c_void, _c_size]),
@@ -281,6 +282,21 @@ class Surface:
 
     EvaluateRandom = Evaluate
 
+    def EvaluateSorted(self, u, v, order: int = 0, out=None, perm=None, stream=None):
+        """FP32 span-GEMM evaluation of points in any order (gn_evaluate_sorted_f32): returns
+        (out, perm) with column k of out the fields at input point perm[k]."""
+        torch = _torch()
+        ud, vd = to_device(u, torch.float32), to_device(v, torch.float32)
+        n = ud.numel()
+        nblk = (3, 9, 18)[order]
+        if out is None:
+            out = torch.empty((nblk, n), dtype=torch.float32, device="cuda")
+        if perm is None:
+            perm = torch.empty(n, dtype=torch.int32, device="cuda")
+        _check(lib().gn_evaluate_sorted_f32(self._h, ud.data_ptr(), vd.data_ptr(), n, order, out.data_ptr(),
+                                            perm.data_ptr(), _c_void(stream_handle(stream))))
+        return out, perm
+
     def EvaluateGrid(self, us, vs, order: int = 0, out=None, stream=None):
         """FP32 MFMA span-GEMM evaluation on the grid us x vs; returns (nblk, len(us)*len(vs))."""
         torch = _torch()
@@ -305,13 +321,16 @@ class Surface:
         return nodes, valid, divided
 
     def Seed(self, x, y, z, exact=True, stream=None):
+        """Seed search: exact=True the exact nearest leaf, False the greedy leaf, "step" the greedy
+        leaf plus one Gauss-Newton step (GN_SEED_GREEDY_STEP).  Returns (leaf, u, v)."""
         torch = _torch()
+        mode = 2 if exact == "step" else (1 if exact else 0)
         xd, yd, zd = (to_device(a, torch.float64) for a in (x, y, z))
         n = xd.numel()
         leaf = torch.empty(n, dtype=torch.int32, device="cuda")
         su = torch.empty(n, dtype=torch.float64, device="cuda")
         sv = torch.empty(n, dtype=torch.float64, device="cuda")
-        _check(lib().gn_seed(self._h, 1 if exact else 0, xd.data_ptr(), yd.data_ptr(), zd.data_ptr(), n,
+        _check(lib().gn_seed(self._h, mode, xd.data_ptr(), yd.data_ptr(), zd.data_ptr(), n,
                              leaf.data_ptr(), su.data_ptr(),
                              sv.data_ptr(), _c_void(stream_handle(stream))))
         return leaf, su, sv

@@ -61,13 +61,16 @@ typedef enum {
     GN_SEED_REF_FIRST_HIT = 1,/* reference NewInverse: index-order scan, first box hit     */
     GN_SEED_GIVEN = 2,        /* caller supplies seeds in u_out / v_out                    */
     GN_SEED_GREEDY_LEAF = 3,  /* greedy box-distance descent, then the nearest leaf centre */
-    GN_SEED_GLOBAL = 4        /* global projection (FP64): Newton from every leaf whose box
+    GN_SEED_GLOBAL = 4,       /* global projection (FP64): Newton from every leaf whose box
                                  could hold a nearer point (branch and bound over the seed tree),
                                  each clamped to its leaf cell, then polished; the nearest foot
                                  point wins.  iters_fp32 of gn_invert_iters receives the number
                                  of leaves solved.  The reference's normal-cone projection
                                  (src/kernel/kernel.cu:471-513, 623-704) is unfinished. */
-                              /* among the 3 x 3 leaves around it (fixed work per point)   */
+    GN_SEED_GREEDY_STEP = 5   /* GN_SEED_GREEDY_LEAF, then one Gauss-Newton step from the leaf's
+                                 seed with the Jacobian stored for that leaf (S_u, S_v at its
+                                 seed), limited to one leaf cell and clamped to the domain: no
+                                 surface evaluation, and about one iteration fewer after it */
 } gn_seed_mode;
 
 /* tree build flags */
@@ -157,6 +160,18 @@ int gn_evaluate_f32(const gn_surface *s, const float *u, const float *v, size_t 
 int gn_evaluate_grid_f32(const gn_surface *s, const float *us, int nu_pts, const float *vs,
                          int nv_pts, int max_order, float *out, gn_stream stream);
 
+/* FP32 span-GEMM form for points in any order (the MFMA counterpart of the FP16 cuBLAS span-GEMM
+ * experiment, src/main.cu:533-663, src/kernel/kernel.cu:1558-1604, for unordered points): the
+ * points are bucketed by span pair on the device (counting sort), each group of 16 points of one
+ * span pair gets its V-side contraction M_c * [V V' V''] from v_mfma_f32_16x16x4f32, and the
+ * results are written in bucket order: column k of out (SoA blocks of n values, as
+ * gn_evaluate_f32) holds the point whose input index is perm[k]; perm is a permutation of
+ * 0..n-1, and the order within one span pair is unspecified.  Degrees <= 3 use the matrix cores;
+ * higher degrees are evaluated per point in the same order.  max_order 0..2, n <= 2^32-1;
+ * allocates its scratch (16 bytes per point) stream-ordered. */
+int gn_evaluate_sorted_f32(const gn_surface *s, const float *u, const float *v, size_t n,
+                           int max_order, float *out, uint32_t *perm, gn_stream stream);
+
 /* ---- subdivision tree (GetPoints / GetSecondDerivatives / Max_Sequential_Addressing_Shared /
  * TightBoundingBox, src/kernel/kernel.cu:375-431, 544-621, 707-733) ------------------------- */
 
@@ -167,8 +182,9 @@ int gn_build_tree(gn_surface *s, int levels, int flags, gn_stream stream);
  * valid[count], divided[count]; count = (64^levels - 1)/63.  Synchronous. */
 int gn_get_tree(const gn_surface *s, double *nodes, uint8_t *valid, uint8_t *divided,
                 size_t count);
-/* Seed search alone: leaf index (reference numbering) and seed (u,v) per point; exact != 0
- * selects GN_SEED_NEAREST_LEAF, exact == 0 GN_SEED_GREEDY_LEAF. */
+/* Seed search alone: leaf index (reference numbering) and seed (u,v) per point; exact == 0
+ * selects GN_SEED_GREEDY_LEAF, exact == 2 GN_SEED_GREEDY_STEP, any other value
+ * GN_SEED_NEAREST_LEAF. */
 int gn_seed(const gn_surface *s, int exact, const double *x, const double *y, const double *z,
             size_t n, int32_t *leaf, double *seed_u, double *seed_v, gn_stream stream);
 
@@ -214,7 +230,8 @@ int gn_batch_invert(gn_batch *b, const gn_segment *segs, int nseg, const double 
                     double *u, double *v, uint8_t *status, gn_stream stream);
 /* Autotuner: times the candidate tile shapes on this work (outputs are overwritten) and keeps
  * the fastest; with cache_path, a shape stored for the same work signature is reused and a newly
- * tuned one is appended to the file. */
+ * tuned one is appended to the file.  With GN_SEED_GIVEN every timed run starts from the seeds
+ * u and v held on entry (a device copy is kept for the call). */
 int gn_batch_tune(gn_batch *b, const gn_segment *segs, int nseg, const double *x, const double *y,
                   const double *z, size_t n, const gn_invert_opts *opts, double *u, double *v,
                   uint8_t *status, const char *cache_path, gn_stream stream);

@@ -610,6 +610,10 @@ void or_build_tree(int p, int q, int nu, int nv, const double *ku, const double 
                    unsigned char *valid, unsigned char *divided)
 {
     or_surf sf = {p, q, nu, nv, ku, kv, MatrixN};
+    /* the reference lays the tree over [0,1]^2 (src/kernel/kernel.cu:383-387, 406-407); the
+     * library maps that square onto the knot domain, u = u0 + Wu x, with cell sizes Wu/(8g),
+     * Wv/(8g) in K -- identical to the reference on [0,1] knots */
+    double u0 = ku[p], wu = ku[nu] - u0, v0 = kv[q], wv = kv[nv] - v0;
     long g = 1;
     for (int L = 0; L < levels; L++, g *= OR_GAP) {
         long base = or_tree_base(L);
@@ -622,7 +626,8 @@ void or_build_tree(int p, int q, int nu, int nv, const double *ku, const double 
                 double M1 = 0, M2 = 0, M3 = 0;
                 for (int tx = 0; tx < OR_PPE; tx++)
                     for (int ty = 0; ty < OR_PPE; ty++) {
-                        double su = base_unit * bx + tx * unit, sv = base_unit * by + ty * unit;
+                        double su = u0 + wu * (base_unit * bx + tx * unit),
+                               sv = v0 + wv * (base_unit * by + ty * unit);
                         or_pt o;
                         eval_one(&sf, su, sv, 2, &o);
                         for (int c = 0; c < 3; c++) {
@@ -638,14 +643,14 @@ void or_build_tree(int p, int q, int nu, int nv, const double *ku, const double 
                         if (e3 > M3) M3 = e3;
                     }
                 (void)mx;
-                double K = (M1 + 2 * M2 + M3) * sqrt(3.0) /
+                double K = (M1 * wu * wu + 2 * M2 * wu * wv + M3 * wv * wv) * sqrt(3.0) /
                            (8.0 * OR_GAP * (double)g * OR_GAP * (double)g);
                 double *nd = nodes + ti * 13;
                 double unitg = 1.0 / g;
-                nd[0] = bx * unitg;
-                nd[1] = by * unitg;
-                nd[2] = (bx + 1) * unitg;
-                nd[3] = (by + 1) * unitg;
+                nd[0] = u0 + wu * (bx * unitg);
+                nd[1] = v0 + wv * (by * unitg);
+                nd[2] = u0 + wu * ((bx + 1) * unitg);
+                nd[3] = v0 + wv * ((by + 1) * unitg);
                 for (int c = 0; c < 3; c++) {
                     nd[4 + c] = lo[c] - K;
                     nd[7 + c] = hi[c] + K;
@@ -850,6 +855,47 @@ void or_seed_greedy_leaf(const double *nodes, int levels, const double *x, const
     for (int l = 0; l <= D; l++) { free(bx[l]); free(b32[l]); }
     free(bx);
     free(b32);
+}
+
+/* The library's GN_SEED_GREEDY_STEP (no reference counterpart): the greedy leaf, then one
+ * Gauss-Newton step from the leaf's seed (its uv centre, whose surface point is the node's
+ * centre C) with J = [S_u S_v] evaluated there, (du, dv) = (J^T J)^-1 J^T (C - P), scaled down
+ * uniformly to at most one leaf cell per direction and clamped to the knot domain; a singular
+ * J^T J keeps the leaf seed.  Written with the same explicit fma steps as the library. */
+void or_seed_greedy_step(int p, int q, int nu, int nv, const double *ku, const double *kv,
+                         const double *MatrixN, const double *nodes, int levels, const double *x,
+                         const double *y, const double *z, long n, int *leaf_out, double *seed_u,
+                         double *seed_v, int nthreads)
+{
+    or_surf sf = {p, q, nu, nv, ku, kv, MatrixN};
+    or_seed_greedy_leaf(nodes, levels, x, y, z, n, leaf_out, seed_u, seed_v, nthreads);
+    double u0 = ku[p], u1 = ku[nu], v0 = kv[q], v1 = kv[nv];
+    double cells = (double)(1L << (3 * (levels - 1)));
+    double hu = (u1 - u0) / cells, hv = (v1 - v0) / cells;
+#ifdef _OPENMP
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel for schedule(static) num_threads(nthreads)
+#endif
+    for (long i = 0; i < n; i++) {
+        const double *C = nodes + (long)leaf_out[i] * 13 + 10;
+        or_pt o;
+        eval_one(&sf, seed_u[i], seed_v[i], 1, &o);
+        const double *Su = o.Su, *Sv = o.Sv;
+        double r0 = C[0] - x[i], r1 = C[1] - y[i], r2 = C[2] - z[i];
+        double a = fma(Su[2], Su[2], fma(Su[1], Su[1], Su[0] * Su[0]));
+        double b = fma(Su[2], Sv[2], fma(Su[1], Sv[1], Su[0] * Sv[0]));
+        double e = fma(Sv[2], Sv[2], fma(Sv[1], Sv[1], Sv[0] * Sv[0]));
+        double gu = fma(Su[2], r2, fma(Su[1], r1, Su[0] * r0));
+        double gv = fma(Sv[2], r2, fma(Sv[1], r1, Sv[0] * r0));
+        double det = fma(a, e, -(b * b));
+        if (!(det > 0.0)) continue;
+        double du = fma(e, gu, -(b * gv)) / det;
+        double dv = fma(a, gv, -(b * gu)) / det;
+        double sc = fmax(1.0, fmax(fabs(du) / hu, fabs(dv) / hv));
+        if (!(sc < INFINITY)) continue;
+        seed_u[i] = fmin(fmax(seed_u[i] - du / sc, u0), u1);
+        seed_v[i] = fmin(fmax(seed_v[i] - dv / sc, v0), v1);
+    }
 }
 
 /* ------------------------------------------------------------------------------------------ */

@@ -53,6 +53,7 @@ def lib():
         L.or_project_global.argtypes = [_I, _I, _I, _I, _P, _P, _P, _P, _I, _P, _P, _P, _L, _I, _I, _D,
                                         _P, _P, _P, _I]
         L.or_seed_greedy_leaf.argtypes = [_P, _I, _P, _P, _P, _L, _P, _P, _P, _I]
+        L.or_seed_greedy_step.argtypes = [_I, _I, _I, _I, _P, _P, _P, _P, _I, _P, _P, _P, _L, _P, _P, _P, _I]
         L.or_curve_eval.argtypes = [_I, _I, _P, _P, _P, _P, _L, _I, _P, _I]
         L.or_curve_seed.argtypes = [_I, _I, _P, _P, _P, _P, _P, _P, _L, _P]
         L.or_curve_invert.argtypes = [_I, _I, _P, _P, _P, _P, _P, _P, _L, _D, _D, _I, _I, _I, _P, _P, _P,
@@ -148,6 +149,18 @@ class OSurf:
                                 z.ctypes.data, n, max_iter, leaf_iter, tol_step, u.ctypes.data, v.ctypes.data,
                                 st.ctypes.data, nthreads)
         return u, v, st
+
+    def seed_greedy_step(self, nodes, levels, x, y, z, nthreads=0):
+        """Greedy leaf, then one Gauss-Newton step with the Jacobian at the leaf's seed (the
+        library's GN_SEED_GREEDY_STEP)."""
+        nodes = _a(nodes)
+        x, y, z = (_a(a) for a in (x, y, z))
+        n = x.size
+        leaf = np.zeros(n, dtype=np.int32)
+        su, sv = np.zeros(n), np.zeros(n)
+        lib().or_seed_greedy_step(*self._args(), nodes.ctypes.data, levels, x.ctypes.data, y.ctypes.data,
+                                  z.ctypes.data, n, leaf.ctypes.data, su.ctypes.data, sv.ctypes.data, nthreads)
+        return leaf, su, sv
 
     def build_tree(self, levels, flags=0):
         cnt = lib().or_tree_base(levels)

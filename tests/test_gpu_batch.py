@@ -126,6 +126,47 @@ def test_batch_graph_replays_match_eager():
     g.close()
 
 
+def test_batch_graph_replays_beside_concurrent_eager_inversions():
+    """A graph keeps the claim-counter slots it was captured with.  They are its own: replays on
+    one stream while more than 256 eager mixed inversions (two counter slots each, so the shared
+    ring wraps twice) run on another stream give bit-identical results on both sides."""
+    import torch
+
+    defs, surfs = _objects(6, 17, with_generic=False)
+    counts = np.random.default_rng(5).integers(500, 3000, len(defs))
+    P, _, _ = _queries(defs, counts, 130)
+    b = G.Batch(surfs)
+    segs = G.segments(np.arange(len(defs)), counts)
+    opts = G.default_opts(**MIXED)
+    ref = [t.cpu().numpy() for t in b.invert(segs, P[0], P[1], P[2], opts)]
+    x, y, z = (torch.from_numpy(np.ascontiguousarray(a)).cuda() for a in P)
+    n = x.numel()
+    u = torch.zeros(n, dtype=torch.float64, device="cuda")
+    v = torch.zeros_like(u)
+    st = torch.zeros(n, dtype=torch.uint8, device="cuda")
+    s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
+    g = b.graph(segs, x, y, z, u, v, st, G.default_opts(**MIXED), stream=s1)
+    # eager work: 300 single-surface mixed inversions of one surface's points on s2
+    a0, c0 = int(segs["start"][0]), int(counts[0])
+    ex, ey, ez = x[a0:a0 + c0].contiguous(), y[a0:a0 + c0].contiguous(), z[a0:a0 + c0].contiguous()
+    eref = [t.clone() for t in surfs[0].NewInverse(ex, ey, ez, G.default_opts(**MIXED))]
+    outs = [(torch.empty(c0, dtype=torch.float64, device="cuda"), torch.empty(c0, dtype=torch.float64, device="cuda"),
+             torch.empty(c0, dtype=torch.uint8, device="cuda")) for _ in range(300)]
+    torch.cuda.synchronize()
+    for i in range(300):
+        if i % 15 == 0:
+            g.launch(stream=s1)
+        surfs[0].NewInverse(ex, ey, ez, G.default_opts(**MIXED), out=outs[i], stream=s2)
+    torch.cuda.synchronize()
+    got = [u.cpu().numpy(), v.cpu().numpy(), st.cpu().numpy()]
+    for a, c in zip(ref, got):
+        assert np.array_equal(a, c)
+    for i, o in enumerate(outs):
+        assert all(torch.equal(a, c) for a, c in zip(eref, o)), i
+    g.close()
+    b.close()
+
+
 def test_batch_tune_and_cache(tmp_path):
     defs, surfs = _objects(6, 13, with_generic=False)
     counts = np.random.default_rng(4).integers(1000, 4000, len(defs))
@@ -154,3 +195,33 @@ def test_batch_rejects_bad_segments():
     s = G.Surface(defs[0].p, defs[0].q, defs[0].ku, defs[0].kv, defs[0].cp, defs[0].w)
     with pytest.raises(G.GnError):
         G.Batch([s])  # no tree
+
+
+def test_batch_tune_with_given_seeds():
+    """The autotuner times every candidate shape from the seeds held in u and v on entry (it keeps
+    a copy), so it tunes given-seed work too; its final results equal an untuned call's from the
+    same seeds."""
+    import torch
+
+    defs, surfs = _objects(6, 19, with_generic=False)
+    counts = np.random.default_rng(6).integers(500, 3000, len(defs))
+    P, u0, v0 = _queries(defs, counts, 150)
+    rng = np.random.default_rng(7)
+    su = np.clip(u0 + rng.uniform(-0.01, 0.01, u0.size), 0, 1)
+    sv = np.clip(v0 + rng.uniform(-0.01, 0.01, v0.size), 0, 1)
+    b = G.Batch(surfs)
+    segs = G.segments(np.arange(len(defs)), counts)
+    opts = G.default_opts(seed_mode=G.SEED_GIVEN, **MIXED)
+
+    def bufs():
+        return (torch.from_numpy(su.copy()).cuda(), torch.from_numpy(sv.copy()).cuda(),
+                torch.empty(su.size, dtype=torch.uint8, device="cuda"))
+
+    tuned = bufs()
+    shape = b.tune(segs, P[0], P[1], P[2], opts, out=tuned)
+    assert shape[0] in (128, 256, 512, 1024, 4096)
+    ref = bufs()
+    b.invert(segs, P[0], P[1], P[2], opts, out=ref)
+    for a, c in zip(tuned, ref):
+        assert torch.equal(a, c)
+    b.close()

@@ -53,8 +53,12 @@ def cases():
         # oracle solutions from the greedy seeds: the reference iteration and the FP64 Newton truth
         gn = os_.gauss_newton(P[0], P[1], P[2], rsu, rsv, tol_step=1e-10, max_iter=20)
         nt = os_.gauss_newton(P[0], P[1], P[2], rsu, rsv, tol_step=1e-12, max_iter=40, newton=1)
+        # greedy leaf + one Gauss-Newton step with the leaf's Jacobian (GN_SEED_GREEDY_STEP)
+        sleaf, ssu, ssv = (_np(t) for t in gs.Seed(P[0], P[1], P[2], exact="step"))
+        tl, tsu, tsv = os_.seed_greedy_step(nodes, 3, P[0], P[1], P[2])
+        nts = os_.gauss_newton(P[0], P[1], P[2], tsu, tsv, tol_step=1e-12, max_iter=40, newton=1)
         out[(p, q)] = dict(sd=sd, os=os_, gs=gs, P=P, leaf=leaf, su=su, sv=sv, rleaf=rl, rsu=rsu, rsv=rsv,
-                           gn=gn, nt=nt)
+                           gn=gn, nt=nt, sleaf=sleaf, ssu=ssu, ssv=ssv, tleaf=tl, tsu=tsu, tsv=tsv, nts=nts)
     yield out
     for c in out.values():
         c["gs"].close()
@@ -87,6 +91,18 @@ def test_seeds_match_oracle(cases, pq):
 
 
 @pytest.mark.parametrize("pq", CLASSES, ids=lambda pq: f"p{pq[0]}q{pq[1]}")
+def test_step_seeds_match_oracle(cases, pq):
+    """GN_SEED_GREEDY_STEP: the same leaf as the oracle, and the stepped seed within 1e-12 of the
+    domain (the Jacobians at the leaf seeds come from two FP64 evaluations that round
+    differently)."""
+    c = cases[pq]
+    same = c["sleaf"] == c["tleaf"]
+    assert same.mean() > 0.999 and np.array_equal(c["sleaf"], c["leaf"])
+    assert np.max(np.abs(c["ssu"] - c["tsu"])[same]) <= 1e-12
+    assert np.max(np.abs(c["ssv"] - c["tsv"])[same]) <= 1e-12
+
+
+@pytest.mark.parametrize("pq", CLASSES, ids=lambda pq: f"p{pq[0]}q{pq[1]}")
 def test_single_surface_plain_and_mixed(cases, pq):
     c = cases[pq]
     P, gs = c["P"], c["gs"]
@@ -98,7 +114,7 @@ def test_single_surface_plain_and_mixed(cases, pq):
     _check_mixed(u, v, st, c["nt"], ("mixed", pq))
 
 
-@pytest.mark.parametrize("mode", ["plain", "mixed"])
+@pytest.mark.parametrize("mode", ["plain", "mixed", "mixed_step"])
 def test_batch_all_classes_against_oracle(cases, mode):
     """One gn_batch_invert over all classes (one persistent launch per class group and precision
     phase), seeds found inside the batch: each point against the oracle from the oracle's seeds."""
@@ -107,14 +123,20 @@ def test_batch_all_classes_against_oracle(cases, mode):
     P = np.concatenate([cases[k]["P"] for k in keys], axis=1)
     counts = np.full(len(keys), N)
     segs = G.segments(np.arange(len(keys)), counts)
-    opts = G.default_opts(**MIXED) if mode == "mixed" else G.default_opts(max_iter=20, tol_step=1e-10)
+    if mode == "plain":
+        opts = G.default_opts(max_iter=20, tol_step=1e-10)
+    else:
+        opts = G.default_opts(**MIXED)
+        if mode == "mixed_step":
+            opts.seed_mode = G.SEED_GREEDY_STEP
     u, v, st = (_np(t) for t in b.invert(segs, P[0], P[1], P[2], opts))
     for i, k in enumerate(keys):
         c = cases[k]
         sl = slice(i * N, (i + 1) * N)
         same_seed = c["leaf"] == c["rleaf"]
         ui, vi, sti = u[sl][same_seed], v[sl][same_seed], st[sl][same_seed]
-        ref = tuple(a[same_seed] for a in (c["gn"] if mode == "plain" else c["nt"]))
+        ref = tuple(a[same_seed] for a in (c["gn"] if mode == "plain" else c["nts"] if mode == "mixed_step"
+                                           else c["nt"]))
         if mode == "plain":
             _check_plain(ui, vi, sti, ref, ("batch", k))
         else:

@@ -147,3 +147,33 @@ def test_closed_form_detects_plausible_mistakes():
     assert np.allclose(variant(), good, rtol=0, atol=1e-14 * scale)
     for bad in (variant(kden=1), variant(ppe=10)):
         assert np.max(np.abs(nodes[:, 4:10] - bad)) > 1e-6 * scale
+
+
+def test_tree_on_a_non_unit_knot_domain_is_the_reparametrised_tree():
+    """The tree lies over the knot domain: reparametrising a surface affinely (knots a + W k)
+    leaves the geometry unchanged, so every node's box and centre point, and the valid/divided
+    flags, equal the [0,1] surface's, and its uv rectangle is the mapped one (S_uu scales by
+    1/Wu^2, which the cell size Wu/(8g) in K undoes)."""
+    from gnurbs import synth
+    sd = synth.surface_G(11, 9, 3, 2, seed=41, jitter=0.3)
+    base = O.OSurf.from_def(sd)
+    a, W, b, V = -2.0, 10.0, 3.0, 0.25
+    moved = O.OSurf(sd.p, sd.q, a + W * sd.ku, b + V * sd.kv, sd.cp, sd.w)
+    n0, v0, d0 = base.build_tree(LEVELS)
+    n1, v1, d1 = moved.build_tree(LEVELS)
+    assert np.allclose(n1[:, 0], a + W * n0[:, 0], rtol=0, atol=1e-13)
+    assert np.allclose(n1[:, 2], a + W * n0[:, 2], rtol=0, atol=1e-13)
+    assert np.allclose(n1[:, 1], b + V * n0[:, 1], rtol=0, atol=1e-15)
+    assert np.allclose(n1[:, 3], b + V * n0[:, 3], rtol=0, atol=1e-15)
+    scale = np.max(np.abs(n0[:, 4:]))
+    assert np.max(np.abs(n1[:, 4:] - n0[:, 4:])) <= 1e-12 * scale
+    # the flags agree except where a box extent ratio sits at NeedDivide's threshold within rounding
+    assert np.mean(v0 == v1) > 0.999 and np.mean(d0 == d1) > 0.999
+    # every leaf's samples lie in its box (the containment the seeds rely on)
+    leaf = n1[O.tree_base(LEVELS - 1):]
+    rng = np.random.default_rng(2)
+    for k in rng.choice(len(leaf), 200, replace=False):
+        uu = rng.uniform(leaf[k, 0], leaf[k, 2], 50)
+        vv = rng.uniform(leaf[k, 1], leaf[k, 3], 50)
+        S = moved.eval(uu, vv, 0)
+        assert np.all(S.T >= leaf[k, 4:7] - 1e-12) and np.all(S.T <= leaf[k, 7:10] + 1e-12)
