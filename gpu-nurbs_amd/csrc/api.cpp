@@ -67,9 +67,9 @@ static void free_tree(gn_surface *s) {
     (void)hipFree(s->d_sbox32);
     (void)hipFree(s->d_sref);
     (void)hipFree(s->d_sleaf32);
-    (void)hipFree(s->d_sjac);
+    (void)hipFree(s->d_sjac32);
     s->d_sleaf32 = nullptr;
-    s->d_sjac = nullptr;
+    s->d_sjac32 = nullptr;
     s->d_tree = nullptr;
     s->d_valid = s->d_divided = nullptr;
     s->d_sbox = s->d_scentre = s->d_sseed = nullptr;
@@ -330,7 +330,7 @@ int gn_build_tree(gn_surface *s, int levels, int flags, gn_stream stream) {
         (rc = dalloc(&s->d_sseed, (size_t)leaves * 2, acc)) ||
         (rc = dalloc(&s->d_sref, (size_t)leaves, acc)) ||
         (rc = dalloc(&s->d_sleaf32, (size_t)leaves, acc)) ||
-        (rc = dalloc(&s->d_sjac, (size_t)leaves * 6, acc))) {
+        (rc = dalloc(&s->d_sjac32, (size_t)leaves * 2, acc))) {
         free_tree(s);
         return rc;
     }
@@ -340,7 +340,7 @@ int gn_build_tree(gn_surface *s, int levels, int flags, gn_stream stream) {
         if (rc) { free_tree(s); return rc; }
     }
     rc = gn::launch_search_build(d, levels, s->d_tree, s->d_sbox, s->d_sbox32, s->d_scentre,
-                                 s->d_sseed, s->d_sref, s->d_sleaf32, s->d_sjac, st);
+                                 s->d_sseed, s->d_sref, s->d_sleaf32, s->d_sjac32, st);
     if (rc) { free_tree(s); return rc; }
     s->tree_levels = levels;
     s->device_bytes += acc;

@@ -68,10 +68,12 @@ __global__ __launch_bounds__(kSeedBlock) void k_batch_seed(const GnBatchObj *__r
         const GnReq &R = reqs[tl.req];
         const int64_t i = tl.start + j;
         const double P[3] = {R.x[i], R.y[i], R.z[i]};
-        const uint32_t c = mode == GN_SEED_NEAREST_LEAF ? nearest_leaf(sd, P) : greedy_leaf(sd, P, lbox, ll);
+        float4 cq;
+        const uint32_t c = mode == GN_SEED_NEAREST_LEAF ? nearest_leaf(sd, P)
+                                                        : greedy_leaf(sd, P, lbox, ll, &cq);
         if (mode == GN_SEED_GREEDY_STEP) {
             double u, v;
-            step_seed(sd, c, P, u, v);
+            step_seed(sd, c, cq, P, u, v);
             R.u[i] = u;
             R.v[i] = v;
         } else {

@@ -43,7 +43,8 @@ struct GnSearchDev {
     const int32_t *ref_index; // per leaf (Morton order) reference tree index
     const float4 *leaf32;     // per leaf (Morton order) centre in FP32 and the reference index
                               // (bits of w), one 16-byte load for the greedy 3 x 3 search
-    const double *jac;        // per leaf (Morton order) S_u, S_v at the leaf's seed (u, v)
+    const float4 *jac32;      // per leaf (Morton order) S_u, S_v at the leaf's seed (u, v) in
+                              // FP32, two 16-byte records (x, y, z, 0)
     double u0, u1, v0, v1;    // the knot domain the leaves tile
 };
 
@@ -118,7 +119,7 @@ struct gn_surface {
     float *d_sbox32 = nullptr;
     int32_t *d_sref = nullptr;
     float4 *d_sleaf32 = nullptr;
-    double *d_sjac = nullptr;
+    float4 *d_sjac32 = nullptr;
     size_t device_bytes = 0;
 
     GnSurfDev dev() const {
@@ -135,7 +136,7 @@ struct gn_surface {
         s.box = d_sbox; s.box32 = d_sbox32; s.centre = d_scentre; s.seed_uv = d_sseed;
         s.ref_index = d_sref;
         s.leaf32 = d_sleaf32;
-        s.jac = d_sjac;
+        s.jac32 = d_sjac32;
         s.u0 = ku.empty() ? 0.0 : ku[p];
         s.u1 = ku.empty() ? 1.0 : ku[nu];
         s.v0 = kv.empty() ? 0.0 : kv[q];
@@ -183,7 +184,7 @@ int launch_tree_level(const GnSurfDev &d, int level, int flags, GnTreeNode *tree
                       uint8_t *valid, uint8_t *divided, hipStream_t st);
 int launch_search_build(const GnSurfDev &d, int levels, const GnTreeNode *tree, double *sbox,
                         float *sbox32, double *scentre, double *sseed, int32_t *sref,
-                        float4 *sleaf32, double *sjac, hipStream_t st);
+                        float4 *sleaf32, float4 *sjac32, hipStream_t st);
 // mode 0: exact nearest leaf centre; mode 1: greedy descent + 3x3 leaf neighbourhood;
 // mode 2: mode 1, then one Gauss-Newton step from the leaf's seed with its stored Jacobian
 int launch_seed(const GnSearchDev &sd, int mode, const double *x, const double *y,

@@ -121,8 +121,10 @@ __device__ __forceinline__ uint32_t greedy_child(const float *__restrict__ level
 // distances.
 // lds, when given, holds the FP32 boxes of levels 1 .. lds_levels (the first part of box32 after
 // the root), so those levels of the descent read LDS.
+// cbest, when given, receives the chosen leaf's FP32 centre.
 __device__ inline uint32_t greedy_leaf(const GnSearchDev &sd, const double P[3],
-                                       const float *lds = nullptr, int lds_levels = 0) {
+                                       const float *lds = nullptr, int lds_levels = 0,
+                                       float4 *cbest = nullptr) {
     const int D = sd.depth;
     if (D == 0) return 0;
     const float p[3] = {(float)P[0], (float)P[1], (float)P[2]};
@@ -145,6 +147,7 @@ __device__ inline uint32_t greedy_leaf(const GnSearchDev &sd, const double P[3],
     float best = INFINITY;
     int best_ref = 0x7fffffff;
     uint32_t best_c = c;
+    float4 bq = q[4];
 #pragma unroll
     for (int k = 0; k < 9; k++) {
         const int x = gx + k / 3 - 1, y = gy + k % 3 - 1;
@@ -156,8 +159,10 @@ __device__ inline uint32_t greedy_leaf(const GnSearchDev &sd, const double P[3],
             best = dd;
             best_ref = ref;
             best_c = mk[k];
+            bq = q[k];
         }
     }
+    if (cbest) *cbest = bq;
     return best_c;
 }
 
@@ -165,31 +170,32 @@ __device__ inline uint32_t greedy_leaf(const GnSearchDev &sd, const double P[3],
 // the Jacobian J = [S_u S_v] stored for the leaf (evaluated at its seed, whose surface point is
 // the leaf's centre C), so no surface evaluation is needed:
 //   (du, dv) = (J^T J)^-1 J^T (C - P),
-// scaled down uniformly to at most one leaf cell in each direction and clamped to the domain.  A
-// singular J^T J keeps the leaf seed.  Every multiply-add is an explicit fma (the oracle does the
-// same), so the seed does not depend on the compiler's contraction.
-__device__ __forceinline__ void step_seed(const GnSearchDev &sd, uint32_t c, const double P[3],
-                                          double &su, double &sv) {
+// in FP32 (a seed needs no more), scaled down uniformly to at most one leaf cell in each
+// direction, applied to the FP64 seed and clamped to the domain.  A singular J^T J keeps the leaf
+// seed.  C is the leaf's FP32 centre (the one the greedy search compared), P the query rounded to
+// FP32.  Every multiply-add is an explicit fmaf and the divisions are IEEE, so the oracle can
+// restate the step operation for operation.
+__device__ __forceinline__ void step_seed(const GnSearchDev &sd, uint32_t c, const float4 &C,
+                                          const double P[3], double &su, double &sv) {
     su = sd.seed_uv[c * 2 + 0];
     sv = sd.seed_uv[c * 2 + 1];
-    const double *C = sd.centre + (size_t)c * 3;
-    const double *J = sd.jac + (size_t)c * 6;
-    const double r0 = C[0] - P[0], r1 = C[1] - P[1], r2 = C[2] - P[2];
-    const double a = fma(J[2], J[2], fma(J[1], J[1], J[0] * J[0]));
-    const double b = fma(J[2], J[5], fma(J[1], J[4], J[0] * J[3]));
-    const double e = fma(J[5], J[5], fma(J[4], J[4], J[3] * J[3]));
-    const double gu = fma(J[2], r2, fma(J[1], r1, J[0] * r0));
-    const double gv = fma(J[5], r2, fma(J[4], r1, J[3] * r0));
-    const double det = fma(a, e, -(b * b));
-    if (!(det > 0.0)) return;
-    const double du = fma(e, gu, -(b * gv)) / det;
-    const double dv = fma(a, gv, -(b * gu)) / det;
-    const double cells = (double)(1u << sd.depth);
-    const double hu = (sd.u1 - sd.u0) / cells, hv = (sd.v1 - sd.v0) / cells;
-    const double sc = fmax(1.0, fmax(fabs(du) / hu, fabs(dv) / hv));
+    const float4 ju = sd.jac32[c * 2], jv = sd.jac32[c * 2 + 1];
+    const float r0 = C.x - (float)P[0], r1 = C.y - (float)P[1], r2 = C.z - (float)P[2];
+    const float a = fmaf(ju.z, ju.z, fmaf(ju.y, ju.y, ju.x * ju.x));
+    const float b = fmaf(ju.z, jv.z, fmaf(ju.y, jv.y, ju.x * jv.x));
+    const float e = fmaf(jv.z, jv.z, fmaf(jv.y, jv.y, jv.x * jv.x));
+    const float gu = fmaf(ju.z, r2, fmaf(ju.y, r1, ju.x * r0));
+    const float gv = fmaf(jv.z, r2, fmaf(jv.y, r1, jv.x * r0));
+    const float det = fmaf(a, e, -(b * b));
+    if (!(det > 0.f)) return;
+    const float du = fmaf(e, gu, -(b * gv)) / det;
+    const float dv = fmaf(a, gv, -(b * gu)) / det;
+    const float cells = (float)(1u << sd.depth);
+    const float hu = (float)(sd.u1 - sd.u0) / cells, hv = (float)(sd.v1 - sd.v0) / cells;
+    const float sc = fmaxf(1.f, fmaxf(fabsf(du) / hu, fabsf(dv) / hv));
     if (!(sc < INFINITY)) return;
-    su = fmin(fmax(su - du / sc, sd.u0), sd.u1);
-    sv = fmin(fmax(sv - dv / sc, sd.v0), sd.v1);
+    su = fmin(fmax(su - (double)(du / sc), sd.u0), sd.u1);
+    sv = fmin(fmax(sv - (double)(dv / sc), sd.v0), sd.v1);
 }
 
 }  // namespace gn

@@ -214,27 +214,25 @@ __global__ void k_search_box32(long total, const double *__restrict__ sbox,
 // S_u, S_v at every leaf's seed (u, v), for GN_SEED_GREEDY_STEP: one thread per leaf.
 template <int K1, int K2, bool EXACT>
 __global__ void k_search_jac(GnSurfDev d, long leaves, const double *__restrict__ sseed,
-                             double *__restrict__ sjac) {
+                             float4 *__restrict__ sjac32) {
     const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= leaves) return;
     Cart<double> o;
     eval_point<K1, K2, 1, EXACT, double>(d, d.ku, d.kv, sseed[c * 2], sseed[c * 2 + 1], o);
-    for (int k = 0; k < 3; k++) {
-        sjac[c * 6 + k] = o.Su[k];
-        sjac[c * 6 + 3 + k] = o.Sv[k];
-    }
+    sjac32[c * 2] = make_float4((float)o.Su[0], (float)o.Su[1], (float)o.Su[2], 0.f);
+    sjac32[c * 2 + 1] = make_float4((float)o.Sv[0], (float)o.Sv[1], (float)o.Sv[2], 0.f);
 }
 
 struct JacLaunch {
     const GnSurfDev &d;
     long leaves;
     const double *sseed;
-    double *sjac;
+    float4 *sjac32;
     hipStream_t st;
     template <int K1, int K2, bool EXACT>
     int operator()() {
         k_search_jac<K1, K2, EXACT><<<grid_for(leaves, 256, 1u << 30), 256, 0, st>>>(d, leaves, sseed,
-                                                                                   sjac);
+                                                                                   sjac32);
         GN_HIP(hipGetLastError());
         return GN_OK;
     }
@@ -242,13 +240,13 @@ struct JacLaunch {
 
 int launch_search_build(const GnSurfDev &d, int levels, const GnTreeNode *tree, double *sbox,
                         float *sbox32, double *scentre, double *sseed, int32_t *sref,
-                        float4 *sleaf32, double *sjac, hipStream_t st) {
+                        float4 *sleaf32, float4 *sjac32, hipStream_t st) {
     const int D = 3 * (levels - 1);
     const long leaves = 1L << (2 * D);
     k_search_leaves<<<grid_for(leaves, 256, 1u << 30), 256, 0, st>>>(levels, tree, sbox, scentre,
                                                                        sseed, sref, sleaf32);
     GN_HIP(hipGetLastError());
-    const int rc = dispatch_degrees(d.p, d.q, JacLaunch{d, leaves, sseed, sjac, st});
+    const int rc = dispatch_degrees(d.p, d.q, JacLaunch{d, leaves, sseed, sjac32, st});
     if (rc) return rc;
     for (int l = D - 1; l >= 0; l--)
         k_search_level<<<grid_for(1L << (2 * l), 256, 1u << 30), 256, 0, st>>>(l, sbox);
@@ -284,11 +282,12 @@ __global__ __launch_bounds__(256) void k_seed(GnSearchDev sd, int mode,
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (size_t)gridDim.x * blockDim.x) {
         const double P[3] = {x[i], y[i], z[i]};
-        const uint32_t c = mode == 0 ? nearest_leaf(sd, P) : greedy_leaf(sd, P, lbox, ll);
+        float4 cq;
+        const uint32_t c = mode == 0 ? nearest_leaf(sd, P) : greedy_leaf(sd, P, lbox, ll, &cq);
         if (leaf) leaf[i] = sd.ref_index[c];
         if (mode == 2) {
             double u, v;
-            step_seed(sd, c, P, u, v);
+            step_seed(sd, c, cq, P, u, v);
             su[i] = u;
             sv[i] = v;
         } else {
@@ -302,8 +301,9 @@ int launch_seed(const GnSearchDev &sd, int mode, const double *x, const double *
                 const double *z, size_t n, int32_t *leaf, double *su, double *sv,
                 hipStream_t st) {
     if (n == 0) return GN_OK;
-    // greedy: a few long-lived workgroups per CU (each stages the upper tree levels once)
-    const unsigned cap = mode == 1 ? 256 * 4 : 256 * 32;
+    // greedy (with or without the step): a few long-lived workgroups per CU, each staging the
+    // upper tree levels in LDS once (a workgroup per 256 points would copy 32 KB per 256 points)
+    const unsigned cap = mode != 0 ? 256 * 4 : 256 * 32;
     k_seed<<<grid_for(n, 256, cap), 256, 0, st>>>(sd, mode, x, y, z, n, leaf, su, sv);
     GN_HIP(hipGetLastError());
     return GN_OK;

@@ -859,9 +859,10 @@ void or_seed_greedy_leaf(const double *nodes, int levels, const double *x, const
 
 /* The library's GN_SEED_GREEDY_STEP (no reference counterpart): the greedy leaf, then one
  * Gauss-Newton step from the leaf's seed (its uv centre, whose surface point is the node's
- * centre C) with J = [S_u S_v] evaluated there, (du, dv) = (J^T J)^-1 J^T (C - P), scaled down
- * uniformly to at most one leaf cell per direction and clamped to the knot domain; a singular
- * J^T J keeps the leaf seed.  Written with the same explicit fma steps as the library. */
+ * centre C) with J = [S_u S_v] evaluated there, (du, dv) = (J^T J)^-1 J^T (C - P), computed in
+ * FP32 from C, P and J rounded to FP32, scaled down uniformly to at most one leaf cell per
+ * direction, applied to the FP64 seed and clamped to the knot domain; a singular J^T J keeps the
+ * leaf seed.  Written with the same explicit fmaf steps as the library. */
 void or_seed_greedy_step(int p, int q, int nu, int nv, const double *ku, const double *kv,
                          const double *MatrixN, const double *nodes, int levels, const double *x,
                          const double *y, const double *z, long n, int *leaf_out, double *seed_u,
@@ -870,8 +871,8 @@ void or_seed_greedy_step(int p, int q, int nu, int nv, const double *ku, const d
     or_surf sf = {p, q, nu, nv, ku, kv, MatrixN};
     or_seed_greedy_leaf(nodes, levels, x, y, z, n, leaf_out, seed_u, seed_v, nthreads);
     double u0 = ku[p], u1 = ku[nu], v0 = kv[q], v1 = kv[nv];
-    double cells = (double)(1L << (3 * (levels - 1)));
-    double hu = (u1 - u0) / cells, hv = (v1 - v0) / cells;
+    float cells = (float)(1L << (3 * (levels - 1)));
+    float hu = (float)(u1 - u0) / cells, hv = (float)(v1 - v0) / cells;
 #ifdef _OPENMP
     if (nthreads <= 0) nthreads = omp_get_max_threads();
 #pragma omp parallel for schedule(static) num_threads(nthreads)
@@ -880,21 +881,23 @@ void or_seed_greedy_step(int p, int q, int nu, int nv, const double *ku, const d
         const double *C = nodes + (long)leaf_out[i] * 13 + 10;
         or_pt o;
         eval_one(&sf, seed_u[i], seed_v[i], 1, &o);
-        const double *Su = o.Su, *Sv = o.Sv;
-        double r0 = C[0] - x[i], r1 = C[1] - y[i], r2 = C[2] - z[i];
-        double a = fma(Su[2], Su[2], fma(Su[1], Su[1], Su[0] * Su[0]));
-        double b = fma(Su[2], Sv[2], fma(Su[1], Sv[1], Su[0] * Sv[0]));
-        double e = fma(Sv[2], Sv[2], fma(Sv[1], Sv[1], Sv[0] * Sv[0]));
-        double gu = fma(Su[2], r2, fma(Su[1], r1, Su[0] * r0));
-        double gv = fma(Sv[2], r2, fma(Sv[1], r1, Sv[0] * r0));
-        double det = fma(a, e, -(b * b));
-        if (!(det > 0.0)) continue;
-        double du = fma(e, gu, -(b * gv)) / det;
-        double dv = fma(a, gv, -(b * gu)) / det;
-        double sc = fmax(1.0, fmax(fabs(du) / hu, fabs(dv) / hv));
+        float ju[3] = {(float)o.Su[0], (float)o.Su[1], (float)o.Su[2]};
+        float jv[3] = {(float)o.Sv[0], (float)o.Sv[1], (float)o.Sv[2]};
+        float r0 = (float)C[0] - (float)x[i], r1 = (float)C[1] - (float)y[i],
+              r2 = (float)C[2] - (float)z[i];
+        float a = fmaf(ju[2], ju[2], fmaf(ju[1], ju[1], ju[0] * ju[0]));
+        float b = fmaf(ju[2], jv[2], fmaf(ju[1], jv[1], ju[0] * jv[0]));
+        float e = fmaf(jv[2], jv[2], fmaf(jv[1], jv[1], jv[0] * jv[0]));
+        float gu = fmaf(ju[2], r2, fmaf(ju[1], r1, ju[0] * r0));
+        float gv = fmaf(jv[2], r2, fmaf(jv[1], r1, jv[0] * r0));
+        float det = fmaf(a, e, -(b * b));
+        if (!(det > 0.f)) continue;
+        float du = fmaf(e, gu, -(b * gv)) / det;
+        float dv = fmaf(a, gv, -(b * gu)) / det;
+        float sc = fmaxf(1.f, fmaxf(fabsf(du) / hu, fabsf(dv) / hv));
         if (!(sc < INFINITY)) continue;
-        seed_u[i] = fmin(fmax(seed_u[i] - du / sc, u0), u1);
-        seed_v[i] = fmin(fmax(seed_v[i] - dv / sc, v0), v1);
+        seed_u[i] = fmin(fmax(seed_u[i] - (double)(du / sc), u0), u1);
+        seed_v[i] = fmin(fmax(seed_v[i] - (double)(dv / sc), v0), v1);
     }
 }
 
