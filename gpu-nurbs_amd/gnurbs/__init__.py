@@ -83,6 +83,7 @@ _SIGS = [
     ("gn_get_tree", _c_int, [_c_void, _c_void, _c_void, _c_void, _c_size]),
     ("gn_seed", _c_int, [_c_void, _c_int, _c_void, _c_void, _c_void, _c_size, _c_void, _c_void, _c_void, _c_void]),
     ("gn_invert", _c_int, [_c_void, _c_void, _c_void, _c_void, _c_size, ctypes.POINTER(InvertOpts), _c_void, _c_void, _c_void, _c_void]),
+    ("gn_invert_noalloc", _c_int, [_c_void, _c_void, _c_void, _c_void, _c_size, ctypes.POINTER(InvertOpts), _c_void, _c_void, _c_void, _c_void]),
     ("gn_invert_iters", _c_int, [_c_void, _c_void, _c_void, _c_void, _c_size, ctypes.POINTER(InvertOpts), _c_void, _c_void, _c_void, _c_void, _c_void, _c_void]),
     ("gn_pcurve", _c_int, [_c_void, _c_void, _c_size, _c_int, _c_int, ctypes.POINTER(InvertOpts), _c_void, _c_void,
                            ctypes.POINTER(_c_double), ctypes.POINTER(_c_size), _c_void]),
@@ -365,6 +366,28 @@ class Surface:
         return u, v, st, it32, it64
 
     Invert = NewInverse
+
+    def NewInverseNoAlloc(self, x, y, z, opts: Optional[InvertOpts] = None, seeds=None, out=None, stream=None):
+        """gn_invert_noalloc: NewInverse with the library's scratch in a fixed device array
+        instead of allocated memory (calls that overlap on one GPU race on it); no fallback.
+        Allocates nothing itself when x, y, z are device tensors and out is given."""
+        torch = _torch()
+        xd, yd, zd = (to_device(a, torch.float64) for a in (x, y, z))
+        n = xd.numel()
+        opts = opts if opts is not None else default_opts()
+        if out is not None:
+            u, v, st = out
+        else:
+            u = torch.empty(n, dtype=torch.float64, device="cuda")
+            v = torch.empty(n, dtype=torch.float64, device="cuda")
+            st = torch.empty(n, dtype=torch.uint8, device="cuda")
+        if seeds is not None:
+            u.copy_(to_device(seeds[0], torch.float64))
+            v.copy_(to_device(seeds[1], torch.float64))
+            opts.seed_mode = SEED_GIVEN
+        _check(lib().gn_invert_noalloc(self._h, xd.data_ptr(), yd.data_ptr(), zd.data_ptr(), n, ctypes.byref(opts),
+                                       u.data_ptr(), v.data_ptr(), st.data_ptr(), _c_void(stream_handle(stream))))
+        return u, v, st
 
     def PCurve(self, curve, samples: int, degree: int = 3, ncp: int = 16, opts: Optional[InvertOpts] = None,
                stream=None):

@@ -201,6 +201,19 @@ int gn_invert_iters(const gn_surface *s, const double *x, const double *y, const
                     size_t n, const gn_invert_opts *opts, double *u_out, double *v_out,
                     uint8_t *status, uint8_t *iters_fp32, uint8_t *iters_fp64, gn_stream stream);
 
+/* gn_invert without allocation.  No counterpart in the reference: it is for callers that cannot
+ * allocate device memory (code under graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The inversion kernels' scratch -- claim counters for two persistent launches and one
+ * continuation list of 2^17 point indices, 528 KB -- lives in a fixed device array of the library
+ * (one per GPU, held until the process ends) instead of memory the library allocates; the caller
+ * passes inputs and outputs as for gn_invert and gets the same results bit for bit.  Every call on
+ * a GPU shares that GPU's array: calls that overlap on one GPU (other streams, other threads) race
+ * and must not be issued.  Limits: n <= 2^32-1 points per call (GN_EINVAL beyond), opts->fallback
+ * must be 0 (the fallback allocates its list; GN_EINVAL). */
+int gn_invert_noalloc(const gn_surface *s, const double *x, const double *y, const double *z,
+                      size_t n, const gn_invert_opts *opts, double *u_out, double *v_out,
+                      uint8_t *status, gn_stream stream);
+
 /* ---- batched inversion: many surfaces in one persistent launch ---------------------------- */
 
 /* A run of points that belong to one object of a batch: points start .. start+count-1 of the
