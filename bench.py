@@ -25,6 +25,17 @@ PEAK_F32 = 157.3e12
 PEAK_F64 = 78.6e12
 
 
+# The mixed form's options: greedy leaf + one Gauss-Newton step with the leaf's stored Jacobian as
+# the seed; FP32 exact-Hessian Newton iterations to a 1e-5 step, at most 8 (the few points that need
+# more are the long dependent chains every wave of the persistent kernel would wait for: a cap of 20
+# cost 0.236 ms per 1M-point step against 0.205 ms, for 0.25% more points within 1e-9); then at most
+# 3 FP64 exact-Hessian closing steps until the Newton certificate bounds the (u,v) error by 1e-10,
+# or a 1e-10 step.  At most 11 iterations in all (the workload allows 20).
+def mixed_options(G):
+    return dict(precision=G.GN_MIXED, newton=G.NEWTON_FP32 | G.NEWTON_FP64, fp32_iters=8, fp64_tail=3,
+                tol_step32=1e-5, tol_step=1e-10, tol_quad=1e-10, seed_mode=G.SEED_GREEDY_STEP)
+
+
 def contract_flops_per_iter(k1, k2):
     """The per-iteration count the workload definition states: 2 (8 k1 k2 + 12 k1) + 40 flops
     (392 for bicubic): the order-1 contraction and the 2x2 Gauss-Newton solve."""
@@ -134,8 +145,10 @@ def bench_cpu_case(G, synth, torch, O, steps, threads):
 def bench_eval(G, synth, torch, steps):
     """Evaluation at scale: bicubic 64x64 surface (generator G, seed 1001), 16,777,216 uniform
     (u, v), point + derivatives to order 2.  FP64 per-point (160 B/pt), FP32 per-point (80 B/pt),
-    and the FP32 MFMA span-GEMM form on a 4096 x 4096 parameter grid (72 B/pt written).  FP32
-    error is measured against the FP64 evaluation (itself held to the FP64 oracle by the tests)."""
+    the FP32 MFMA span-GEMM form on the same unordered points (sorted by span pair on the device,
+    results in span order with the permutation) and on a 4096 x 4096 parameter grid (72 B/pt
+    written).  FP32 error is measured against the FP64 evaluation (itself held to the FP64 oracle
+    by the tests)."""
     import numpy as np
 
     n = 1 << 24
@@ -174,7 +187,26 @@ def bench_eval(G, synth, torch, steps):
         err.append(e.max().item() / scale)
     res["fp32"] = {"ms": ms, "points_per_s": n / (ms / 1e3), "GB_per_s": n * 80 / (ms / 1e3) / 1e9,
                    "hbm_frac": n * 80 / (ms / 1e3) / PEAK_HBM, "max_rel_err_vs_fp64": max(err)}
-    del ref
+    # the MFMA span-GEMM form for the same unordered points: counting sort by span pair, then the
+    # V-side contraction of every 16 points of one span pair on v_mfma_f32_16x16x4f32; outputs in
+    # span order with the permutation (84 B/pt: 8 in, 72 out, 4 perm; the sort adds 36 B/pt)
+    perm = torch.empty(n, dtype=torch.int32, device="cuda")
+    ms = timed(lambda: gs.EvaluateSorted(u32, v32, 2, out=out32, perm=perm))
+    p64 = perm.long()
+    refs = ref[:, p64]
+    err = []
+    for blk in range(6):
+        r = refs[3 * blk:3 * blk + 3]
+        e = (out32[3 * blk:3 * blk + 3].double() - r).abs()
+        scale = max(r.abs().max().item(), 1e-2 * refs[0:3].abs().max().item())
+        err.append(e.max().item() / scale)
+    del refs, p64
+    res["fp32_span_gemm_mfma_sorted"] = {
+        "ms": ms, "points_per_s": n / (ms / 1e3), "GB_per_s": n * 84 / (ms / 1e3) / 1e9,
+        "hbm_frac": n * 84 / (ms / 1e3) / PEAK_HBM, "hbm_frac_with_sort_traffic": n * 120 / (ms / 1e3) / PEAK_HBM,
+        "max_rel_err_vs_fp64": max(err),
+        "note": "outputs in span order + permutation (gn_evaluate_sorted_f32); includes the device counting sort"}
+    del ref, perm
     side = 4096
     us = torch.linspace(0, 1, side, dtype=torch.float32, device="cuda")
     ms = timed(lambda: gs.EvaluateGrid(us, us, 2, out=out32))
@@ -219,8 +251,7 @@ def bench_batch(G, synth, torch, steps, count=1024):
     u = torch.empty(n, dtype=torch.float64, device="cuda")
     v = torch.empty_like(u)
     st = torch.empty(n, dtype=torch.uint8, device="cuda")
-    mixed = dict(precision=G.GN_MIXED, newton=G.NEWTON_FP32 | G.NEWTON_FP64, fp32_iters=20, fp64_tail=3,
-                 tol_step32=1e-4, tol_step=1e-10, tol_quad=1e-10, seed_mode=G.SEED_GREEDY_STEP)
+    mixed = mixed_options(G)
     shape = b.tune(segs, x, y, z, G.default_opts(**mixed), out=(u, v, st))
     stream = torch.cuda.Stream()
     res = {"workload": f"{count} surfaces (degrees 2..5, nets 4..48, jittered knots), {n} points "
@@ -365,8 +396,7 @@ def bench_serve(G, synth, torch, duration=1.0, loads=(0.5, 0.8), budget=32 << 30
                 "mean_step_ms": float(dur.mean() * 1e3), "fixed_ms": float(a * 1e3),
                 "in_step_points_per_s": float(1.0 / b) if b > 0 else None}
 
-    mixed = dict(precision=G.GN_MIXED, newton=G.NEWTON_FP32 | G.NEWTON_FP64, fp32_iters=20, fp64_tail=3,
-                 tol_step32=1e-4, tol_step=1e-10, tol_quad=1e-10, seed_mode=G.SEED_GREEDY_STEP)
+    mixed = mixed_options(G)
     plain = dict(precision=G.GN_FP64, max_iter=20, tol_step=1e-10)
     try:
         for name, opts in (("mixed", mixed), ("plain_fp64", plain)):
@@ -475,12 +505,8 @@ def main():
     st = torch.empty(N, dtype=torch.uint8, device=dev)
     torch.cuda.synchronize()
 
-    # mixed: greedy leaf + one Gauss-Newton step with the leaf's stored Jacobian as the seed, FP32
-    # exact-Hessian bulk to a 1e-4 step, then FP64 exact-Hessian closing steps (<= 3) until the
-    # Newton certificate bounds the (u,v) error by 1e-10, or a 1e-10 step
-    mixed_opts = dict(precision=G.GN_MIXED, newton=G.NEWTON_FP32 | G.NEWTON_FP64, fp32_iters=20,
-                      fp64_tail=3, tol_step32=1e-4, tol_step=1e-10, tol_quad=1e-10,
-                      seed_mode=G.SEED_GREEDY_STEP)
+    # mixed: mixed_options (stepped seeds, <= 8 FP32 Newton iterations, <= 3 FP64 closing steps)
+    mixed_opts = mixed_options(G)
     variants = {
         "mixed": G.default_opts(**mixed_opts),
         # the reference iteration (J^T J Gauss-Newton, src/kernel/kernel.cu:151-249) in FP64 from the
@@ -649,7 +675,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": "projection: bicubic rational surface, 32x32 net (generator G, seed 1002), "
                                f"{N} queries offset N(0,1e-2) along the normal, depth-6 tree (64x64 leaves) "
-                               "greedy-leaf seeds + one Gauss-Newton step (leaf Jacobian), <=20 FP32 + <=3 FP64 "
+                               "greedy-leaf seeds + one Gauss-Newton step (leaf Jacobian), <=8 FP32 + <=3 FP64 "
                                "iterations",
                    "points": N, "surface": "p=q=3, 32x32", "tree_depth": 6},
         "roofline": roof,

@@ -4,6 +4,11 @@
 // first step and hands the few points that continue to a lane-refill pass over a list.
 #include "invert_common.hpp"
 
+// Points per wave of the list pass's grid (the list's length is known only on the device).
+#ifndef GN_LIST_WAVE_POINTS
+#define GN_LIST_WAVE_POINTS 1024
+#endif
+
 namespace gn {
 
 // The first FP64 closing step of every point, streamed: almost every point stops after it (the
@@ -152,7 +157,7 @@ struct MixedLaunch {
                 // points, 16 us at one per 1,024 or 512)
                 refill_shape((const void *)k, a.n, grid, slices);
                 slices = kMaxSlices;
-                const size_t gl = a.n / (1024 * (kInvBlock / 64));
+                const size_t gl = a.n / ((size_t)GN_LIST_WAVE_POINTS * (kInvBlock / 64));
                 const unsigned gmin = kMaxSlices / (kInvBlock / 64);
                 if (gl < grid) grid = (unsigned)gl;
                 if (grid < gmin) grid = gmin;
