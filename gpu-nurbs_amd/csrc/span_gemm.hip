@@ -6,19 +6,23 @@
 // (V_j = [1, s_j, s_j^2, s_j^3] of point j, or its first or second derivative), one
 // v_mfma_f32_16x16x4f32 each for V, V' and V''.  What is left per point is the U-side dot
 // products R^(a,b)_c = U^(a)_j . Y^(b)_c[., j] and the rational quotient, on the vector ALUs.
-// Unordered points rarely share a span within a wave, so they are first bucketed by span pair
-// with a counting sort (the span search compares the point's parameters, widened to FP64, with
-// the FP64 knots, as every other path does):
-//   1. k_sg_count    per chunk of points, an LDS histogram of span keys, stored per chunk;
-//   2. k_sg_colscan  per key, each chunk's first slot within the key (a column scan over the
-//      k_sg_keyscan  chunks), then the keys' starts (one scan over the key totals);
-//   3. k_sg_scatter  each point to its key's next slot of its chunk (LDS cursors): the sorted
-//                    parameters and perm[k] = the input index of sorted point k;
-//   4. k_sg_eval     one wave per 64 sorted points: per 16-point group and span pair in it, the
+// Unordered points rarely share a span within a wave, so they are first grouped by span pair:
+//   1. k_sg_sort     one workgroup per block of kSgBlockPoints consecutive input points: an LDS
+//                    histogram of the span-pair keys, its exclusive scan, then each point to its
+//                    key's next slot within the block's own output range (LDS cursors), writing the
+//                    sorted parameters and perm[k] = the input index of output k.  The span search
+//                    compares the FP32 parameters, widened to FP64, with the FP64 knots, as every
+//                    other path does.  Keeping each block in its own range keeps the scattered
+//                    writes inside a window the caches hold (a global counting sort over all the
+//                    points wrote runs of a few points per key and cache line, and took 1.2 ms of
+//                    write-backs at 16.8M points); a block of 65,536 points still gives runs of ~18
+//                    points per key on a 61 x 61-span surface.
+//   2. k_sg_eval     one wave per 64 sorted points: per 16-point group and span pair in it, the
 //                    three MFMAs, the U-side dots, a transpose through LDS, the quotient; outputs
 //                    are written in sorted order (coalesced).
-// Outputs follow the sorted order, which the caller maps back through perm; the order of points
-// within one key and chunk is unspecified (LDS atomics), each point's values are not.  This form
+// Outputs follow the sorted order, which the caller maps back through perm: within each block of
+// kSgBlockPoints consecutive input points, grouped by span pair in increasing key order; the order
+// within one key of a block is unspecified (LDS atomics), each point's values are not.  This form
 // takes degrees up to 3 in each direction (one 16 x 4 tile holds the four components' rows);
 // higher degrees use the per-point kernel and the sort alone.
 #include "gn_device.hpp"
@@ -29,14 +33,9 @@ int cu_count();  // runtime.hip
 
 typedef float f4v __attribute__((ext_vector_type(4)));
 
-constexpr int kSgBlock = 256;
-constexpr int kSgMaxLdsKeys = 12288;  // 48 KB of LDS cursors; more keys use global atomics
-
-struct SgShape {
-    int chunks;        // chunks of consecutive input points (one workgroup each)
-    size_t chunk;      // points per chunk
-    int keys;          // span pairs: su * sv
-};
+constexpr int kSgSortThreads = 1024;
+constexpr size_t kSgBlockPoints = 65536;
+constexpr int kSgMaxLdsKeys = 12288;  // 48 KB of LDS counters; more keys count in global memory
 
 // Span pair key and local parameters of an FP32 point.
 __device__ __forceinline__ int sg_key(const GnSurfDev &d, double inv_u, double inv_v, float uf,
@@ -54,92 +53,76 @@ __device__ __forceinline__ double sg_inv(const double *k, int p, int cp) {
     return (double)(cp - p) / (k[cp] - k[p]);  // the uniform-knot guess of find_span_guess
 }
 
-__global__ __launch_bounds__(kSgBlock) void k_sg_count(GnSurfDev d, const float *__restrict__ u,
-                                                       const float *__restrict__ v, size_t n,
-                                                       SgShape sh, uint32_t *__restrict__ wcount) {
-    extern __shared__ uint32_t hist[];
+// One workgroup per block of kSgBlockPoints input points.  cnt: the block's key counters (LDS,
+// or, for more than kSgMaxLdsKeys keys, its zeroed slice of gcnt).  Loads are issued four points
+// per thread at a time, so the block's latency overlaps.
+__global__ __launch_bounds__(kSgSortThreads) void k_sg_sort(GnSurfDev d, const float *__restrict__ u,
+                                                            const float *__restrict__ v, size_t n,
+                                                            int keys, uint32_t *__restrict__ gcnt,
+                                                            float *__restrict__ su,
+                                                            float *__restrict__ sv,
+                                                            uint32_t *__restrict__ perm) {
+    extern __shared__ uint32_t lcnt[];
+    __shared__ uint32_t part[kSgSortThreads];
     const double inv_u = sg_inv(d.ku, d.p, d.nu), inv_v = sg_inv(d.kv, d.q, d.nv);
-    const bool lds = sh.keys <= kSgMaxLdsKeys;
-    uint32_t *out = wcount + (size_t)blockIdx.x * sh.keys;
+    const bool lds = keys <= kSgMaxLdsKeys;
+    uint32_t *cnt = lds ? lcnt : gcnt + (size_t)blockIdx.x * keys;
     if (lds)
-        for (int k = threadIdx.x; k < sh.keys; k += kSgBlock) hist[k] = 0u;
+        for (int k = threadIdx.x; k < keys; k += kSgSortThreads) cnt[k] = 0u;
     __syncthreads();
-    const size_t i0 = (size_t)blockIdx.x * sh.chunk;
-    const size_t i1 = i0 + sh.chunk < n ? i0 + sh.chunk : n;
-    for (size_t i = i0 + threadIdx.x; i < i1; i += kSgBlock) {
-        const int key = sg_key(d, inv_u, inv_v, u[i], v[i]);
-        if (lds) atomicAdd(&hist[key], 1u);
-        else atomicAdd(&out[key], 1u);  // zeroed by the launcher
+    const size_t i0 = (size_t)blockIdx.x * kSgBlockPoints;
+    const size_t i1 = i0 + kSgBlockPoints < n ? i0 + kSgBlockPoints : n;
+    constexpr int U = 4;
+    for (size_t b = i0 + threadIdx.x; b < i1; b += (size_t)U * kSgSortThreads) {
+        float uu[U], vv[U];
+#pragma unroll
+        for (int k = 0; k < U; k++) {
+            const size_t i = b + (size_t)k * kSgSortThreads;
+            uu[k] = i < i1 ? u[i] : 0.f;
+            vv[k] = i < i1 ? v[i] : 0.f;
+        }
+#pragma unroll
+        for (int k = 0; k < U; k++)
+            if (b + (size_t)k * kSgSortThreads < i1) atomicAdd(&cnt[sg_key(d, inv_u, inv_v, uu[k], vv[k])], 1u);
     }
     __syncthreads();
-    if (lds)
-        for (int k = threadIdx.x; k < sh.keys; k += kSgBlock) out[k] = hist[k];
-}
-
-// Column scan: thread k turns chunk counts of key k into offsets within the key (chunk order),
-// and writes the key's total.
-__global__ __launch_bounds__(kSgBlock) void k_sg_colscan(SgShape sh, uint32_t *__restrict__ wcount,
-                                                         uint32_t *__restrict__ total) {
-    const int k = blockIdx.x * kSgBlock + threadIdx.x;
-    if (k >= sh.keys) return;
-    uint32_t run = 0;
-    for (int w = 0; w < sh.chunks; w++) {
-        uint32_t *p = wcount + (size_t)w * sh.keys + k;
-        const uint32_t c = *p;
-        *p = run;
-        run += c;
-    }
-    total[k] = run;
-}
-
-// Exclusive scan of the key totals (one workgroup): start[k], start[keys] = n.
-__global__ __launch_bounds__(1024) void k_sg_keyscan(int keys, const uint32_t *__restrict__ total,
-                                                     uint32_t *__restrict__ start) {
-    __shared__ uint32_t part[1024];
-    const int per = (keys + 1023) / 1024;
+    // exclusive scan of the counters: per thread a run of consecutive keys, then the runs' sums
+    const int per = (keys + kSgSortThreads - 1) / kSgSortThreads;
     const int k0 = threadIdx.x * per;
     uint32_t sum = 0;
-    for (int k = k0; k < k0 + per && k < keys; k++) sum += total[k];
+    for (int k = k0; k < k0 + per && k < keys; k++) sum += cnt[k];
     part[threadIdx.x] = sum;
     __syncthreads();
-    for (int o = 1; o < 1024; o <<= 1) {  // inclusive Hillis-Steele scan
+    for (int o = 1; o < kSgSortThreads; o <<= 1) {  // inclusive Hillis-Steele scan of the sums
         const uint32_t x = threadIdx.x >= (unsigned)o ? part[threadIdx.x - o] : 0u;
         __syncthreads();
         part[threadIdx.x] += x;
         __syncthreads();
     }
-    uint32_t run = threadIdx.x ? part[threadIdx.x - 1] : 0u;
+    uint32_t run = (uint32_t)i0 + (threadIdx.x ? part[threadIdx.x - 1] : 0u);
     for (int k = k0; k < k0 + per && k < keys; k++) {
-        start[k] = run;
-        run += total[k];
+        const uint32_t c = cnt[k];
+        cnt[k] = run;
+        run += c;
     }
-    if (threadIdx.x == 1023) start[keys] = part[1023];
-}
-
-__global__ __launch_bounds__(kSgBlock) void k_sg_scatter(GnSurfDev d, const float *__restrict__ u,
-                                                         const float *__restrict__ v, size_t n,
-                                                         SgShape sh,
-                                                         uint32_t *__restrict__ wcount,
-                                                         const uint32_t *__restrict__ start,
-                                                         float *__restrict__ su,
-                                                         float *__restrict__ sv,
-                                                         uint32_t *__restrict__ perm) {
-    extern __shared__ uint32_t cur[];
-    const double inv_u = sg_inv(d.ku, d.p, d.nu), inv_v = sg_inv(d.kv, d.q, d.nv);
-    const bool lds = sh.keys <= kSgMaxLdsKeys;
-    uint32_t *base = wcount + (size_t)blockIdx.x * sh.keys;
-    if (lds)
-        for (int k = threadIdx.x; k < sh.keys; k += kSgBlock) cur[k] = start[k] + base[k];
     __syncthreads();
-    const size_t i0 = (size_t)blockIdx.x * sh.chunk;
-    const size_t i1 = i0 + sh.chunk < n ? i0 + sh.chunk : n;
-    for (size_t i = i0 + threadIdx.x; i < i1; i += kSgBlock) {
-        const float uf = u[i], vf = v[i];
-        const int key = sg_key(d, inv_u, inv_v, uf, vf);
-        const uint32_t pos = lds ? atomicAdd(&cur[key], 1u) : start[key] + atomicAdd(&base[key], 1u);
-        su[pos] = uf;
-        sv[pos] = vf;
-        perm[pos] = (uint32_t)i;
+    for (size_t b = i0 + threadIdx.x; b < i1; b += (size_t)U * kSgSortThreads) {
+        float uu[U], vv[U];
+#pragma unroll
+        for (int k = 0; k < U; k++) {
+            const size_t i = b + (size_t)k * kSgSortThreads;
+            uu[k] = i < i1 ? u[i] : 0.f;
+            vv[k] = i < i1 ? v[i] : 0.f;
+        }
+#pragma unroll
+        for (int k = 0; k < U; k++) {
+            const size_t i = b + (size_t)k * kSgSortThreads;
+            if (i >= i1) continue;
+            const uint32_t pos = atomicAdd(&cnt[sg_key(d, inv_u, inv_v, uu[k], vv[k])], 1u);
+            su[pos] = uu[k];
+            sv[pos] = vv[k];
+            perm[pos] = (uint32_t)i;
+        }
     }
 }
 
@@ -268,30 +251,24 @@ int launch_eval_sorted32(const GnSurfDev &d, const float *u, const float *v, siz
                          float *out, uint32_t *perm, hipStream_t st) {
     if (n == 0) return GN_OK;
     if (n > (size_t)UINT32_MAX) return fail(GN_EINVAL, "span-order evaluation takes at most 2^32-1 points");
-    SgShape sh;
-    sh.keys = d.su * d.sv;
-    sh.chunks = 2 * cu_count();
-    sh.chunk = (n + sh.chunks - 1) / sh.chunks;
-    sh.chunks = (int)((n + sh.chunk - 1) / sh.chunk);
-    const size_t wc = (size_t)sh.chunks * sh.keys;
-    // scratch: sorted parameters, per-chunk counts, key totals and starts (stream-ordered)
-    const size_t bytes = sizeof(float) * 2 * n + sizeof(uint32_t) * (wc + 2 * (size_t)sh.keys + 1) + 256;
+    const int keys = d.su * d.sv;
+    const size_t blocks = (n + kSgBlockPoints - 1) / kSgBlockPoints;
+    const bool lds = keys <= kSgMaxLdsKeys;
+    // scratch: sorted parameters, and the blocks' counters when they do not fit LDS (stream-ordered)
+    const size_t gcnt_words = lds ? 0 : blocks * (size_t)keys;
+    const size_t bytes = sizeof(float) * 2 * n + sizeof(uint32_t) * gcnt_words + 256;
     char *buf = nullptr;
     GN_HIP(hipMallocAsync((void **)&buf, bytes, st));
     float *su = (float *)buf, *sv = su + n;
-    uint32_t *wcount = (uint32_t *)(sv + n), *total = wcount + wc, *start = total + sh.keys;
-    const bool lds = sh.keys <= kSgMaxLdsKeys;
-    const size_t shmem = lds ? sizeof(uint32_t) * sh.keys : 0;
+    uint32_t *gcnt = (uint32_t *)(sv + n);
     hipError_t e = hipSuccess;
-    if (!lds) e = hipMemsetAsync(wcount, 0, sizeof(uint32_t) * wc, st);
+    if (!lds) e = hipMemsetAsync(gcnt, 0, sizeof(uint32_t) * gcnt_words, st);
     if (e == hipSuccess) {
-        k_sg_count<<<sh.chunks, kSgBlock, shmem, st>>>(d, u, v, n, sh, wcount);
-        k_sg_colscan<<<(sh.keys + kSgBlock - 1) / kSgBlock, kSgBlock, 0, st>>>(sh, wcount, total);
-        k_sg_keyscan<<<1, 1024, 0, st>>>(sh.keys, total, start);
-        k_sg_scatter<<<sh.chunks, kSgBlock, shmem, st>>>(d, u, v, n, sh, wcount, start, su, sv, perm);
+        k_sg_sort<<<(unsigned)blocks, kSgSortThreads, lds ? sizeof(uint32_t) * keys : 0, st>>>(
+            d, u, v, n, keys, gcnt, su, sv, perm);
         e = hipGetLastError();
     }
-    int rc = e == hipSuccess ? GN_OK : hip_fail(e, "span sort launches");
+    int rc = e == hipSuccess ? GN_OK : hip_fail(e, "span sort launch");
     if (!rc) {
         if (d.p <= 3 && d.q <= 3) {
             const unsigned grid = grid_for(n, 256, (unsigned)cu_count() * 8);

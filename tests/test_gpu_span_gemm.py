@@ -1,8 +1,9 @@
 """gn_evaluate_sorted_f32: the FP32 span-GEMM form for points in any order (counting sort by span
 pair, v_mfma_f32_16x16x4f32 for the V-side contraction of each 16-point group of one span pair).
 
-Checks: perm is a permutation of the input indices and the outputs are grouped by span pair in
-increasing key order (keys from the oracle's findSpan, src/NURBS/Evaluate.cu:504-533); every field
+Checks: perm is a permutation of the input indices, each block of 65,536 consecutive input points
+keeps its own output range, grouped by span pair in increasing key order (keys from the oracle's
+findSpan, src/NURBS/Evaluate.cu:504-533); every field
 at sorted point k equals the FP64 oracle at input point perm[k] within 1e-5 of the field scale
 (the north star's FP32 bound), for degrees 1..3 on the matrix cores and higher degrees on the
 per-point fallback, for orders 0..2, on jittered and repeated knots, knots outside [0,1], a span
@@ -20,6 +21,7 @@ from helpers import blocks, field_close
 pytestmark = pytest.mark.gpu
 
 FIELDS = ("S", "Su", "Sv", "Suu", "Suv", "Svv")
+BLOCK = 65536  # input points per sort block
 
 
 def _np(t):
@@ -41,7 +43,9 @@ def _check(sd, u, v, order):
     assert np.array_equal(np.sort(perm), np.arange(n))
     us, vs = u32[perm].astype(np.float64), v32[perm].astype(np.float64)
     k = _keys(sd, us, vs)
-    assert np.all(np.diff(k) >= 0)  # grouped by span pair, keys increasing
+    for b0 in range(0, n, BLOCK):  # each block of input points sorted within its own range
+        assert np.all(perm[b0:b0 + BLOCK] // BLOCK == b0 // BLOCK)
+        assert np.all(np.diff(k[b0:b0 + BLOCK]) >= 0)  # grouped by span pair, keys increasing
     ref = blocks(O.OSurf.from_def(sd).eval(us, vs, order))
     got = blocks(out)
     floor = 1e-2 * np.max(np.abs(ref["S"]))
@@ -119,6 +123,8 @@ def test_sorted_16M_points_against_per_point_kernel_and_oracle():
     seen.index_add_(0, p64, torch.ones(n, dtype=torch.int32, device="cuda"))
     assert bool((seen == 1).all())
     del seen
+    blk = torch.arange(n, device="cuda") // BLOCK
+    assert bool((p64 // BLOCK == blk).all())
     ref = gs.Evaluate(u[p64].contiguous(), v[p64].contiguous(), 2, precision=G.GN_FP32)
     for b in range(6):
         r = ref[3 * b:3 * b + 3]
