@@ -98,7 +98,8 @@ __device__ __forceinline__ int step_group(int cls, const GnSurfDev &d, T Px, T P
         if constexpr (class_group((P - 2) * 4 + (Q - 2)) == GRP)                                \
             return gn_step<P + 1, Q + 1, true, T, NEWTON,                                       \
                            WAVE ? kPatchWavePipe                                                \
-                                : (PREG && (P + 1) * (Q + 1) <= 16) ? kPatchRegs : kPatchGlobal>( \
+                                : (PREG && (P + 1) * (Q + 1) <= 16) ? kPatchRegs : kPatchGlobal, \
+                           false, true>(                                                        \
                 d, d.ku, d.kv, Px, Py, Pz, cu, cv, tol_dist, tol_step, tol_quad, clamp,         \
                 domain_of(d), sc, pr, wave_lds);                                                \
         break;
@@ -111,9 +112,9 @@ __device__ __forceinline__ int step_group(int cls, const GnSurfDev &d, T Px, T P
             break;
     }
     if constexpr (GRP == 2 && !WAVE)
-        return gn_step<GN_MAXK, GN_MAXK, false, T, NEWTON>(d, d.ku, d.kv, Px, Py, Pz, cu, cv,
-                                                           tol_dist, tol_step, tol_quad, clamp,
-                                                           domain_of(d), sc);
+        return gn_step<GN_MAXK, GN_MAXK, false, T, NEWTON, kPatchGlobal, false, true>(
+            d, d.ku, d.kv, Px, Py, Pz, cu, cv, tol_dist, tol_step, tol_quad, clamp, domain_of(d),
+            sc);
     return GN_PT_NONFINITE;  // not reached: the tiles of a group launch are of its classes
 }
 

@@ -41,6 +41,34 @@ __device__ __forceinline__ int find_span_guess(const double *__restrict__ k, int
     return lo;
 }
 
+// find_span for the iteration kernels, where a span search is a chain of dependent loads from
+// global memory (batched surfaces) or LDS: the uniform-knot guess (an FP32 estimate, clamped
+// to [p, cp-1]) is tested first, then the one neighbour on u's side of it, and only a u outside
+// those spans is bisected.  Any bracketing span is the unique one, so the result is find_span's
+// in every case.
+__device__ __forceinline__ int find_span_near(const double *__restrict__ k, int p, int cp, double u) {
+    const double kp = k[p], kc = k[cp];
+    if (u >= kc) return cp - 1;
+    if (u <= kp) return p;
+    if (u != u) return (p + cp) >> 1;
+    float f = (float)(u - kp) * __builtin_amdgcn_rcpf((float)(kc - kp)) * (float)(cp - p);
+    f = fminf(fmaxf(f, 0.f), (float)(cp - p - 1));  // also maps a NaN estimate to 0
+    const int m = p + (int)f;
+    const double a1 = k[m], a2 = k[m + 1];
+    if (a1 <= u && u < a2) return m;
+    if (u < a1) {
+        if (m > p && k[m - 1] <= u) return m - 1;
+    } else if (m + 2 <= cp && u < k[m + 2]) {
+        return m + 1;
+    }
+    int lo = p, hi = cp;
+    while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        if (u < k[mid]) hi = mid; else lo = mid;
+    }
+    return lo;
+}
+
 template <typename T>
 struct Hom {  // homogeneous 4-vector
     T x, y, z, w;
@@ -346,7 +374,10 @@ struct SpanCache {
 // staged by the whole wave with the next row prefetched (wave-uniform call required).
 enum { kPatchGlobal = 0, kPatchRegs = 1, kPatchWavePipe = 2 };
 
-template <int K1, int K2, int ORDER, bool EXACT, typename T, int PSRC = kPatchGlobal>
+// GKNOTS: the knots are in global memory (batched surfaces), where the span search's dependent
+// loads are long: find_span_near.  With the knots in LDS the bisection costs fewer registers.
+template <int K1, int K2, int ORDER, bool EXACT, typename T, int PSRC = kPatchGlobal,
+          bool GKNOTS = false>
 __device__ __forceinline__ void eval_point_cached(const GnSurfDev &d, const double *__restrict__ ku,
                                                   const double *__restrict__ kv, double u, double v,
                                                   SpanCache &sc, Cart<T> &o,
@@ -354,7 +385,7 @@ __device__ __forceinline__ void eval_point_cached(const GnSurfDev &d, const doub
     constexpr bool PREG = PSRC == kPatchRegs;
     bool moved = false;
     if (!(u >= sc.ulo && u < sc.uhi)) {
-        const int m = find_span(ku, d.p, d.nu, u);
+        const int m = GKNOTS ? find_span_near(ku, d.p, d.nu, u) : find_span(ku, d.p, d.nu, u);
         sc.ulo = ku[m];
         sc.uhi = ku[m + 1];
         sc.im = m - d.p;
@@ -362,7 +393,7 @@ __device__ __forceinline__ void eval_point_cached(const GnSurfDev &d, const doub
         moved = true;
     }
     if (!(v >= sc.vlo && v < sc.vhi)) {
-        const int n = find_span(kv, d.q, d.nv, v);
+        const int n = GKNOTS ? find_span_near(kv, d.q, d.nv, v) : find_span(kv, d.q, d.nv, v);
         sc.vlo = kv[n];
         sc.vhi = kv[n + 1];
         sc.in = n - d.q;
