@@ -432,6 +432,9 @@ def bench_serve(G, synth, torch, duration=1.0, loads=(0.5, 0.8), budget=32 << 30
                     "max_ms": float(lat.max()), "submit_lag_p99_ms": float(np.percentile(lag, 99)),
                     "steps": int(st1.steps - st0.steps), "max_step_requests": int(st1.max_step_requests),
                     "step_model": step_model(*steps),
+                    "host_launch_ms_mean": (st1.launch_ns_total - st0.launch_ns_total) / 1e6 /
+                                           max(1, st1.steps - st0.steps),
+                    "host_launch_ms_max_so_far": st1.launch_ns_max / 1e6,
                 }
             out["arena_peak_bytes"] = int(srv.stats().arena_peak)
             srv.close()

@@ -68,8 +68,10 @@ static void free_tree(gn_surface *s) {
     (void)hipFree(s->d_sref);
     (void)hipFree(s->d_sleaf32);
     (void)hipFree(s->d_sjac32);
+    (void)hipFree(s->d_scone32);
     s->d_sleaf32 = nullptr;
     s->d_sjac32 = nullptr;
+    s->d_scone32 = nullptr;
     s->d_tree = nullptr;
     s->d_valid = s->d_divided = nullptr;
     s->d_sbox = s->d_scentre = s->d_sseed = nullptr;
@@ -330,7 +332,8 @@ int gn_build_tree(gn_surface *s, int levels, int flags, gn_stream stream) {
         (rc = dalloc(&s->d_sseed, (size_t)leaves * 2, acc)) ||
         (rc = dalloc(&s->d_sref, (size_t)leaves, acc)) ||
         (rc = dalloc(&s->d_sleaf32, (size_t)leaves, acc)) ||
-        (rc = dalloc(&s->d_sjac32, (size_t)leaves * 2, acc))) {
+        (rc = dalloc(&s->d_sjac32, (size_t)leaves * 2, acc)) ||
+        (rc = dalloc(&s->d_scone32, (size_t)leaves, acc))) {
         free_tree(s);
         return rc;
     }
@@ -340,7 +343,8 @@ int gn_build_tree(gn_surface *s, int levels, int flags, gn_stream stream) {
         if (rc) { free_tree(s); return rc; }
     }
     rc = gn::launch_search_build(d, levels, s->d_tree, s->d_sbox, s->d_sbox32, s->d_scentre,
-                                 s->d_sseed, s->d_sref, s->d_sleaf32, s->d_sjac32, st);
+                                 s->d_sseed, s->d_sref, s->d_sleaf32, s->d_sjac32, s->d_scone32,
+                                 st);
     if (rc) { free_tree(s); return rc; }
     s->tree_levels = levels;
     s->device_bytes += acc;
@@ -358,6 +362,30 @@ int gn_get_tree(const gn_surface *s, double *nodes, uint8_t *valid, uint8_t *div
     if (nodes) GN_HIP(hipMemcpy(nodes, s->d_tree, sizeof(GnTreeNode) * c, hipMemcpyDeviceToHost));
     if (valid) GN_HIP(hipMemcpy(valid, s->d_valid, c, hipMemcpyDeviceToHost));
     if (divided) GN_HIP(hipMemcpy(divided, s->d_divided, c, hipMemcpyDeviceToHost));
+    return GN_OK;
+}
+
+int gn_get_normal_cones(const gn_surface *s, float *cones, size_t count) {
+    if (!s) return fail(GN_EINVAL, "null surface");
+    if (!s->tree_levels) return fail(GN_ESTATE, "tree not built");
+    if (!cones) return fail(GN_EINVAL, "null buffer");
+    long g = 1;
+    for (int l = 1; l < s->tree_levels; l++) g *= 8;
+    const size_t leaves = (size_t)(g * g);
+    if (count < leaves) return fail(GN_EINVAL, "count smaller than the leaf count");
+    std::vector<float4> m(leaves);
+    GN_HIP(hipDeviceSynchronize());
+    GN_HIP(hipMemcpy(m.data(), s->d_scone32, sizeof(float4) * leaves, hipMemcpyDeviceToHost));
+    // Morton order (x bits odd) -> leaf index bx * g + by
+    for (long bx = 0; bx < g; bx++)
+        for (long by = 0; by < g; by++) {
+            uint32_t c = 0;
+            for (int b = 0; b < 16; b++)
+                c |= (((uint32_t)bx >> b & 1u) << (2 * b + 1)) | (((uint32_t)by >> b & 1u) << (2 * b));
+            const float4 &f = m[c];
+            float *o = cones + (size_t)(bx * g + by) * 4;
+            o[0] = f.x; o[1] = f.y; o[2] = f.z; o[3] = f.w;
+        }
     return GN_OK;
 }
 

@@ -252,6 +252,7 @@ int gn_batch_create(gn_surface *const *surfaces, int count, gn_stream stream, gn
         ob.sd.ref_index = (const int32_t *)(base + parts[i][9].off);
         ob.sd.leaf32 = (const float4 *)(base + parts[i][10].off);
         ob.sd.jac32 = (const float4 *)(base + parts[i][11].off);
+        ob.sd.cone32 = nullptr;  // normal cones serve the global projection only
         ob.cls = degree_class(s->p, s->q);
         ob.pad_ = 0;
         b->cls[i] = ob.cls;

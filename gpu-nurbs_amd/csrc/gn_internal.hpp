@@ -45,6 +45,8 @@ struct GnSearchDev {
                               // (bits of w), one 16-byte load for the greedy 3 x 3 search
     const float4 *jac32;      // per leaf (Morton order) S_u, S_v at the leaf's seed (u, v) in
                               // FP32, two 16-byte records (x, y, z, 0)
+    const float4 *cone32;     // per leaf (Morton order) normal cone: unit axis, half-angle
+                              // (radians, rounded up; 4 when unbounded); null: no cone pruning
     double u0, u1, v0, v1;    // the knot domain the leaves tile
 };
 
@@ -120,6 +122,7 @@ struct gn_surface {
     int32_t *d_sref = nullptr;
     float4 *d_sleaf32 = nullptr;
     float4 *d_sjac32 = nullptr;
+    float4 *d_scone32 = nullptr;
     size_t device_bytes = 0;
 
     GnSurfDev dev() const {
@@ -137,6 +140,7 @@ struct gn_surface {
         s.ref_index = d_sref;
         s.leaf32 = d_sleaf32;
         s.jac32 = d_sjac32;
+        s.cone32 = d_scone32;
         s.u0 = ku.empty() ? 0.0 : ku[p];
         s.u1 = ku.empty() ? 1.0 : ku[nu];
         s.v0 = kv.empty() ? 0.0 : kv[q];
@@ -184,7 +188,7 @@ int launch_tree_level(const GnSurfDev &d, int level, int flags, GnTreeNode *tree
                       uint8_t *valid, uint8_t *divided, hipStream_t st);
 int launch_search_build(const GnSurfDev &d, int levels, const GnTreeNode *tree, double *sbox,
                         float *sbox32, double *scentre, double *sseed, int32_t *sref,
-                        float4 *sleaf32, float4 *sjac32, hipStream_t st);
+                        float4 *sleaf32, float4 *sjac32, float4 *scone32, hipStream_t st);
 // mode 0: exact nearest leaf centre; mode 1: greedy descent + 3x3 leaf neighbourhood;
 // mode 2: mode 1, then one Gauss-Newton step from the leaf's seed with its stored Jacobian
 int launch_seed(const GnSearchDev &sd, int mode, const double *x, const double *y,

@@ -1,11 +1,14 @@
 // project.hip -- global projection (GN_SEED_GLOBAL): the nearest point of the surface to each
 // query, not just the local minimum that Newton reaches from one seed.  The reference's version
-// (normal-cone culling, src/kernel/kernel.cu:471-513, 623-704) is unfinished; this one is a branch
-// and bound over the seed quadtree whose leaf boxes contain their pieces of surface:
+// (normal-cone culling, src/kernel/kernel.cu:471-513, 623-704) is unfinished: its test tries the
+// box corners only, for one sign of the normal.  This one is a branch and bound over the seed
+// quadtree whose leaf boxes contain their pieces of surface:
 //   1. bound: exact nearest leaf centre, then projected Newton over the whole domain;
 //   2. every leaf whose FP32 box (a conservative lower bound) is not farther than the best foot
-//      point so far gets a projected Newton solve clamped to its uv cell, from the cell centre;
-//      a nearer result tightens the bound, which prunes the rest of the depth-first walk;
+//      point so far, and whose normal cone allows a foot point inside it (cone_may_hold; leaves
+//      on the domain boundary, where the nearest point need not be a foot point, always pass),
+//      gets a projected Newton solve clamped to its uv cell, from the cell centre; a nearer
+//      result tightens the bound, which prunes the rest of the depth-first walk;
 //   3. the nearest foot point is polished by projected Newton over the whole domain (kept unless it
 //      is farther by more than rounding, 1e-9 relative in squared distance).
 // The minimum is found whenever the solve inside the leaf that holds it reaches it: leaves of the
@@ -20,6 +23,60 @@
 namespace gn {
 
 constexpr int kLeafIter = 8;  // projected Newton steps inside one leaf cell
+
+// Whether a leaf can hold an interior foot point of P: a point X of its box b with P - X parallel
+// to a normal n of its cone (unit axis a, half-angle theta < 90 degrees), i.e. whether the box meets
+// the double cone {X : ((X - P) . a)^2 >= cos^2(theta) |X - P|^2} with apex P.  The angle between
+// X - P and the line of a has, over a box that the line misses, its minimum on an edge (inside the
+// box or a face it could only have one where the line meets it), so the test is: P in the box, or
+// the line P + t a meets the box, or some edge has a point with q(s) >= 0, q being that quadratic
+// along the edge.  Exact up to rounding; FP64 on the FP32 box's bounds, with the half-angle widened
+// by another 2e-6 rad.
+__device__ __forceinline__ bool cone_may_hold(const float4 cn, const float *__restrict__ b,
+                                              const double P[3]) {
+    if (!(cn.w < 1.5f)) return true;  // no bounded cone
+    const double a[3] = {cn.x, cn.y, cn.z};
+    const double lo[3] = {b[0], b[1], b[2]}, hi[3] = {b[3], b[4], b[5]};
+    if (!(P[0] == P[0] && P[1] == P[1] && P[2] == P[2])) return true;
+    // the line P + t a against the slabs
+    double t0 = -INFINITY, t1 = INFINITY;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        if (a[k] != 0.0) {
+            const double r = 1.0 / a[k];
+            const double ta = (lo[k] - P[k]) * r, tb = (hi[k] - P[k]) * r;
+            t0 = fmax(t0, fmin(ta, tb));
+            t1 = fmin(t1, fmax(ta, tb));
+        } else if (P[k] < lo[k] || P[k] > hi[k]) {
+            t0 = INFINITY;  // parallel to this slab and outside it
+        }
+    }
+    if (t0 <= t1) return true;  // also P inside the box (t = 0)
+    const double c = cos((double)cn.w + 2e-6), c2 = c * c;
+    // the 12 edges: along axis k, the other two coordinates at their bounds
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const int k1 = k == 0 ? 1 : 0, k2 = k == 2 ? 1 : 2;
+        const double d[3] = {k == 0 ? hi[0] - lo[0] : 0.0, k == 1 ? hi[1] - lo[1] : 0.0,
+                             k == 2 ? hi[2] - lo[2] : 0.0};
+        const double be = d[k] * a[k], ee = d[k] * d[k];
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            double w[3];
+            w[k] = lo[k] - P[k];
+            w[k1] = ((e & 1) ? hi[k1] : lo[k1]) - P[k1];
+            w[k2] = ((e & 2) ? hi[k2] : lo[k2]) - P[k2];
+            const double al = w[0] * a[0] + w[1] * a[1] + w[2] * a[2];
+            const double ga = w[0] * w[0] + w[1] * w[1] + w[2] * w[2];
+            const double de = w[k] * d[k];
+            // q(s) = A s^2 + 2 B s + C on s in [0, 1]
+            const double A = be * be - c2 * ee, B = al * be - c2 * de, C = al * al - c2 * ga;
+            if (C >= 0.0 || A + 2.0 * B + C >= 0.0) return true;
+            if (A < 0.0 && B > 0.0 && B < -A && C - B * B / A >= 0.0) return true;  // vertex -B/A
+        }
+    }
+    return false;
+}
 
 // Projected Newton from (cu, cv) inside dom, with backtracking: a step after which the squared
 // distance grew by more than 1e-6 relative is halved, up to kHalvings times in a row, before a new
@@ -91,6 +148,7 @@ __device__ __forceinline__ void project_point(const GnSurfDev &d, const GnSearch
     if (D > 0) {
         const float p[3] = {(float)P[0], (float)P[1], (float)P[2]};
         // stackless depth-first walk (Morton-code successor), pruned by the current best
+        const uint32_t side = 1u << D;
         int l = 1;
         uint32_t c = 0;
         while (true) {
@@ -99,7 +157,12 @@ __device__ __forceinline__ void project_point(const GnSurfDev &d, const GnSearch
             if (lb <= best) {
                 if (l < D) {
                     descend = true;
-                } else if (c != c0) {
+                } else if (c != c0 &&
+                           (!sd.cone32 || compact2(c >> 1) - 1u >= side - 2u ||
+                            compact2(c) - 1u >= side - 2u ||
+                            cone_may_hold(sd.cone32[c], sd.box32 + (qoff(l) + c) * 6, P))) {
+                    // (the two index tests pass the leaves on the domain boundary, ix or iy
+                    // 0 or side - 1, by unsigned wrap-around)
                     const uint32_t ix = compact2(c >> 1), iy = compact2(c);
                     const Domain cr{full.u0 + wu * (ix * cell), full.u0 + wu * ((ix + 1) * cell),
                                     full.v0 + wv * (iy * cell), full.v0 + wv * ((iy + 1) * cell)};
@@ -151,8 +214,17 @@ __device__ __forceinline__ void project_point(const GnSurfDev &d, const GnSearch
     }
 }
 
+// Two waves per SIMD are forced where the spills that costs stay small: patches of up to 20
+// entries (bicubic: 12 bytes per lane; one wave without the bound, 13.3 against 8.2 ms on 1M
+// projection queries; degree (3,4): 160 bytes, 14.3 against 9.1 ms) and the generic kernel (80
+// bytes, 12.9 against 7.2 ms).  Larger exact patches run one wave with AGPRs as spill room:
+// forced to two, degree (5,5) spilled 744 bytes per lane and took 24.5 against 18.3 ms.
 template <int K1, int K2, bool EXACT>
-__global__ __launch_bounds__(256) void k_project_global(GnSurfDev d, GnSearchDev sd, InvArgs a) {
+constexpr int project_waves() { return (!EXACT || K1 * K2 <= 20) ? 2 : 1; }
+
+template <int K1, int K2, bool EXACT>
+__global__ __launch_bounds__(256, (project_waves<K1, K2, EXACT>())) void k_project_global(GnSurfDev d,
+                                                                              GnSearchDev sd, InvArgs a) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < a.n;
          i += (size_t)gridDim.x * blockDim.x)
         project_point<K1, K2, EXACT>(d, sd, a, i, true);

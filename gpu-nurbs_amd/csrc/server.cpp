@@ -281,7 +281,14 @@ void gn_server::run_slot(Slot &sl) {
         if (!sl.go) return;  // quit with nothing assigned
         sl.go = false;
         lk.unlock();
+        const int64_t t_launch = gn_now_ns_impl();
         bool ok = launch_step(sl) == GN_OK;
+        {
+            const int64_t dt = gn_now_ns_impl() - t_launch;
+            std::lock_guard<std::mutex> g(mu);
+            stats.launch_ns_total += dt;
+            stats.launch_ns_max = std::max(stats.launch_ns_max, dt);
+        }
         if (!ok) {
             // the launch joins every side stream it forked into sl.st even when it fails; the
             // side streams are waited for as well, so no kernel of the failed step still writes

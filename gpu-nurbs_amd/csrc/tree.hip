@@ -223,6 +223,113 @@ __global__ void k_search_jac(GnSurfDev d, long leaves, const double *__restrict_
     sjac32[c * 2 + 1] = make_float4((float)o.Sv[0], (float)o.Sv[1], (float)o.Sv[2], 0.f);
 }
 
+// Normal cone of every leaf (NormalBoundingBox + NormalBoxToNormalTreeNode,
+// src/kernel/kernel.cu:623-704, 1605-1636; normal derivatives src/NURBS/Evaluate.cu:1125-1135):
+// one wave per leaf samples N = S_u x S_v on the tree's 9 x 9 grid of the leaf cell, takes the
+// box of those normals enlarged by K = (M1 Wu^2 + 2 M2 Wu Wv + M3 Wv^2) sqrt(3) / (8 (8g)^2) with
+// M* the sample maxima of |N_uu|inf, |N_uv|inf, |N_vv|inf (the position boxes' K applied to N),
+// and the cone around the box centre m through the farthest box corner.  A box that holds the
+// origin, or a corner 90 degrees or more from m, gives no bounded cone (half-angle 4).  FP64;
+// stored in FP32 with the half-angle rounded up past 1e-6 rad, more than the axis' rounding.
+__device__ __forceinline__ void cross3(const double *a, const double *b, double *o) {
+    o[0] = a[1] * b[2] - a[2] * b[1];
+    o[1] = a[2] * b[0] - a[0] * b[2];
+    o[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+template <int K1, int K2, bool EXACT>
+__global__ __launch_bounds__(64) void k_search_cone(GnSurfDev d, int levels,
+                                                    float4 *__restrict__ cone32) {
+    long g = 1;
+    for (int l = 1; l < levels; l++) g *= 8;
+    const long idx = blockIdx.x;  // bx * g + by
+    const long bx = idx / g, by = idx % g;
+    const double base_unit = 1.0 / g, unit = base_unit / 8;
+    const double u0 = d.ku[d.p], wu = d.ku[d.nu] - u0, v0 = d.kv[d.q], wv = d.kv[d.nv] - v0;
+    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    double M1 = 0, M2 = 0, M3 = 0;
+    for (int sidx = threadIdx.x; sidx < 81; sidx += 64) {
+        const int tx = sidx / 9, ty = sidx % 9;
+        const double su = u0 + wu * (base_unit * bx + tx * unit),
+                     sv = v0 + wv * (base_unit * by + ty * unit);
+        Cart<double> o;
+        eval_point<K1, K2, 3, EXACT, double>(d, d.ku, d.kv, su, sv, o);
+        double N[3], a[3], b[3], c[3], D[3];
+        cross3(o.Su, o.Sv, N);
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            lo[k] = fmin(lo[k], N[k]);
+            hi[k] = fmax(hi[k], N[k]);
+        }
+        cross3(o.Suuu, o.Sv, a);  // N_uu = S_uuu x S_v + 2 S_uu x S_uv + S_u x S_uuv
+        cross3(o.Suu, o.Suv, b);
+        cross3(o.Su, o.Suuv, c);
+        for (int k = 0; k < 3; k++) D[k] = a[k] + 2.0 * b[k] + c[k];
+        M1 = fmax(M1, fmax(fabs(D[0]), fmax(fabs(D[1]), fabs(D[2]))));
+        cross3(o.Suuv, o.Sv, a);  // N_uv = S_uuv x S_v + S_uu x S_vv + S_u x S_uvv
+        cross3(o.Suu, o.Svv, b);
+        cross3(o.Su, o.Suvv, c);
+        for (int k = 0; k < 3; k++) D[k] = a[k] + b[k] + c[k];
+        M2 = fmax(M2, fmax(fabs(D[0]), fmax(fabs(D[1]), fabs(D[2]))));
+        cross3(o.Suvv, o.Sv, a);  // N_vv = S_uvv x S_v + 2 S_uv x S_vv + S_u x S_vvv
+        cross3(o.Suv, o.Svv, b);
+        cross3(o.Su, o.Svvv, c);
+        for (int k = 0; k < 3; k++) D[k] = a[k] + 2.0 * b[k] + c[k];
+        M3 = fmax(M3, fmax(fabs(D[0]), fmax(fabs(D[1]), fabs(D[2]))));
+    }
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        lo[k] = wave_min(lo[k]);
+        hi[k] = wave_max(hi[k]);
+    }
+    M1 = wave_max(M1);
+    M2 = wave_max(M2);
+    M3 = wave_max(M3);
+    if (threadIdx.x != 0) return;
+    const double K = (M1 * wu * wu + 2 * M2 * wu * wv + M3 * wv * wv) * sqrt(3.0) /
+                     (8.0 * 8.0 * (double)g * 8.0 * (double)g);
+    bool zero = true;
+    double m[3];
+    for (int k = 0; k < 3; k++) {
+        lo[k] -= K;
+        hi[k] += K;
+        if (!(lo[k] <= 0.0 && hi[k] >= 0.0)) zero = false;
+        m[k] = 0.5 * (lo[k] + hi[k]);
+    }
+    double th = 0.0;
+    for (int cn = 0; cn < 8 && !zero; cn++) {
+        const double x[3] = {(cn & 4) ? hi[0] : lo[0], (cn & 2) ? hi[1] : lo[1],
+                             (cn & 1) ? hi[2] : lo[2]};
+        double cr[3];
+        cross3(m, x, cr);
+        th = fmax(th, atan2(sqrt(cr[0] * cr[0] + cr[1] * cr[1] + cr[2] * cr[2]),
+                            m[0] * x[0] + m[1] * x[1] + m[2] * x[2]));
+    }
+    const double mm = sqrt(m[0] * m[0] + m[1] * m[1] + m[2] * m[2]);
+    float4 out = make_float4(0.f, 0.f, 0.f, 4.f);
+    if (!zero && th < 0.5 * M_PI && mm > 0.0) {
+        float t32 = (float)(th + 1e-6);
+        if ((double)t32 < th + 1e-6) t32 = nextafterf(t32, INFINITY);
+        out = make_float4((float)(m[0] / mm), (float)(m[1] / mm), (float)(m[2] / mm), t32);
+    }
+    cone32[morton2((uint32_t)bx, (uint32_t)by)] = out;
+}
+
+struct ConeLaunch {
+    const GnSurfDev &d;
+    int levels;
+    float4 *cone32;
+    hipStream_t st;
+    template <int K1, int K2, bool EXACT>
+    int operator()() {
+        long g = 1;
+        for (int l = 1; l < levels; l++) g *= 8;
+        k_search_cone<K1, K2, EXACT><<<(unsigned)(g * g), 64, 0, st>>>(d, levels, cone32);
+        GN_HIP(hipGetLastError());
+        return GN_OK;
+    }
+};
+
 struct JacLaunch {
     const GnSurfDev &d;
     long leaves;
@@ -240,13 +347,14 @@ struct JacLaunch {
 
 int launch_search_build(const GnSurfDev &d, int levels, const GnTreeNode *tree, double *sbox,
                         float *sbox32, double *scentre, double *sseed, int32_t *sref,
-                        float4 *sleaf32, float4 *sjac32, hipStream_t st) {
+                        float4 *sleaf32, float4 *sjac32, float4 *scone32, hipStream_t st) {
     const int D = 3 * (levels - 1);
     const long leaves = 1L << (2 * D);
     k_search_leaves<<<grid_for(leaves, 256, 1u << 30), 256, 0, st>>>(levels, tree, sbox, scentre,
                                                                        sseed, sref, sleaf32);
     GN_HIP(hipGetLastError());
-    const int rc = dispatch_degrees(d.p, d.q, JacLaunch{d, leaves, sseed, sjac32, st});
+    int rc = dispatch_degrees(d.p, d.q, JacLaunch{d, leaves, sseed, sjac32, st});
+    if (!rc) rc = dispatch_degrees(d.p, d.q, ConeLaunch{d, levels, scone32, st});
     if (rc) return rc;
     for (int l = D - 1; l >= 0; l--)
         k_search_level<<<grid_for(1L << (2 * l), 256, 1u << 30), 256, 0, st>>>(l, sbox);

@@ -81,6 +81,7 @@ _SIGS = [
     ("gn_evaluate_grid_f32", _c_int, [_c_void, _c_void, _c_int, _c_void, _c_int, _c_int, _c_void, _c_void]),
     ("gn_build_tree", _c_int, [_c_void, _c_int, _c_int, _c_void]),
     ("gn_get_tree", _c_int, [_c_void, _c_void, _c_void, _c_void, _c_size]),
+    ("gn_get_normal_cones", _c_int, [_c_void, _c_void, _c_size]),
     ("gn_seed", _c_int, [_c_void, _c_int, _c_void, _c_void, _c_void, _c_size, _c_void, _c_void, _c_void, _c_void]),
     ("gn_invert", _c_int, [_c_void, _c_void, _c_void, _c_void, _c_size, ctypes.POINTER(InvertOpts), _c_void, _c_void, _c_void, _c_void]),
     ("gn_invert_noalloc", _c_int, [_c_void, _c_void, _c_void, _c_void, _c_size, ctypes.POINTER(InvertOpts), _c_void, _c_void, _c_void, _c_void]),
@@ -321,6 +322,14 @@ class Surface:
         _check(lib().gn_get_tree(self._h, nodes.ctypes.data, valid.ctypes.data, divided.ctypes.data, cnt))
         return nodes, valid, divided
 
+    def GetNormalCones(self):
+        """(leaves, 4) float32: unit axis and half-angle of each finest-level leaf (index bx*g + by);
+        half-angle 4 marks a leaf with no bounded cone."""
+        g = 8 ** (self.tree_levels - 1)
+        out = np.zeros((g * g, 4), dtype=np.float32)
+        _check(lib().gn_get_normal_cones(self._h, out.ctypes.data, g * g))
+        return out
+
     def Seed(self, x, y, z, exact=True, stream=None):
         """Seed search: exact=True the exact nearest leaf, False the greedy leaf, "step" the greedy
         leaf plus one Gauss-Newton step (GN_SEED_GREEDY_STEP).  Returns (leaf, u, v)."""
@@ -485,7 +494,7 @@ class RequestInfo(ctypes.Structure):
 class ServerStats(ctypes.Structure):
     _fields_ = [("submitted", ctypes.c_int64), ("completed", ctypes.c_int64), ("steps", ctypes.c_int64),
                 ("points", ctypes.c_int64), ("max_step_requests", ctypes.c_int64), ("arena_bytes", _c_size),
-                ("arena_peak", _c_size)]
+                ("arena_peak", _c_size), ("launch_ns_total", ctypes.c_int64), ("launch_ns_max", ctypes.c_int64)]
 
 
 class _DeviceView:

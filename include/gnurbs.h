@@ -62,10 +62,11 @@ typedef enum {
     GN_SEED_GIVEN = 2,        /* caller supplies seeds in u_out / v_out                    */
     GN_SEED_GREEDY_LEAF = 3,  /* greedy box-distance descent, then the nearest leaf centre */
     GN_SEED_GLOBAL = 4,       /* global projection (FP64): Newton from every leaf whose box
-                                 could hold a nearer point (branch and bound over the seed tree),
-                                 each clamped to its leaf cell, then polished; the nearest foot
-                                 point wins.  iters_fp32 of gn_invert_iters receives the number
-                                 of leaves solved.  The reference's normal-cone projection
+                                 could hold a nearer point and whose normal cone could hold the
+                                 foot point's normal (branch and bound over the seed tree), each
+                                 clamped to its leaf cell, then polished; the nearest foot point
+                                 wins.  iters_fp32 of gn_invert_iters receives the number of
+                                 leaves solved.  The reference's normal-cone projection
                                  (src/kernel/kernel.cu:471-513, 623-704) is unfinished. */
     GN_SEED_GREEDY_STEP = 5   /* GN_SEED_GREEDY_LEAF, then one Gauss-Newton step from the leaf's
                                  seed with the Jacobian stored for that leaf (S_u, S_v at its
@@ -182,6 +183,13 @@ int gn_build_tree(gn_surface *s, int levels, int flags, gn_stream stream);
  * valid[count], divided[count]; count = (64^levels - 1)/63.  Synchronous. */
 int gn_get_tree(const gn_surface *s, double *nodes, uint8_t *valid, uint8_t *divided,
                 size_t count);
+/* Host copy of the leaves' normal cones (NormalBoundingBox + NormalBoxToNormalTreeNode,
+ * src/kernel/kernel.cu:623-704, 1605-1636): per leaf of the finest level, in leaf order bx*g+by
+ * (g = 8^(levels-1)), 4 floats: unit axis and half-angle in radians; a half-angle of 4 marks a
+ * leaf with no bounded cone.  The cone holds the normal S_u x S_v box of the leaf's 9 x 9
+ * samples enlarged by the tree's K bound applied to the normal's second derivatives.
+ * count >= g*g.  Synchronous. */
+int gn_get_normal_cones(const gn_surface *s, float *cones, size_t count);
 /* Seed search alone: leaf index (reference numbering) and seed (u,v) per point; exact == 0
  * selects GN_SEED_GREEDY_LEAF, exact == 2 GN_SEED_GREEDY_STEP, any other value
  * GN_SEED_NEAREST_LEAF. */
@@ -314,6 +322,8 @@ typedef struct {
     int64_t submitted, completed, steps, points; /* totals since creation                    */
     int64_t max_step_requests;                   /* most requests in one step                */
     size_t arena_bytes, arena_peak;              /* result arena capacity and peak use       */
+    int64_t launch_ns_total, launch_ns_max;      /* host time spent launching steps (building */
+                                                 /* the step, its copy and its kernel launches) */
 } gn_server_stats;
 
 /* Objects 0 .. nsurf-1 are the batch's surfaces (batch may be NULL), nsurf .. nsurf+ncurves-1

@@ -29,6 +29,10 @@
  *   or_gauss_newton     round trips on on-surface points, orthogonality of the residual
  *   or_build_tree       containment of dense samples, index formula, NeedDivide truth table
  *   or_seed_*           brute force
+ *   or_leaf_cones       every normal of a dense sample grid of each leaf lies in its cone; exact
+ *                       half-angles of circular-cylinder leaves (the normal turns with the arc);
+ *                       the normal derivatives against finite differences of S_u x S_v
+ *                       (tests/test_oracle_cones.py)
  *   or_project_global   never farther than any densely sampled surface point; never farther than
  *                       the local method; GPU branch and bound equals this exhaustive form
  *   or_curve_*          (no reference counterpart) exact circle arcs, scipy BSpline with textbook
@@ -677,6 +681,124 @@ void or_build_tree(int p, int q, int nu, int nv, const double *ku, const double 
                 for (int c = 0; c < 3; c++) nd[10 + c] = o.S[c];
             }
     }
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* normal cones of the tree leaves -- NormalBoundingBox / NormalBoxToNormalTreeNode /             */
+/* GetNd{UU,UV,VV}s, src/kernel/kernel.cu:623-704, 1486-1556, 1605-1636, with the normal          */
+/* derivatives of src/NURBS/Evaluate.cu:1125-1135                                                 */
+/* ------------------------------------------------------------------------------------------ */
+static void cross3(const double *a, const double *b, double *o)
+{
+    o[0] = a[1] * b[2] - a[2] * b[1];
+    o[1] = a[2] * b[0] - a[0] * b[2];
+    o[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+static double amax3(const double *a) { return fmax(fabs(a[0]), fmax(fabs(a[1]), fabs(a[2]))); }
+
+/* angle between a and b, by atan2 of |a x b| and a . b */
+static double angle3(const double *a, const double *b)
+{
+    double c[3];
+    cross3(a, b, c);
+    return atan2(sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]),
+                 a[0] * b[0] + a[1] * b[1] + a[2] * b[2]);
+}
+
+/* N = S_u x S_v and its second derivatives (src/NURBS/Evaluate.cu:1125-1135; there N_uv also
+ * carries S_uv x S_uv, which is zero) */
+static void normal_ders(const or_pt *o, double *N, double *Nuu, double *Nuv, double *Nvv)
+{
+    double a[3], b[3], c[3];
+    cross3(o->Su, o->Sv, N);
+    cross3(o->Suuu, o->Sv, a);
+    cross3(o->Suu, o->Suv, b);
+    cross3(o->Su, o->Suuv, c);
+    for (int k = 0; k < 3; k++) Nuu[k] = a[k] + 2.0 * b[k] + c[k];
+    cross3(o->Suuv, o->Sv, a);
+    cross3(o->Suu, o->Svv, b);
+    cross3(o->Su, o->Suvv, c);
+    for (int k = 0; k < 3; k++) Nuv[k] = a[k] + b[k] + c[k];
+    cross3(o->Suvv, o->Sv, a);
+    cross3(o->Suv, o->Svv, b);
+    cross3(o->Su, o->Svvv, c);
+    for (int k = 0; k < 3; k++) Nvv[k] = a[k] + 2.0 * b[k] + c[k];
+}
+
+/* out[12 i ..]: N, N_uu, N_uv, N_vv at (u[i], v[i]) */
+void or_normal_ders(int p, int q, int nu, int nv, const double *ku, const double *kv,
+                    const double *MatrixN, const double *u, const double *v, long n, double *out)
+{
+    or_surf sf = {p, q, nu, nv, ku, kv, MatrixN};
+    for (long i = 0; i < n; i++) {
+        or_pt o;
+        eval_one(&sf, u[i], v[i], 3, &o);
+        normal_ders(&o, out + 12 * i, out + 12 * i + 3, out + 12 * i + 6, out + 12 * i + 9);
+    }
+}
+
+/* For every leaf (level levels-1, g = 8^(levels-1) per side, index bx*g + by): the box of the
+ * normal N = S_u x S_v over the leaf's 9 x 9 samples (the tree's sample grid), enlarged by
+ * K = (M1 Wu^2 + 2 M2 Wu Wv + M3 Wv^2) sqrt(3) / (8 (8g)^2) with M* the sample maxima of
+ * |N_uu|inf, |N_uv|inf, |N_vv|inf (the position boxes' K, applied to N), then the cone around the
+ * box centre m that holds the box: half-angle = the largest angle between m and a box corner.
+ * out[4*leaf]: unit axis (3) and half-angle; half-angle 4 (> pi/2) when the box holds the origin
+ * or a corner is 90 degrees or more from m (no bounded cone).  The reference stops at the box
+ * and cone; its use in Project (kernel.cu:471-513) tests box corners only and is unfinished. */
+void or_leaf_cones(int p, int q, int nu, int nv, const double *ku, const double *kv,
+                   const double *MatrixN, int levels, double *out)
+{
+    or_surf sf = {p, q, nu, nv, ku, kv, MatrixN};
+    double u0 = ku[p], wu = ku[nu] - u0, v0 = kv[q], wv = kv[nv] - v0;
+    long g = 1;
+    for (int L = 1; L < levels; L++) g *= OR_GAP;
+    for (long bx = 0; bx < g; bx++)
+        for (long by = 0; by < g; by++) {
+            double base_unit = 1.0 / g, unit = base_unit / OR_GAP;
+            double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+            double M1 = 0, M2 = 0, M3 = 0;
+            for (int tx = 0; tx < OR_PPE; tx++)
+                for (int ty = 0; ty < OR_PPE; ty++) {
+                    double su = u0 + wu * (base_unit * bx + tx * unit),
+                           sv = v0 + wv * (base_unit * by + ty * unit);
+                    or_pt o;
+                    eval_one(&sf, su, sv, 3, &o);
+                    double N[3], Nuu[3], Nuv[3], Nvv[3];
+                    normal_ders(&o, N, Nuu, Nuv, Nvv);
+                    for (int k = 0; k < 3; k++) {
+                        if (N[k] < lo[k]) lo[k] = N[k];
+                        if (N[k] > hi[k]) hi[k] = N[k];
+                    }
+                    M1 = fmax(M1, amax3(Nuu));
+                    M2 = fmax(M2, amax3(Nuv));
+                    M3 = fmax(M3, amax3(Nvv));
+                }
+            double K = (M1 * wu * wu + 2 * M2 * wu * wv + M3 * wv * wv) * sqrt(3.0) /
+                       (8.0 * OR_GAP * (double)g * OR_GAP * (double)g);
+            double *o4 = out + (bx * g + by) * 4;
+            int zero = 1;
+            double m[3];
+            for (int k = 0; k < 3; k++) {
+                lo[k] -= K;
+                hi[k] += K;
+                if (!(lo[k] <= 0.0 && hi[k] >= 0.0)) zero = 0;
+                m[k] = 0.5 * (lo[k] + hi[k]);
+            }
+            double th = 0.0;
+            for (int cn = 0; cn < 8 && !zero; cn++) {
+                double x[3] = {(cn & 4) ? hi[0] : lo[0], (cn & 2) ? hi[1] : lo[1], (cn & 1) ? hi[2] : lo[2]};
+                th = fmax(th, angle3(m, x));
+            }
+            double mm = sqrt(m[0] * m[0] + m[1] * m[1] + m[2] * m[2]);
+            if (zero || !(th < 0.5 * M_PI) || !(mm > 0.0)) {
+                o4[0] = o4[1] = o4[2] = 0.0;
+                o4[3] = 4.0;
+            } else {
+                for (int k = 0; k < 3; k++) o4[k] = m[k] / mm;
+                o4[3] = th;
+            }
+        }
 }
 
 /* ------------------------------------------------------------------------------------------ */

@@ -50,6 +50,8 @@ def lib():
         L.or_invert_first_hit.argtypes = [_I, _I, _I, _I, _P, _P, _P, _P, _P, _L, _P, _P, _P, _L,
                                           _D, _D, _I, _I, _P, _P, _P, _P, _I]
         L.or_seed_nearest_leaf.argtypes = [_P, _I, _P, _P, _P, _L, _P, _P, _P]
+        L.or_leaf_cones.argtypes = [_I, _I, _I, _I, _P, _P, _P, _I, _P]
+        L.or_normal_ders.argtypes = [_I, _I, _I, _I, _P, _P, _P, _P, _P, _L, _P]
         L.or_project_global.argtypes = [_I, _I, _I, _I, _P, _P, _P, _P, _I, _P, _P, _P, _L, _I, _I, _D,
                                         _P, _P, _P, _I]
         L.or_seed_greedy_leaf.argtypes = [_P, _I, _P, _P, _P, _L, _P, _P, _P, _I]
@@ -161,6 +163,21 @@ class OSurf:
         lib().or_seed_greedy_step(*self._args(), nodes.ctypes.data, levels, x.ctypes.data, y.ctypes.data,
                                   z.ctypes.data, n, leaf.ctypes.data, su.ctypes.data, sv.ctypes.data, nthreads)
         return leaf, su, sv
+
+    def normal_ders(self, u, v):
+        """(n, 4, 3): N = S_u x S_v, N_uu, N_uv, N_vv at (u, v)."""
+        u, v = _a(u), _a(v)
+        out = np.zeros((u.size, 4, 3))
+        lib().or_normal_ders(*self._args(), u.ctypes.data, v.ctypes.data, u.size, out.ctypes.data)
+        return out
+
+    def leaf_cones(self, levels):
+        """Normal cones of the leaves (level levels-1, index bx*g + by): (leaves, 4) array of unit
+        axis and half-angle (4.0: unbounded)."""
+        g = 8 ** (levels - 1)
+        out = np.zeros((g * g, 4))
+        lib().or_leaf_cones(*self._args(), levels, out.ctypes.data)
+        return out
 
     def build_tree(self, levels, flags=0):
         cnt = lib().or_tree_base(levels)

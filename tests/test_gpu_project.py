@@ -1,7 +1,10 @@
-"""Global projection on the GPU (GN_SEED_GLOBAL: branch and bound over the seed quadtree) against
-the oracle's exhaustive form (a leaf-clamped projected Newton from every leaf), and against
-properties no algorithm choice can fake: never farther than any densely sampled surface point,
-never farther than the local method."""
+"""Global projection on the GPU (GN_SEED_GLOBAL: branch and bound over the seed quadtree, pruned by
+leaf boxes and normal cones) against the oracle's exhaustive form (a leaf-clamped projected Newton
+from every leaf, no pruning), and against properties no algorithm choice can fake: never farther
+than any densely sampled surface point, never farther than the local method.  The leaves' normal
+cones against the oracle's or_leaf_cones."""
+import os
+
 import numpy as np
 import pytest
 
@@ -28,6 +31,8 @@ def _queries(os_, n, scale, seed):
 def _dist(os_, P, u, v):
     return np.linalg.norm(os_.eval(u, v, 0) - P, axis=0)
 
+
+LEAVES_SOLVED_MAX = 3.05  # boxes alone: 3.30 on the 1M-query workload, cones: 2.88
 
 GLOBAL = dict(precision=G.GN_FP64, newton=G.NEWTON_FP64, seed_mode=G.SEED_GLOBAL, max_iter=20,
               tol_step=1e-12)
@@ -58,6 +63,102 @@ def test_global_matches_exhaustive_oracle_and_dense_samples(bumpy):
     for i in range(P.shape[1]):
         assert d[i] <= np.min(np.linalg.norm(S - P[:, i:i + 1], axis=0)) + 1e-12, i
     assert np.all(ncand >= 1) and np.mean(ncand) > 1  # pruning leaves some candidates
+
+
+def test_normal_cones_match_oracle(bumpy):
+    cases = [(bumpy[0], bumpy[1], 3)]
+    sd5 = synth.surface_G(10, 9, 5, 4, seed=43, jitter=0.2)
+    g5 = G.Surface(sd5.p, sd5.q, sd5.ku, sd5.kv, sd5.cp, sd5.w)
+    g5.BuildTree(2)
+    cases.append((sd5, g5, 2))
+    for sd, g, levels in cases:
+        got = g.GetNormalCones().astype(np.float64)
+        ref = O.OSurf.from_def(sd).leaf_cones(levels)
+        bo, bg = ref[:, 3] < 1.5, got[:, 3] < 1.5
+        assert np.array_equal(bo, bg) and bo.mean() > 0.5
+        # half-angle: the oracle's plus the 1e-6 rad margin, rounded up to FP32
+        dth = got[bo, 3] - ref[bo, 3]
+        assert np.all(dth >= 1e-6 - 1e-12) and np.all(dth <= 1e-6 + 2e-7), (dth.min(), dth.max())
+        # axis: the oracle's, rounded to FP32 per component (~1e-7 rad)
+        ga, ra = got[bo, :3], ref[bo, :3]
+        ang = np.arctan2(np.linalg.norm(np.cross(ga, ra), axis=1), np.sum(ga * ra, axis=1))
+        assert np.all(ang <= 5e-7), ang.max()
+        assert np.all(got[~bo, 3] == 4.0)
+    g5.close()
+
+
+def test_cone_pruning_keeps_inner_and_outer_foot_points():
+    """An exact quarter cylinder (radius 1): queries inside (radius 0.5, the foot point's normal
+    points away from the query) and outside (radius 1.6).  The nearest point is radial, at the
+    query's angle; the cones must pass the leaf that holds it whichever side the query is on."""
+    r2 = np.sqrt(2) / 2
+    cp = np.array([[[1, 0, 0], [1, 0, 2]], [[1, 1, 0], [1, 1, 2]], [[0, 1, 0], [0, 1, 2]]], dtype=float)
+    w = np.array([[1, 1], [r2, r2], [1, 1]])
+    g = G.Surface(2, 1, np.array([0, 0, 0, 1, 1, 1.0]), np.array([0, 0, 1, 1.0]), cp, w)
+    g.BuildTree(3)
+    rng = np.random.default_rng(9)
+    n = 4000
+    phi = rng.uniform(0.05, np.pi / 2 - 0.05, n)
+    rad = np.where(rng.random(n) < 0.5, 0.5, 1.6)
+    z = rng.uniform(0.1, 1.9, n)
+    P = np.stack([rad * np.cos(phi), rad * np.sin(phi), z])
+    gu, gv, st, ncand, _ = (_np(t) for t in g.NewInverse(P[0], P[1], P[2], G.default_opts(**GLOBAL), iters=True))
+    os_ = O.OSurf(2, 1, np.array([0, 0, 0, 1, 1, 1.0]), np.array([0, 0, 1, 1.0]), cp, w)
+    S = os_.eval(gu, gv, 0)
+    assert np.max(np.abs(np.arctan2(S[1], S[0]) - phi)) < 1e-9
+    assert np.max(np.abs(S[2] - z)) < 1e-9
+    assert np.all(st == G.PT_CONVERGED)
+    g.close()
+
+
+def test_cone_pruning_cuts_leaf_solves():
+    """The projection workload's surface and query spread (bicubic 32 x 32, offsets N(0, 1e-2) along
+    the normal): results equal the exhaustive oracle's on a subset, with few leaves solved."""
+    sd = synth.surface_G(32, 32, 3, 3, seed=1002)
+    g = G.Surface(sd.p, sd.q, sd.ku, sd.kv, sd.cp, sd.w)
+    g.BuildTree(3)
+    os_ = O.OSurf.from_def(sd)
+    _, _, P = _queries(os_, 20000, 1e-2, 5)
+    gu, gv, st, ncand, _ = (_np(t) for t in g.NewInverse(P[0], P[1], P[2], G.default_opts(**GLOBAL), iters=True))
+    nodes, _, _ = os_.build_tree(3)
+    sel = np.arange(0, P.shape[1], 100)
+    Ps = P[:, sel]
+    ou, ov, _ = os_.project_global(nodes, 3, Ps[0].copy(), Ps[1].copy(), Ps[2].copy(), max_iter=20, leaf_iter=8,
+                                   tol_step=1e-12)
+    d, do = _dist(os_, Ps, gu[sel], gv[sel]), _dist(os_, Ps, ou, ov)
+    assert np.all(np.abs(d - do) <= 1e-12 + 1e-9 * do), np.max(np.abs(d - do))
+    print("leaves solved per query", ncand.mean())
+    assert ncand.mean() < LEAVES_SOLVED_MAX
+    g.close()
+
+
+@pytest.mark.parametrize("name,cap", [("test_face2.nu", 2.6), ("center_bk.nu", 2.2)])
+def test_cone_pruning_on_reference_models(golden_dir, name, cap):
+    """The reference's CAD models, whose leaf cones are a few degrees wide: far fewer leaves solved
+    (boxes alone: about 4.2 and 4.7 per query) and the exhaustive oracle's nearest points."""
+    from helpers import read_nu
+    p, q, ku, kv, cp, w = read_nu(os.path.join(golden_dir, name))
+    g = G.Surface(p, q, ku, kv, cp, w)
+    g.BuildTree(3)
+    os_ = O.OSurf(p, q, ku, kv, cp, w)
+    rng = np.random.default_rng(11)
+    n = 20000
+    u0, v0 = rng.random(n), rng.random(n)
+    ev = os_.eval(u0, v0, 1)
+    nrm = np.cross(ev[3:6].T, ev[6:9].T).T
+    nrm /= np.linalg.norm(nrm, axis=0)
+    diag = np.linalg.norm(cp.reshape(-1, 3).max(0) - cp.reshape(-1, 3).min(0))
+    P = ev[0:3] + rng.normal(0.0, 1e-2 * diag, n) * nrm
+    gu, gv, st, ncand, _ = (_np(t) for t in g.NewInverse(P[0], P[1], P[2], G.default_opts(**GLOBAL), iters=True))
+    assert ncand.mean() < cap, ncand.mean()
+    nodes, _, _ = os_.build_tree(3)
+    sel = np.arange(0, n, 100)
+    Ps = P[:, sel]
+    ou, ov, _ = os_.project_global(nodes, 3, Ps[0].copy(), Ps[1].copy(), Ps[2].copy(), max_iter=20, leaf_iter=8,
+                                   tol_step=1e-12)
+    d, do = _dist(os_, Ps, gu[sel], gv[sel]), _dist(os_, Ps, ou, ov)
+    assert np.all(np.abs(d - do) <= 1e-12 + 1e-9 * do), np.max(np.abs(d - do))
+    g.close()
 
 
 def test_global_never_farther_than_local_and_sometimes_nearer(bumpy):
