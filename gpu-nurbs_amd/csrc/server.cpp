@@ -370,7 +370,17 @@ void gn_server::run() {
                 continue;
             }
         }
-        cv_work.wait_for(lk, std::chrono::milliseconds(2), [this] { return dirty || stop; });
+        // Sleeps, releasing mu, until a submit or a finished step marks the state dirty.  While
+        // stopping with steps in flight the wait must release mu too: the slots' threads need it
+        // to finish their steps (a predicate that held on `stop` alone returned at once, so this
+        // loop kept mu and the slots blocked in finish: gn_server_destroy never returned).
+        cv_work.wait_for(lk, std::chrono::milliseconds(2), [this] {
+            if (dirty) return true;
+            if (!stop) return false;
+            for (const Slot &sl : slots)
+                if (sl.busy) return false;
+            return true;
+        });
     }
 }
 

@@ -173,6 +173,20 @@ def test_concurrent_clients(world):
         srv.close()
 
 
+@pytest.mark.timeout(120)
+def test_close_with_steps_in_flight(world):
+    """gn_server_destroy right after submits, while steps run, returns every time: the dispatcher
+    releases its lock while it waits for the slots' steps to finish (it once held it, and the slots
+    blocked finishing their steps).  A request of a closed server is either done or failed."""
+    Ps = [queries(world, k, 2000, 50 + k) for k in range(8)]
+    for rep in range(30):
+        srv = G.Server(world["batch"], world["curves"], G.default_opts(), hbm_budget=1 << 28)
+        for k in range(8):
+            srv.submit(k, *Ps[k])
+        srv.close()  # returns: no deadlock with the steps in flight
+
+
+@pytest.mark.timeout(120)
 def test_server_errors(world):
     srv = G.Server(world["batch"], world["curves"], G.default_opts(), hbm_budget=1 << 28)
     try:
