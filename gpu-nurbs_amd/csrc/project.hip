@@ -23,6 +23,7 @@
 namespace gn {
 
 constexpr int kLeafIter = 8;  // projected Newton steps inside one leaf cell
+constexpr int kCand = 8;      // leaves collected per lane before the wave solves them
 
 // Whether a leaf can hold an interior foot point of P: a point X of its box b with P - X parallel
 // to a normal n of its cone (unit axis a, half-angle theta < 90 degrees), i.e. whether the box meets
@@ -129,9 +130,11 @@ __device__ __forceinline__ int solve_in(const GnSurfDev &d, const double *P, dou
 
 // The branch and bound for query i.  Global mode (counts): also writes the number of leaves
 // solved to it32.  Fallback (!counts): the (u, v) already in place is kept when it is nearer.
+// cand: the workgroup's LDS candidate slots, kCand per thread (blocks of 256 threads).
 template <int K1, int K2, bool EXACT>
 __device__ __forceinline__ void project_point(const GnSurfDev &d, const GnSearchDev &sd,
-                                              const InvArgs &a, size_t i, bool counts) {
+                                              const InvArgs &a, size_t i, bool counts,
+                                              uint32_t *cand) {
     const Domain full = domain_of(d);
     const int D = sd.depth;
     // leaf cells: the unit square's 2^D x 2^D cells mapped onto the knot domain, exactly as the
@@ -147,48 +150,66 @@ __device__ __forceinline__ void project_point(const GnSurfDev &d, const GnSearch
     int ncand = 1;
     if (D > 0) {
         const float p[3] = {(float)P[0], (float)P[1], (float)P[2]};
-        // stackless depth-first walk (Morton-code successor), pruned by the current best
+        // stackless depth-first walk (Morton-code successor), pruned by the current best.  The
+        // walk collects up to kCand leaves at a time into this lane's LDS slots, then the wave
+        // solves them in rounds, each lane its r-th leaf: lanes walk different paths, but they
+        // solve together.  A collected leaf is solved only if its box is still not farther than
+        // the best so far; a leaf box is never nearer than its ancestors' boxes, so exactly the
+        // leaves of a walk that solved each leaf as it reached it are solved, in the same order.
         const uint32_t side = 1u << D;
+        uint32_t *cq = cand + threadIdx.x * kCand;
         int l = 1;
         uint32_t c = 0;
-        while (true) {
-            const double lb = (double)box_d2(sd.box32 + (qoff(l) + c) * 6, p);
-            bool descend = false;
-            if (lb <= best) {
-                if (l < D) {
-                    descend = true;
-                } else if (c != c0 &&
-                           (!sd.cone32 || compact2(c >> 1) - 1u >= side - 2u ||
-                            compact2(c) - 1u >= side - 2u ||
-                            cone_may_hold(sd.cone32[c], sd.box32 + (qoff(l) + c) * 6, P))) {
-                    // (the two index tests pass the leaves on the domain boundary, ix or iy
-                    // 0 or side - 1, by unsigned wrap-around)
-                    const uint32_t ix = compact2(c >> 1), iy = compact2(c);
-                    const Domain cr{full.u0 + wu * (ix * cell), full.u0 + wu * ((ix + 1) * cell),
-                                    full.v0 + wv * (iy * cell), full.v0 + wv * ((iy + 1) * cell)};
-                    double cu = sd.seed_uv[2 * c], cv = sd.seed_uv[2 * c + 1], d2;
-                    const int s2 = solve_in<K1, K2, EXACT>(d, P, cu, cv, cr, kLeafIter, tol_step,
-                                                           d2);
-                    ncand++;
-                    if (d2 < best) {
-                        best = d2;
-                        bu = cu;
-                        bv = cv;
-                        bst = s2;
+        bool walking = true;
+        while (__any(walking)) {  // wave-uniform
+            int nq = 0;
+            while (walking && nq < kCand) {
+                const double lb = (double)box_d2(sd.box32 + (qoff(l) + c) * 6, p);
+                bool descend = false;
+                if (lb <= best) {
+                    if (l < D) {
+                        descend = true;
+                    } else if (c != c0 &&
+                               (!sd.cone32 || compact2(c >> 1) - 1u >= side - 2u ||
+                                compact2(c) - 1u >= side - 2u ||
+                                cone_may_hold(sd.cone32[c], sd.box32 + (qoff(l) + c) * 6, P))) {
+                        // (the two index tests pass the leaves on the domain boundary, ix or iy
+                        // 0 or side - 1, by unsigned wrap-around)
+                        cq[nq++] = c;
                     }
                 }
+                if (descend) {
+                    c <<= 2;
+                    l++;
+                    continue;
+                }
+                while (l > 1 && (c & 3u) == 3u) {
+                    c >>= 2;
+                    l--;
+                }
+                if (l == 1 && (c & 3u) == 3u) {
+                    walking = false;
+                    break;
+                }
+                c++;
             }
-            if (descend) {
-                c <<= 2;
-                l++;
-                continue;
+            for (int r = 0; __any(r < nq); r++) {  // wave-uniform rounds
+                if (r >= nq) continue;
+                const uint32_t lc = cq[r];
+                if (!((double)box_d2(sd.box32 + (qoff(D) + lc) * 6, p) <= best)) continue;
+                const uint32_t ix = compact2(lc >> 1), iy = compact2(lc);
+                const Domain cr{full.u0 + wu * (ix * cell), full.u0 + wu * ((ix + 1) * cell),
+                                full.v0 + wv * (iy * cell), full.v0 + wv * ((iy + 1) * cell)};
+                double cu = sd.seed_uv[2 * lc], cv = sd.seed_uv[2 * lc + 1], d2;
+                const int s2 = solve_in<K1, K2, EXACT>(d, P, cu, cv, cr, kLeafIter, tol_step, d2);
+                ncand++;
+                if (d2 < best) {
+                    best = d2;
+                    bu = cu;
+                    bv = cv;
+                    bst = s2;
+                }
             }
-            while (l > 1 && (c & 3u) == 3u) {
-                c >>= 2;
-                l--;
-            }
-            if (l == 1 && (c & 3u) == 3u) break;
-            c++;
         }
     }
     double pu = bu, pv = bv, pd2;
@@ -223,11 +244,12 @@ template <int K1, int K2, bool EXACT>
 constexpr int project_waves() { return (!EXACT || K1 * K2 <= 20) ? 2 : 1; }
 
 template <int K1, int K2, bool EXACT>
-__global__ __launch_bounds__(256, (project_waves<K1, K2, EXACT>())) void k_project_global(GnSurfDev d,
-                                                                              GnSearchDev sd, InvArgs a) {
+__global__ __launch_bounds__(256, (project_waves<K1, K2, EXACT>())) void k_project_global(
+    GnSurfDev d, GnSearchDev sd, InvArgs a) {
+    __shared__ uint32_t cand[256 * kCand];
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < a.n;
          i += (size_t)gridDim.x * blockDim.x)
-        project_point<K1, K2, EXACT>(d, sd, a, i, true);
+        project_point<K1, K2, EXACT>(d, sd, a, i, true, cand);
 }
 
 // The fallback: the queries listed in list[0 .. *count) (iteration counts left as they were).
@@ -235,10 +257,11 @@ template <int K1, int K2, bool EXACT>
 __global__ __launch_bounds__(256) void k_project_list(GnSurfDev d, GnSearchDev sd, InvArgs a,
                                                       const uint32_t *__restrict__ list,
                                                       const uint32_t *__restrict__ count) {
+    __shared__ uint32_t cand[256 * kCand];
     const size_t cnt = *count;
     for (size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x; j < cnt;
          j += (size_t)gridDim.x * blockDim.x)
-        project_point<K1, K2, EXACT>(d, sd, a, list[j], false);
+        project_point<K1, K2, EXACT>(d, sd, a, list[j], false, cand);
 }
 
 __global__ void k_fb_zero(uint32_t *count) { *count = 0u; }
