@@ -125,7 +125,7 @@ __device__ __forceinline__ Domain domain_of(const GnSurfDev &d) {
 // one-dimensional second derivative where that is positive (a plain clamp of the coupled step can
 // walk uphill along an edge).  rr_out: the squared residual at the point evaluated.
 template <int K1, int K2, bool EXACT, typename T, bool NEWTON, int PSRC = kPatchGlobal,
-          bool ACTIVE = false, bool GKNOTS = false>
+          bool ACTIVE = false, bool NEAR = false>
 __device__ __forceinline__ int gn_step(const GnSurfDev &d, const double *__restrict__ ku,
                                        const double *__restrict__ kv, T Px, T Py, T Pz,
                                        double &cu, double &cv, double tol_dist, double tol_step,
@@ -133,7 +133,7 @@ __device__ __forceinline__ int gn_step(const GnSurfDev &d, const double *__restr
                                        SpanCache &sc, Hom<T> *pr = nullptr,
                                        float4 *wave_lds = nullptr, T *rr_out = nullptr) {
     Cart<T> o;
-    eval_point_cached<K1, K2, NEWTON ? 2 : 1, EXACT, T, PSRC, GKNOTS>(d, ku, kv, cu, cv, sc, o, pr,
+    eval_point_cached<K1, K2, NEWTON ? 2 : 1, EXACT, T, PSRC, NEAR>(d, ku, kv, cu, cv, sc, o, pr,
                                                               wave_lds);
     const T r0 = o.S[0] - Px, r1 = o.S[1] - Py, r2 = o.S[2] - Pz;
     const T rr = r0 * r0 + r1 * r1 + r2 * r2;
@@ -345,7 +345,7 @@ __global__ __launch_bounds__(kInvBlock, (EXACT && sizeof(T) == 4 && K1 * K2 <= 1
         const bool act = idx >= 0 && it < lim;
         if (kWave || act) {
             double tu = act ? cu : dom.u0, tv = act ? cv : dom.v0;
-            const int s2 = gn_step<K1, K2, EXACT, T, NEWTON, PSRC>(
+            const int s2 = gn_step<K1, K2, EXACT, T, NEWTON, PSRC, false, MODE == kBulk>(
                 d, ku, kv, Px, Py, Pz, tu, tv, tol_dist, tol_step, tol_quad, a.o.clamp, dom, sc,
                 pr.m, wave_lds);
             if (act) {
