@@ -69,8 +69,9 @@ __global__ __launch_bounds__(kSeedBlock) void k_batch_seed(const GnBatchObj *__r
         const int64_t i = tl.start + j;
         const double P[3] = {R.x[i], R.y[i], R.z[i]};
         float4 cq;
-        const uint32_t c = mode == GN_SEED_NEAREST_LEAF ? nearest_leaf(sd, P)
-                                                        : greedy_leaf(sd, P, lbox, ll, &cq);
+        const uint32_t c = mode == GN_SEED_NEAREST_LEAF
+                               ? nearest_leaf(sd, P)
+                               : greedy_leaf(sd, P, lbox, ll, &cq, mode != GN_SEED_GREEDY_STEP);
         if (mode == GN_SEED_GREEDY_STEP) {
             double u, v;
             step_seed(sd, c, cq, P, u, v);

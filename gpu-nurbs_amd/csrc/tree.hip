@@ -364,7 +364,7 @@ int launch_search_build(const GnSurfDev &d, int levels, const GnTreeNode *tree, 
     return GN_OK;
 }
 
-// One lane per query; mode 0 exact nearest leaf, mode 1 greedy + 3x3, mode 2 greedy + 3x3 and
+// One lane per query; mode 0 exact nearest leaf, mode 1 greedy + 3x3, mode 2 greedy descent and
 // the Gauss-Newton step of step_seed.
 // Levels of the seed quadtree whose FP32 boxes the greedy seed kernel stages in LDS: levels
 // 1..5 are 1364 boxes, 32 KB.
@@ -391,7 +391,8 @@ __global__ __launch_bounds__(256) void k_seed(GnSearchDev sd, int mode,
          i += (size_t)gridDim.x * blockDim.x) {
         const double P[3] = {x[i], y[i], z[i]};
         float4 cq;
-        const uint32_t c = mode == 0 ? nearest_leaf(sd, P) : greedy_leaf(sd, P, lbox, ll, &cq);
+        const uint32_t c = mode == 0 ? nearest_leaf(sd, P)
+                                     : greedy_leaf(sd, P, lbox, ll, &cq, mode != 2);
         if (leaf) leaf[i] = sd.ref_index[c];
         if (mode == 2) {
             double u, v;

@@ -68,10 +68,11 @@ typedef enum {
                                  wins.  iters_fp32 of gn_invert_iters receives the number of
                                  leaves solved.  The reference's normal-cone projection
                                  (src/kernel/kernel.cu:471-513, 623-704) is unfinished. */
-    GN_SEED_GREEDY_STEP = 5   /* GN_SEED_GREEDY_LEAF, then one Gauss-Newton step from the leaf's
-                                 seed with the Jacobian stored for that leaf (S_u, S_v at its
-                                 seed), limited to one leaf cell and clamped to the domain: no
-                                 surface evaluation, and about one iteration fewer after it */
+    GN_SEED_GREEDY_STEP = 5   /* the greedy descent's leaf (GN_SEED_GREEDY_LEAF without its 3 x 3
+                                 neighbour refinement), then one Gauss-Newton step from the
+                                 leaf's seed with the Jacobian stored for that leaf (S_u, S_v at
+                                 its seed), limited to one leaf cell and clamped to the domain:
+                                 no surface evaluation, and about one iteration fewer after it */
 } gn_seed_mode;
 
 /* tree build flags */

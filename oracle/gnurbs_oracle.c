@@ -878,9 +878,10 @@ void or_seed_nearest_leaf(const double *nodes, int levels, const double *x, cons
  * the lower child, children ordered (x,y) = (0,0),(0,1),(1,0),(1,1)); then the nearest leaf
  * centre (FP32 centres and distances, ties to the lower reference index) among the 3 x 3 leaves
  * around it. */
-void or_seed_greedy_leaf(const double *nodes, int levels, const double *x, const double *y,
-                         const double *z, long n, int *leaf_out, double *seed_u, double *seed_v,
-                         int nthreads)
+/* refine: 1 = the 3 x 3 neighbour refinement of GN_SEED_GREEDY_LEAF; 0 = the descent's leaf */
+static void greedy_leaf_impl(const double *nodes, int levels, const double *x, const double *y,
+                             const double *z, long n, int *leaf_out, double *seed_u,
+                             double *seed_v, int nthreads, int refine)
 {
     int D = 3 * (levels - 1);
     long side = 1L << D, base = or_tree_base(levels - 1);
@@ -959,7 +960,7 @@ void or_seed_greedy_leaf(const double *nodes, int levels, const double *x, const
         /* 3 x 3 leaves around the descent's leaf: FP32 centres and distances */
         float best = INFINITY;
         long bi = base + ix * side + iy;
-        for (int dx = -1; dx <= 1; dx++)
+        for (int dx = -1; dx <= 1 && refine; dx++)
             for (int dy = -1; dy <= 1; dy++) {
                 long xx = ix + dx, yy = iy + dy;
                 if (xx < 0 || yy < 0 || xx >= side || yy >= side) continue;
@@ -979,7 +980,23 @@ void or_seed_greedy_leaf(const double *nodes, int levels, const double *x, const
     free(b32);
 }
 
-/* The library's GN_SEED_GREEDY_STEP (no reference counterpart): the greedy leaf, then one
+void or_seed_greedy_leaf(const double *nodes, int levels, const double *x, const double *y,
+                         const double *z, long n, int *leaf_out, double *seed_u, double *seed_v,
+                         int nthreads)
+{
+    greedy_leaf_impl(nodes, levels, x, y, z, n, leaf_out, seed_u, seed_v, nthreads, 1);
+}
+
+/* The greedy descent's leaf alone (no neighbour refinement), as GN_SEED_GREEDY_STEP starts from. */
+void or_seed_greedy_descent(const double *nodes, int levels, const double *x, const double *y,
+                            const double *z, long n, int *leaf_out, double *seed_u,
+                            double *seed_v, int nthreads)
+{
+    greedy_leaf_impl(nodes, levels, x, y, z, n, leaf_out, seed_u, seed_v, nthreads, 0);
+}
+
+/* The library's GN_SEED_GREEDY_STEP (no reference counterpart): the greedy descent's leaf (no
+ * neighbour refinement: the step below moves up to a leaf cell anyway), then one
  * Gauss-Newton step from the leaf's seed (its uv centre, whose surface point is the node's
  * centre C) with J = [S_u S_v] evaluated there, (du, dv) = (J^T J)^-1 J^T (C - P), computed in
  * FP32 from C, P and J rounded to FP32, scaled down uniformly to at most one leaf cell per
@@ -991,7 +1008,7 @@ void or_seed_greedy_step(int p, int q, int nu, int nv, const double *ku, const d
                          double *seed_v, int nthreads)
 {
     or_surf sf = {p, q, nu, nv, ku, kv, MatrixN};
-    or_seed_greedy_leaf(nodes, levels, x, y, z, n, leaf_out, seed_u, seed_v, nthreads);
+    or_seed_greedy_descent(nodes, levels, x, y, z, n, leaf_out, seed_u, seed_v, nthreads);
     double u0 = ku[p], u1 = ku[nu], v0 = kv[q], v1 = kv[nv];
     float cells = (float)(1L << (3 * (levels - 1)));
     float hu = (float)(u1 - u0) / cells, hv = (float)(v1 - v0) / cells;

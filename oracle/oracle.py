@@ -55,6 +55,7 @@ def lib():
         L.or_project_global.argtypes = [_I, _I, _I, _I, _P, _P, _P, _P, _I, _P, _P, _P, _L, _I, _I, _D,
                                         _P, _P, _P, _I]
         L.or_seed_greedy_leaf.argtypes = [_P, _I, _P, _P, _P, _L, _P, _P, _P, _I]
+        L.or_seed_greedy_descent.argtypes = [_P, _I, _P, _P, _P, _L, _P, _P, _P, _I]
         L.or_seed_greedy_step.argtypes = [_I, _I, _I, _I, _P, _P, _P, _P, _I, _P, _P, _P, _L, _P, _P, _P, _I]
         L.or_curve_eval.argtypes = [_I, _I, _P, _P, _P, _P, _L, _I, _P, _I]
         L.or_curve_seed.argtypes = [_I, _I, _P, _P, _P, _P, _P, _P, _L, _P]
@@ -230,6 +231,18 @@ def seed_greedy_leaf(nodes, levels, x, y, z, nthreads=0):
     su, sv = np.zeros(n), np.zeros(n)
     lib().or_seed_greedy_leaf(nodes.ctypes.data, levels, x.ctypes.data, y.ctypes.data, z.ctypes.data, n,
                               leaf.ctypes.data, su.ctypes.data, sv.ctypes.data, nthreads)
+    return leaf, su, sv
+
+
+def seed_greedy_descent(nodes, levels, x, y, z, nthreads=0):
+    """The greedy descent's leaf and seed, without the 3 x 3 refinement (GN_SEED_GREEDY_STEP's start)."""
+    nodes = _a(nodes)
+    x, y, z = (_a(a) for a in (x, y, z))
+    n = x.size
+    leaf = np.zeros(n, dtype=np.int32)
+    su, sv = np.zeros(n), np.zeros(n)
+    lib().or_seed_greedy_descent(nodes.ctypes.data, levels, x.ctypes.data, y.ctypes.data, z.ctypes.data, n,
+                                 leaf.ctypes.data, su.ctypes.data, sv.ctypes.data, nthreads)
     return leaf, su, sv
 
 

@@ -92,12 +92,16 @@ def test_seeds_match_oracle(cases, pq):
 
 @pytest.mark.parametrize("pq", CLASSES, ids=lambda pq: f"p{pq[0]}q{pq[1]}")
 def test_step_seeds_match_oracle(cases, pq):
-    """GN_SEED_GREEDY_STEP: the same leaf as the oracle, and the same FP32 step: bit-identical
-    seeds except where a leaf Jacobian entry, evaluated in FP64 by two computations that round
-    differently, rounds to a different FP32 value (then within 1e-9)."""
+    """GN_SEED_GREEDY_STEP: the same leaf as the oracle (the greedy descent's, without the 3 x 3
+    refinement of GN_SEED_GREEDY_LEAF), and the same FP32 step: bit-identical seeds except where a
+    leaf Jacobian entry, evaluated in FP64 by two computations that round differently, rounds to a
+    different FP32 value (then within 1e-9)."""
     c = cases[pq]
     same = c["sleaf"] == c["tleaf"]
-    assert same.mean() > 0.999 and np.array_equal(c["sleaf"], c["leaf"])
+    assert same.mean() > 0.999
+    # the 3 x 3 refinement of GN_SEED_GREEDY_LEAF moves the leaf of some of the queries
+    ref = c["leaf"]
+    assert 0.5 < (c["sleaf"] == ref).mean() < 1.0  # about 2/3 on these surfaces
     eq = (c["ssu"] == c["tsu"]) & (c["ssv"] == c["tsv"])
     assert eq[same].mean() > 0.99
     assert np.max(np.abs(c["ssu"] - c["tsu"])[same]) <= 1e-9

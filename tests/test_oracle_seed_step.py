@@ -1,8 +1,8 @@
-"""The oracle's greedy-leaf seed with one FP32 Gauss-Newton step (the library's GN_SEED_GREEDY_STEP;
-no reference counterpart -- the reference seeds Gauss-Newton at a node's uv centre,
-src/kernel/kernel.cu:156, 433-469), pinned by an independent numpy restatement on scipy's
-B-spline basis, and by what the step is for: seeds nearer the foot points and fewer iterations
-after them."""
+"""The oracle's greedy-descent seed with one FP32 Gauss-Newton step (the library's
+GN_SEED_GREEDY_STEP; no reference counterpart -- the reference seeds Gauss-Newton at a node's uv
+centre, src/kernel/kernel.cu:156, 433-469), pinned by independent numpy restatements of the descent
+(FP32 boxes built from the tree's nodes) and of the step (scipy's B-spline basis), and by what the
+step is for: seeds nearer the foot points and fewer iterations after them."""
 import numpy as np
 
 import oracle as O
@@ -22,12 +22,63 @@ def _case(seed, n=20000, levels=3, ku_scale=1.0):
     return sd, ku, os_, nodes, u0, v0, P
 
 
+def _descent(nodes, levels, P):
+    """numpy restatement of the greedy descent: quadtree boxes from the leaves' boxes (each widened
+    to hold its centre), rounded outward to FP32 after a margin of 2^-20 (1 + the root's largest
+    coordinate magnitude); at each level the child of least FP32 squared box distance (the first
+    of equals); returns the reference index of the leaf reached."""
+    D = 3 * (levels - 1)
+    side = 1 << D
+    base = sum(64 ** l for l in range(levels - 1))
+    nd = nodes[base:base + side * side]
+    lo = np.minimum(nd[:, 4:7], nd[:, 10:13]).reshape(side, side, 3)
+    hi = np.maximum(nd[:, 7:10], nd[:, 10:13]).reshape(side, side, 3)
+    los, his = [lo], [hi]
+    for _ in range(D):
+        l, h = los[0], his[0]
+        s = l.shape[0] // 2
+        los.insert(0, l.reshape(s, 2, s, 2, 3).min(axis=(1, 3)))
+        his.insert(0, h.reshape(s, 2, s, 2, 3).max(axis=(1, 3)))
+    mag = max(np.abs(los[0]).max(), np.abs(his[0]).max())
+    m = (1.0 + mag) * 2.0 ** -20
+
+    def down(x):
+        f = (x - m).astype(np.float32)
+        return np.where(f.astype(np.float64) > x - m, np.nextafter(f, np.float32(-np.inf)), f)
+
+    def up(x):
+        f = (x + m).astype(np.float32)
+        return np.where(f.astype(np.float64) < x + m, np.nextafter(f, np.float32(np.inf)), f)
+
+    l32 = [down(x) for x in los]
+    h32 = [up(x) for x in his]
+    p = P.T.astype(np.float32)
+    n = p.shape[0]
+    ix, iy = np.zeros(n, dtype=np.int64), np.zeros(n, dtype=np.int64)
+    for lev in range(1, D + 1):
+        best = np.full(n, np.inf, dtype=np.float32)
+        bx, by = 2 * ix, 2 * iy
+        for a in range(4):
+            cx, cy = 2 * ix + (a >> 1), 2 * iy + (a & 1)
+            bl, bh = l32[lev][cx, cy], h32[lev][cx, cy]
+            e = np.maximum(np.maximum(bl - p, p - bh), np.float32(0))
+            s = np.float32(0)
+            for k in range(3):  # fmaf(e, e, s) in FP32: exact products of FP32 values, one rounding
+                s = (e[:, k].astype(np.float64) ** 2 + np.asarray(s, np.float64)).astype(np.float32)
+            take = s < best
+            best = np.where(take, s, best)
+            bx, by = np.where(take, cx, bx), np.where(take, cy, by)
+        ix, iy = bx, by
+    return (base + ix * side + iy).astype(np.int32)
+
+
 def test_step_equals_an_independent_gauss_newton_step():
     for scale in (1.0, 5.0):  # also on a knot domain other than [0, 1]
         sd, ku, os_, nodes, u0, v0, P = _case(3, ku_scale=scale)
         leaf, su, sv = os_.seed_greedy_step(nodes, 3, P[0], P[1], P[2])
-        gl, gu_, gv_ = O.seed_greedy_leaf(nodes, 3, P[0], P[1], P[2])
+        gl, gu_, gv_ = O.seed_greedy_descent(nodes, 3, P[0], P[1], P[2])
         assert np.array_equal(leaf, gl)
+        assert np.array_equal(gl, _descent(nodes, 3, P))
         # numpy: J at the leaf's seed from scipy's basis, C from the tree node
         d = scipy_eval(sd.p, sd.q, ku, sd.kv, sd.cp, sd.w, gu_, gv_, order=1)
         C = nodes[leaf, 10:13].T
