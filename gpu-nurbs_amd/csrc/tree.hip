@@ -368,7 +368,16 @@ int launch_search_build(const GnSurfDev &d, int levels, const GnTreeNode *tree, 
 // the Gauss-Newton step of step_seed.
 // Levels of the seed quadtree whose FP32 boxes the greedy seed kernel stages in LDS: levels
 // 1..5 are 1364 boxes, 32 KB.
-constexpr int kSeedLdsLevels = 5;
+// Staged levels and workgroups per CU of the greedy seed kernel (GN_SEED_GREEDY_STEP on 1M
+// projection queries: 5 levels x 4 workgroups 41 us; 5 x 5 45 us; 4 x 8 43 us; 4 x 16 45 us;
+// 3 x 16 47 us).
+#ifndef GN_SEED_LDS_LEVELS
+#define GN_SEED_LDS_LEVELS 5
+#endif
+#ifndef GN_SEED_WG_PER_CU
+#define GN_SEED_WG_PER_CU 4
+#endif
+constexpr int kSeedLdsLevels = GN_SEED_LDS_LEVELS;
 
 __global__ __launch_bounds__(256) void k_seed(GnSearchDev sd, int mode,
                                               const double *__restrict__ x,
@@ -412,7 +421,7 @@ int launch_seed(const GnSearchDev &sd, int mode, const double *x, const double *
     if (n == 0) return GN_OK;
     // greedy (with or without the step): a few long-lived workgroups per CU, each staging the
     // upper tree levels in LDS once (a workgroup per 256 points would copy 32 KB per 256 points)
-    const unsigned cap = mode != 0 ? 256 * 4 : 256 * 32;
+    const unsigned cap = mode != 0 ? 256 * GN_SEED_WG_PER_CU : 256 * 32;
     k_seed<<<grid_for(n, 256, cap), 256, 0, st>>>(sd, mode, x, y, z, n, leaf, su, sv);
     GN_HIP(hipGetLastError());
     return GN_OK;
