@@ -107,16 +107,58 @@ class NurbsSurface {
                       gn_stream st = nullptr) const {
         check(gn_evaluate_grid_f32(s_, us, nu, vs, nv, order, out, st));
     }
+    // FP32 points in any order on the matrix cores: column k of out is input point perm[k]
+    void EvaluateSorted(const float *u, const float *v, size_t n, int order, float *out,
+                        uint32_t *perm, gn_stream st = nullptr) const {
+        check(gn_evaluate_sorted_f32(s_, u, v, n, order, out, perm, st));
+    }
     // tree of `levels` 8x8 levels (the reference's `level` loop of src/main.cu:283)
     void BuildTree(int levels, int flags = 0, gn_stream st = nullptr) {
         check(gn_build_tree(s_, levels, flags, st));
         levels_ = levels;
+    }
+    // host copy of the tree: 13 doubles per node (uv box, xyz box, centre), valid, divided
+    void GetTree(std::vector<double> &nodes, std::vector<uint8_t> &valid,
+                 std::vector<uint8_t> &divided) const {
+        size_t count = 0, g = 1;
+        for (int l = 0; l < levels_; l++, g *= 8) count += g * g;
+        nodes.resize(13 * count);
+        valid.resize(count);
+        divided.resize(count);
+        check(gn_get_tree(s_, nodes.data(), valid.data(), divided.data(), count));
+    }
+    // normal cones of the finest-level leaves: axis xyz and half-angle (4: unbounded) per leaf
+    std::vector<float> GetNormalCones() const {
+        size_t g = 1;
+        for (int l = 1; l < levels_; l++) g *= 8;
+        std::vector<float> c(4 * g * g);
+        check(gn_get_normal_cones(s_, c.data(), g * g));
+        return c;
+    }
+    // seed search alone (exact: 0 greedy leaf, 2 greedy descent + one Gauss-Newton step, other
+    // values the exact nearest leaf) into device leaf indices and seeds
+    void Seed(const double *x, const double *y, const double *z, size_t n, int32_t *leaf,
+              double *su, double *sv, int exact = 0, gn_stream st = nullptr) const {
+        check(gn_seed(s_, exact, x, y, z, n, leaf, su, sv, st));
     }
     // inversion of device points (x, y, z) into device (u, v, status)
     void NewInverse(const double *x, const double *y, const double *z, size_t n, double *u,
                     double *v, uint8_t *status, const gn_invert_opts &o = default_opts(),
                     gn_stream st = nullptr) const {
         check(gn_invert(s_, x, y, z, n, &o, u, v, status, st));
+    }
+    // NewInverse with per-point iteration counts (FP32 bulk, FP64); either may be null
+    void NewInverseIters(const double *x, const double *y, const double *z, size_t n, double *u,
+                         double *v, uint8_t *status, uint8_t *iters_fp32, uint8_t *iters_fp64,
+                         const gn_invert_opts &o = default_opts(), gn_stream st = nullptr) const {
+        check(gn_invert_iters(s_, x, y, z, n, &o, u, v, status, iters_fp32, iters_fp64, st));
+    }
+    // NewInverse for callers that cannot allocate (scratch is a fixed device array; calls that
+    // overlap on one GPU race)
+    void NewInverseNoAlloc(const double *x, const double *y, const double *z, size_t n, double *u,
+                           double *v, uint8_t *status, const gn_invert_opts &o = default_opts(),
+                           gn_stream st = nullptr) const {
+        check(gn_invert_noalloc(s_, x, y, z, n, &o, u, v, status, st));
     }
     // p-curve of `c` on this surface: control points (u, v) and knots of the fitted B-spline
     struct PCurveFit {
