@@ -25,8 +25,8 @@ PEAK_F32 = 157.3e12
 PEAK_F64 = 78.6e12
 
 
-# The mixed form's options: greedy leaf + one Gauss-Newton step with the leaf's stored Jacobian as
-# the seed; FP32 exact-Hessian Newton iterations to a 1e-5 step, at most 8 (the few points that need
+# The mixed form's options: the greedy descent's leaf + one Gauss-Newton step with the leaf's stored
+# Jacobian as the seed; FP32 exact-Hessian Newton iterations to a 1e-5 step, at most 8 (the few points that need
 # more are the long dependent chains every wave of the persistent kernel would wait for: a cap of 20
 # cost 0.236 ms per 1M-point step against 0.205 ms, for 0.25% more points within 1e-9); then at most
 # 3 FP64 exact-Hessian closing steps until the Newton certificate bounds the (u,v) error by 1e-10,
@@ -679,7 +679,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": "projection: bicubic rational surface, 32x32 net (generator G, seed 1002), "
                                f"{N} queries offset N(0,1e-2) along the normal, depth-6 tree (64x64 leaves) "
-                               "greedy-leaf seeds + one Gauss-Newton step (leaf Jacobian), <=8 FP32 + <=3 FP64 "
+                               "greedy-descent seeds + one Gauss-Newton step (leaf Jacobian), <=8 FP32 + <=3 FP64 "
                                "iterations",
                    "points": N, "surface": "p=q=3, 32x32", "tree_depth": 6},
         "roofline": roof,

@@ -108,7 +108,8 @@ __host__ int dispatch_inv(const GnSurfDev &d, F &&f) {
 // One Gauss-Newton update in arithmetic T on the parameter state (cu, cv) (always FP64).
 // Returns kStepContinue, or the point's final status: GN_PT_CONVERGED_DIST (|S-P| < tol_dist,
 // checked before the update as the reference does), GN_PT_CONVERGED (step below tol_step after
-// the update), GN_PT_SINGULAR (det(J^T J) == 0) or GN_PT_NONFINITE.
+// the update), GN_PT_SINGULAR (det(J^T J) == 0) or GN_PT_NONFINITE (a NaN or infinite residual,
+// e.g. a query with a non-finite coordinate, or a non-finite update).
 // With NEWTON the residual-curvature term sum_k r_k d2S_k is added to J^T J when the sum stays
 // positive definite (exact Hessian of |S - P|^2 / 2), otherwise the plain J^T J is used.
 // Parameter domain [u0, u1] x [v0, v1] of a surface: k[p], k[nu] and k[q], k[nv].
@@ -138,7 +139,7 @@ __device__ __forceinline__ int gn_step(const GnSurfDev &d, const double *__restr
     const T r0 = o.S[0] - Px, r1 = o.S[1] - Py, r2 = o.S[2] - Pz;
     const T rr = r0 * r0 + r1 * r1 + r2 * r2;
     if (rr_out) *rr_out = rr;
-    if (!(rr == rr)) return GN_PT_NONFINITE;
+    if (!(rr < T(INFINITY))) return GN_PT_NONFINITE;  // NaN or infinite query (or residual)
     if (tol_dist > 0 && sqrt(rr) < T(tol_dist)) return GN_PT_CONVERGED_DIST;
     T a = o.Su[0] * o.Su[0] + o.Su[1] * o.Su[1] + o.Su[2] * o.Su[2];
     T b = o.Su[0] * o.Sv[0] + o.Su[1] * o.Sv[1] + o.Su[2] * o.Sv[2];

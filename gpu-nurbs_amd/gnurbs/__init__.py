@@ -332,7 +332,7 @@ class Surface:
 
     def Seed(self, x, y, z, exact=True, stream=None):
         """Seed search: exact=True the exact nearest leaf, False the greedy leaf, "step" the greedy
-        leaf plus one Gauss-Newton step (GN_SEED_GREEDY_STEP).  Returns (leaf, u, v)."""
+        descent's leaf plus one Gauss-Newton step (GN_SEED_GREEDY_STEP).  Returns (leaf, u, v)."""
         torch = _torch()
         mode = 2 if exact == "step" else (1 if exact else 0)
         xd, yd, zd = (to_device(a, torch.float64) for a in (x, y, z))

@@ -470,7 +470,7 @@ static int gn_one_in(const or_surf *sf, const double P[3], const or_gn_opts *op,
             av = cv;
             prr = rr;
         }
-        if (!(dist == dist)) { st = OR_NONFINITE; break; }
+        if (!(dist < INFINITY)) { st = OR_NONFINITE; break; } /* NaN or infinite residual */
         if (dist < op->tol_dist) { st = OR_CONV_DIST; break; }
         /* J = [Su Sv]; (J^T J)^-1 J^T (S - P), src/kernel/kernel.cu:195-239 */
         double a = 0, b = 0, c = 0, gu = 0, gv = 0;

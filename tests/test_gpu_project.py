@@ -161,6 +161,43 @@ def test_cone_pruning_on_reference_models(golden_dir, name, cap):
     g.close()
 
 
+def test_global_on_a_collapsed_edge_and_non_finite_queries():
+    """A control-net edge collapsed to one point (S_v = 0 there: its leaves get no bounded cone and
+    are never cone-pruned) against the exhaustive oracle; queries with NaN or infinite coordinates
+    end non-converged without disturbing the others."""
+    sd = synth.surface_G(10, 10, 3, 3, seed=44)
+    cp = sd.cp.copy()
+    cp[0, :, :] = cp[0, 0, :]
+    g = G.Surface(sd.p, sd.q, sd.ku, sd.kv, cp, sd.w)
+    g.BuildTree(3)
+    os_ = O.OSurf(sd.p, sd.q, sd.ku, sd.kv, cp, sd.w)
+    cones = g.GetNormalCones()
+    assert np.all(cones[:64, 3] == 4.0)  # the leaves along the collapsed edge (bx = 0)
+    rng = np.random.default_rng(12)
+    n = 400
+    u0, v0 = rng.random(n), rng.random(n)
+    ev = os_.eval(u0, v0, 0)
+    P = ev + rng.normal(0.0, 0.05, (3, n))
+    P[0, 5], P[1, 17], P[2, 33] = np.nan, np.inf, -np.inf
+    bad = np.zeros(n, dtype=bool)
+    bad[[5, 17, 33]] = True
+    gu, gv, st = (_np(t) for t in g.NewInverse(P[0], P[1], P[2], G.default_opts(**GLOBAL)))
+    assert np.all(st[bad] == G.PT_NONFINITE)
+    # the local modes report them the same way (FP64 and mixed)
+    for opts in (dict(precision=G.GN_FP64, max_iter=20, tol_step=1e-10),
+                 dict(precision=G.GN_MIXED, newton=G.NEWTON_FP32 | G.NEWTON_FP64, tol_step32=1e-5,
+                      fp32_iters=8, tol_quad=1e-10, seed_mode=G.SEED_GREEDY_STEP)):
+        _, _, sl = (_np(t) for t in g.NewInverse(P[0], P[1], P[2], G.default_opts(**opts)))
+        assert np.all(sl[bad] == G.PT_NONFINITE), (opts, sl[bad])
+    nodes, _, _ = os_.build_tree(3)
+    Pg = P[:, ~bad]
+    ou, ov, _ = os_.project_global(nodes, 3, Pg[0].copy(), Pg[1].copy(), Pg[2].copy(), max_iter=20, leaf_iter=8,
+                                   tol_step=1e-12)
+    d, do = _dist(os_, Pg, gu[~bad], gv[~bad]), _dist(os_, Pg, ou, ov)
+    assert np.all(np.abs(d - do) <= 1e-12 + 1e-9 * do), np.max(np.abs(d - do))
+    g.close()
+
+
 def test_global_never_farther_than_local_and_sometimes_nearer(bumpy):
     sd, g, os_, nodes = bumpy
     _, _, P = _queries(os_, 2000, 0.2, 2)
