@@ -149,7 +149,7 @@ __device__ int curve_loop(const GnCurveDev &c, const double P[3], double &t, int
         curve_eval<NEWTON ? 2 : 1, T>(c, t, o);
         const T r0 = o.C[0] - Px, r1 = o.C[1] - Py, r2 = o.C[2] - Pz;
         const T rr = r0 * r0 + r1 * r1 + r2 * r2;
-        if (!(rr == rr)) { st = GN_PT_NONFINITE; break; }
+        if (!(rr < T(INFINITY))) { st = GN_PT_NONFINITE; break; }  // NaN or infinite residual
         if (tol_dist > 0 && sqrt(rr) < T(tol_dist)) { st = GN_PT_CONVERGED_DIST; break; }
         const T f = o.C1[0] * r0 + o.C1[1] * r1 + o.C1[2] * r2;
         const T c1 = o.C1[0] * o.C1[0] + o.C1[1] * o.C1[1] + o.C1[2] * o.C1[2];

@@ -1270,7 +1270,7 @@ void or_curve_invert(int p, int cp, const double *knots, const double *xyz, cons
             curve_eval_one(&c, t, newton ? 2 : 1, o);
             double r[3] = {o[0] - P[0], o[1] - P[1], o[2] - P[2]};
             double rr = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
-            if (!(rr == rr)) { st = OR_NONFINITE; break; }
+            if (!(rr < INFINITY)) { st = OR_NONFINITE; break; } /* NaN or infinite residual */
             if (tol_dist > 0 && sqrt(rr) < tol_dist) { st = OR_CONV_DIST; break; }
             double f = o[3] * r[0] + o[4] * r[1] + o[5] * r[2];
             double fp = o[3] * o[3] + o[4] * o[4] + o[5] * o[5];

@@ -87,6 +87,28 @@ def test_invert_fp64_vs_oracle(newton):
     assert (st == rst).mean() > 0.99
 
 
+def test_non_finite_queries():
+    """Queries with a NaN or infinite coordinate end GN_PT_NONFINITE (FP64 and mixed), as in the
+    oracle, and the others keep their results."""
+    cd = synth.helix_curve(64, 3, 14)
+    g, o = pair(cd)
+    rng = np.random.default_rng(4)
+    ts = rng.uniform(0, 1, 2000)
+    P = o.eval(ts) + rng.normal(0, 0.01, (3, ts.size))
+    P[0, 3], P[1, 10], P[2, 20] = np.nan, np.inf, -np.inf
+    bad = np.zeros(ts.size, dtype=bool)
+    bad[[3, 10, 20]] = True
+    seed = o.seed(*P)
+    rt, rst, _ = o.invert(*P, seed, newton=1)
+    assert np.all(rst[bad] == O.NONFINITE)
+    for opts in (G.default_opts(newton=G.NEWTON_FP64), G.default_opts(precision=G.GN_MIXED, newton=3)):
+        t, st = g.Invert(*P, opts, seeds=seed)
+        t, st = t.cpu().numpy(), st.cpu().numpy()
+        assert np.all(st[bad] == G.PT_NONFINITE), st[bad]
+        both = (st == O.CONV_STEP) & (rst == O.CONV_STEP)
+        assert both[~bad].mean() > 0.97 and np.max(np.abs(t[both] - rt[both])) <= 1e-9
+
+
 @pytest.mark.parametrize("newton", [0, 3])
 def test_mixed_round_trip(newton):
     cd = synth.helix_curve(64, 3, 13)
