@@ -627,7 +627,7 @@ def main():
     g_ms = (time.perf_counter() - g0) * 1e3
     global_res = {"ms": g_ms, "points_per_s": ok_count() / (g_ms / 1e3), "ok_frac": ok_count() / N,
                   "leaves_solved_mean": float(ncand.float().mean().item()),
-                  "note": "GN_SEED_GLOBAL: exact nearest leaf + Newton, then leaf-clamped projected Newton in "
+                  "note": "GN_SEED_GLOBAL: greedy seed leaf + Newton, then leaf-clamped projected Newton in "
                           "every leaf whose box may hold a nearer point and whose normal cone may hold its "
                           "normal (solved in wave-wide rounds), then a polish (FP64); one timed call"}
 

@@ -3,7 +3,8 @@
 // (normal-cone culling, src/kernel/kernel.cu:471-513, 623-704) is unfinished: its test tries the
 // box corners only, for one sign of the normal.  This one is a branch and bound over the seed
 // quadtree whose leaf boxes contain their pieces of surface:
-//   1. bound: exact nearest leaf centre, then projected Newton over the whole domain;
+//   1. bound: the greedy seed leaf (descent + 3 x 3), then projected Newton over the whole domain
+//      (any start gives a valid bound; the exact nearest leaf centre cost a second tree walk);
 //   2. every leaf whose FP32 box (a conservative lower bound) is not farther than the best foot
 //      point so far, and whose normal cone allows a foot point inside it (cone_may_hold; leaves
 //      on the domain boundary, where the nearest point need not be a foot point, always pass),
@@ -144,7 +145,7 @@ __device__ __forceinline__ void project_point(const GnSurfDev &d, const GnSearch
     const int max_iter = a.o.max_iter;
     const double tol_step = a.o.tol_step;
     const double P[3] = {a.x[i], a.y[i], a.z[i]};
-    const uint32_t c0 = nearest_leaf(sd, P);
+    const uint32_t c0 = greedy_leaf(sd, P);
     double bu = sd.seed_uv[2 * c0], bv = sd.seed_uv[2 * c0 + 1], best;
     int bst = solve_in<K1, K2, EXACT>(d, P, bu, bv, full, max_iter, tol_step, best);
     int ncand = 1;

@@ -1051,7 +1051,7 @@ void or_seed_greedy_step(int p, int q, int nu, int nv, const double *ku, const d
 /* Global projection (the library's GN_SEED_GLOBAL; the reference's normal-cone projection,
  * src/kernel/kernel.cu:471-513, 623-704, is unfinished).  Exhaustive form of the library's
  * branch and bound: exact-Hessian projected Newton (a coordinate held on its bound while the
- * descent points outward) from the nearest leaf centre over the whole domain, then
+ * descent points outward) from the greedy seed leaf (or_seed_greedy_leaf) over the whole domain, then
  * from the centre of EVERY leaf with the iterate clamped to that leaf's uv cell (leaf_iter
  * steps), the nearest foot point kept (ties to the earlier candidate), then a full-domain Newton
  * polish from it that is kept unless it is farther by more than rounding (1e-9 relative).  The tree is the reference's (uv cells of
@@ -1074,6 +1074,9 @@ void or_project_global(int p, int q, int nu, int nv, const double *ku, const dou
     long base = or_tree_base(levels - 1), side = 1L << (3 * (levels - 1));
     or_gn_opts full = {0.0, tol_step, max_iter, 1, 1, 1, 0.0},
                leaf = {0.0, tol_step, leaf_iter, 1, 1, 1, 0.0};
+    int *gleaf = malloc(sizeof(int) * (n > 0 ? n : 1));
+    double *gsu = malloc(sizeof(double) * (n > 0 ? n : 1)), *gsv = malloc(sizeof(double) * (n > 0 ? n : 1));
+    greedy_leaf_impl(nodes, levels, x, y, z, n, gleaf, gsu, gsv, nthreads, 1);
 #ifdef _OPENMP
     if (nthreads <= 0) nthreads = omp_get_max_threads();
 #pragma omp parallel for schedule(dynamic, 16) num_threads(nthreads)
@@ -1082,15 +1085,8 @@ void or_project_global(int p, int q, int nu, int nv, const double *ku, const dou
 #endif
     for (long i = 0; i < n; i++) {
         double P[3] = {x[i], y[i], z[i]};
-        /* nearest leaf centre (ties to the lower index) */
-        long bl = 0;
-        double bd = INFINITY;
-        for (long t = 0; t < side * side; t++) {
-            const double *nd = nodes + (base + t) * 13;
-            double dx = P[0] - nd[10], dy = P[1] - nd[11], dz = P[2] - nd[12];
-            double d2 = dx * dx + dy * dy + dz * dz;
-            if (d2 < bd) { bd = d2; bl = t; }
-        }
+        /* the greedy seed leaf (descent + 3 x 3, as or_seed_greedy_leaf) starts the bound solve */
+        long bl = gleaf[i] - base;
         const double *nd0 = nodes + (base + bl) * 13;
         double bu = (nd0[0] + nd0[2]) / 2, bv = (nd0[1] + nd0[3]) / 2;
         int bst = gn_one_in(&sf, P, &full, &bu, &bv, NULL, NULL);
@@ -1111,6 +1107,9 @@ void or_project_global(int p, int q, int nu, int nv, const double *ku, const dou
         v_out[i] = bv;
         status[i] = (unsigned char)bst;
     }
+    free(gleaf);
+    free(gsu);
+    free(gsv);
 }
 
 /* ------------------------------------------------------------------------------------------ */
