@@ -4,6 +4,7 @@
 // src/main.cu:34-43, calls exit()).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -69,6 +70,10 @@ static void free_tree(gn_surface *s) {
     (void)hipFree(s->d_sleaf32);
     (void)hipFree(s->d_sjac32);
     (void)hipFree(s->d_scone32);
+    (void)hipFree(s->d_grid);
+    s->d_grid = nullptr;
+    s->device_bytes -= s->tree_bytes + s->grid_bytes;
+    s->tree_bytes = s->grid_bytes = 0;
     s->d_sleaf32 = nullptr;
     s->d_sjac32 = nullptr;
     s->d_scone32 = nullptr;
@@ -347,7 +352,128 @@ int gn_build_tree(gn_surface *s, int levels, int flags, gn_stream stream) {
                                  st);
     if (rc) { free_tree(s); return rc; }
     s->tree_levels = levels;
+    s->tree_bytes = acc;
     s->device_bytes += acc;
+    return GN_OK;
+}
+
+int gn_build_seed_grid(gn_surface *s, int resolution, gn_stream stream) {
+    if (!s) return fail(GN_EINVAL, "null surface");
+    if (!s->tree_levels) return fail(GN_ESTATE, "tree not built");
+    if (resolution < 0 || resolution > 1024) return fail(GN_EINVAL, "resolution must be 0..1024");
+    hipStream_t st = (hipStream_t)stream;
+    const int D = 3 * (s->tree_levels - 1);
+    const long leaves = 1L << (2 * D);
+    std::vector<float4> lf((size_t)leaves);
+    GN_HIP(hipMemcpyAsync(lf.data(), s->d_sleaf32, sizeof(float4) * leaves, hipMemcpyDeviceToHost, st));
+    GN_HIP(hipStreamSynchronize(st));
+    // geometry: cubic cells of the largest extent of the leaf centres / R, R/16 cells (at least
+    // one) of margin on every side.  Plain FP32 operations, one per statement, so that the
+    // oracle's restatement rounds identically.
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (const float4 &c : lf) {
+        const float e[3] = {c.x, c.y, c.z};
+        for (int k = 0; k < 3; k++) {
+            lo[k] = std::fmin(lo[k], e[k]);
+            hi[k] = std::fmax(hi[k], e[k]);
+        }
+    }
+    float ext = 0.f;
+    for (int k = 0; k < 3; k++) {
+        if (!std::isfinite(lo[k]) || !std::isfinite(hi[k]))
+            return fail(GN_EINVAL, "leaf centres are not finite");
+        const float w = hi[k] - lo[k];
+        ext = std::fmax(ext, w);
+    }
+    if (!(ext > 0.f) || !std::isfinite(ext)) ext = 1.f;
+    int dims[3];
+    float org[3], cell = 0.f;
+    size_t cells = 0;
+    auto shape = [&](int R) {
+        const int pad = std::max(1, R / 16);
+        cell = ext / (float)R;
+        cells = 1;
+        for (int k = 0; k < 3; k++) {
+            const float w = hi[k] - lo[k];
+            const float span = w / cell;
+            const float off = (float)pad * cell;
+            org[k] = lo[k] - off;
+            dims[k] = (int)std::ceil(span) + 2 * pad;
+            cells *= (size_t)dims[k];
+        }
+    };
+    // the build compares every cell with every leaf: at most 2^36 pairs (about a second)
+    const size_t max_pairs = (size_t)1 << 36;
+    if (resolution) {
+        shape(resolution);
+        if (cells > ((size_t)1 << 25) || cells * (size_t)leaves > max_pairs)
+            return fail(GN_EINVAL, "seed grid above 2^25 cells or 2^36 cell-leaf pairs: lower the resolution");
+    } else {
+        int R = std::min(256, 2 << D);
+        shape(R);
+        while (R > 8 && (cells > ((size_t)1 << 25) || cells * (size_t)leaves > max_pairs)) shape(R /= 2);
+        if (cells > ((size_t)1 << 25) || cells * (size_t)leaves > max_pairs)
+            return fail(GN_EINVAL, "seed grid too large for this tree");
+    }
+    const int wide = leaves > 65536 ? 1 : 0;
+    const size_t bytes = cells * (wide ? 4 : 2);
+    GN_HIP(hipStreamSynchronize(st));
+    (void)hipFree(s->d_grid);
+    s->d_grid = nullptr;
+    s->device_bytes -= s->grid_bytes;
+    s->grid_bytes = 0;
+    GN_HIP(hipMalloc(&s->d_grid, bytes));
+    const int rc = gn::launch_seed_grid_build(s->d_sleaf32, D, dims, org, cell, wide, s->d_grid, st);
+    if (rc) {
+        (void)hipFree(s->d_grid);
+        s->d_grid = nullptr;
+        return rc;
+    }
+    s->grid_wide = wide;
+    s->gcell = cell;
+    s->ginv = 1.f / cell;
+    for (int k = 0; k < 3; k++) {
+        s->gdim[k] = dims[k];
+        s->gorg[k] = org[k];
+    }
+    s->grid_bytes = bytes;
+    s->device_bytes += bytes;
+    return GN_OK;
+}
+
+int gn_get_seed_grid(const gn_surface *s, int32_t *leaf, size_t count, int dims[3],
+                     float origin[3], float *cell) {
+    if (!s) return fail(GN_EINVAL, "null surface");
+    if (!s->d_grid) return fail(GN_ESTATE, "seed grid not built");
+    const size_t cells = (size_t)s->gdim[0] * s->gdim[1] * s->gdim[2];
+    for (int k = 0; k < 3; k++) {
+        if (dims) dims[k] = s->gdim[k];
+        if (origin) origin[k] = s->gorg[k];
+    }
+    if (cell) *cell = s->gcell;
+    if (!leaf) return GN_OK;
+    if (count < cells) return fail(GN_EINVAL, "count smaller than the cell count");
+    const int D = 3 * (s->tree_levels - 1);
+    std::vector<uint32_t> m(cells);
+    GN_HIP(hipDeviceSynchronize());
+    if (s->grid_wide) {
+        GN_HIP(hipMemcpy(m.data(), s->d_grid, sizeof(uint32_t) * cells, hipMemcpyDeviceToHost));
+    } else {
+        std::vector<uint16_t> h(cells);
+        GN_HIP(hipMemcpy(h.data(), s->d_grid, sizeof(uint16_t) * cells, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < cells; i++) m[i] = h[i];
+    }
+    // Morton (x bits odd) -> reference index of the finest level
+    long base = 0, g = 1;
+    for (int l = 0; l < s->tree_levels - 1; l++) { base += g * g; g *= 8; }
+    for (size_t i = 0; i < cells; i++) {
+        uint32_t bx = 0, by = 0;
+        for (int b = 0; b < D; b++) {
+            bx |= ((m[i] >> (2 * b + 1)) & 1u) << b;
+            by |= ((m[i] >> (2 * b)) & 1u) << b;
+        }
+        leaf[i] = (int32_t)(base + (long)bx * g + by);
+    }
     return GN_OK;
 }
 
@@ -394,9 +520,11 @@ int gn_seed(const gn_surface *s, int exact, const double *x, const double *y, co
     if (!s) return fail(GN_EINVAL, "null surface");
     if (!s->tree_levels) return fail(GN_ESTATE, "tree not built");
     if (n && (!x || !y || !z || !su || !sv)) return fail(GN_EINVAL, "null buffer");
-    // exact: 0 greedy leaf, 2 greedy leaf + Gauss-Newton step, any other value the exact search
-    return gn::launch_seed(s->search(), exact == 0 ? 1 : exact == 2 ? 2 : 0, x, y, z, n, leaf, su,
-                           sv, (hipStream_t)stream);
+    if (exact == 3 && !s->d_grid) return fail(GN_ESTATE, "seed grid not built (gn_build_seed_grid)");
+    // exact: 0 greedy leaf, 2 greedy leaf + Gauss-Newton step, 3 grid leaf + the step, any other
+    // value the exact search
+    return gn::launch_seed(s->search(), exact == 0 ? 1 : exact == 2 ? 2 : exact == 3 ? 3 : 0, x, y,
+                           z, n, leaf, su, sv, (hipStream_t)stream);
 }
 
 int gn_invert(const gn_surface *s, const double *x, const double *y, const double *z, size_t n,
@@ -427,8 +555,10 @@ int check_invert(const gn_surface *s, const double *x, const double *y, const do
     if (n && (!x || !y || !z || !u || !v || !status)) return fail(GN_EINVAL, "null buffer");
     if (o.precision != GN_FP64 && o.precision != GN_MIXED)
         return fail(GN_EINVAL, "precision must be GN_FP64 or GN_MIXED");
-    if (o.seed_mode < GN_SEED_NEAREST_LEAF || o.seed_mode > GN_SEED_GREEDY_STEP)
+    if (o.seed_mode < GN_SEED_NEAREST_LEAF || o.seed_mode > GN_SEED_GRID_STEP)
         return fail(GN_EINVAL, "bad seed_mode");
+    if (o.seed_mode == GN_SEED_GRID_STEP && !s->d_grid)
+        return fail(GN_ESTATE, "seed grid not built (gn_build_seed_grid)");
     if (o.seed_mode == GN_SEED_GLOBAL && o.precision != GN_FP64)
         return fail(GN_EINVAL, "global projection runs in FP64 (precision GN_FP64)");
     if (o.fallback && !s->tree_levels)

@@ -139,6 +139,8 @@ int check_opts(const gn_invert_opts &o) {
         return fail(GN_EINVAL, "first-hit seeding is a single-surface reference mode");
     if (o.seed_mode == GN_SEED_GLOBAL)
         return fail(GN_EINVAL, "the global projection is a single-surface mode");
+    if (o.seed_mode == GN_SEED_GRID_STEP)
+        return fail(GN_EINVAL, "seed grids serve single-surface inversion (gn_invert) only");
     if (o.seed_mode < GN_SEED_NEAREST_LEAF || o.seed_mode > GN_SEED_GREEDY_STEP)
         return fail(GN_EINVAL, "bad seed_mode");
     if (o.max_iter < 0 || o.fp32_iters < 0 || o.fp64_tail < 0 || o.max_iter > 10000 ||
@@ -253,6 +255,7 @@ int gn_batch_create(gn_surface *const *surfaces, int count, gn_stream stream, gn
         ob.sd.leaf32 = (const float4 *)(base + parts[i][10].off);
         ob.sd.jac32 = (const float4 *)(base + parts[i][11].off);
         ob.sd.cone32 = nullptr;  // normal cones serve the global projection only
+        ob.sd.grid = nullptr;    // seed grids serve single-surface inversion only
         ob.cls = degree_class(s->p, s->q);
         ob.pad_ = 0;
         b->cls[i] = ob.cls;

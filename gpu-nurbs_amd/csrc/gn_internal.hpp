@@ -48,6 +48,14 @@ struct GnSearchDev {
     const float4 *cone32;     // per leaf (Morton order) normal cone: unit axis, half-angle
                               // (radians, rounded up; 4 when unbounded); null: no cone pruning
     double u0, u1, v0, v1;    // the knot domain the leaves tile
+    // Seed grid (GN_SEED_GRID_STEP): per cell of a uniform grid over the leaf centres, the Morton
+    // index of the leaf whose FP32 centre is nearest to the cell centre; uint16 cells when
+    // grid_wide == 0, uint32 otherwise; null: not built.  Cell (i, j, k) is entry
+    // (i * gdim[1] + j) * gdim[2] + k and covers gorg + [i, i + 1) / ginv along x (y, z alike).
+    const void *grid;
+    int grid_wide;
+    int gdim[3];
+    float gorg[3], ginv;
 };
 
 // One object of a batch: the surface and its seed tree (pointers into the batch's pack) and its
@@ -123,7 +131,11 @@ struct gn_surface {
     float4 *d_sleaf32 = nullptr;
     float4 *d_sjac32 = nullptr;
     float4 *d_scone32 = nullptr;
-    size_t device_bytes = 0;
+    // seed grid (gn_build_seed_grid), freed with the tree
+    void *d_grid = nullptr;
+    int grid_wide = 0, gdim[3] = {0, 0, 0};
+    float gorg[3] = {0.f, 0.f, 0.f}, gcell = 0.f, ginv = 0.f;
+    size_t device_bytes = 0, tree_bytes = 0, grid_bytes = 0;
 
     GnSurfDev dev() const {
         GnSurfDev d;
@@ -141,6 +153,13 @@ struct gn_surface {
         s.leaf32 = d_sleaf32;
         s.jac32 = d_sjac32;
         s.cone32 = d_scone32;
+        s.grid = d_grid;
+        s.grid_wide = grid_wide;
+        for (int k = 0; k < 3; k++) {
+            s.gdim[k] = gdim[k];
+            s.gorg[k] = gorg[k];
+        }
+        s.ginv = ginv;
         s.u0 = ku.empty() ? 0.0 : ku[p];
         s.u1 = ku.empty() ? 1.0 : ku[nu];
         s.v0 = kv.empty() ? 0.0 : kv[q];
@@ -186,11 +205,17 @@ int launch_eval_sorted32(const GnSurfDev &d, const float *u, const float *v, siz
                          float *out, uint32_t *perm, hipStream_t st);
 int launch_tree_level(const GnSurfDev &d, int level, int flags, GnTreeNode *tree,
                       uint8_t *valid, uint8_t *divided, hipStream_t st);
+// Seed grid over the leaves' FP32 centres: cell (i, j, k) of dims gets the Morton index of the
+// leaf nearest to its centre org + (i + 1/2, j + 1/2, k + 1/2) * cell (ties to the lower
+// reference index), as uint16 (wide == 0) or uint32.
+int launch_seed_grid_build(const float4 *leaf32, int depth, const int dims[3], const float org[3],
+                           float cell, int wide, void *grid, hipStream_t st);
 int launch_search_build(const GnSurfDev &d, int levels, const GnTreeNode *tree, double *sbox,
                         float *sbox32, double *scentre, double *sseed, int32_t *sref,
                         float4 *sleaf32, float4 *sjac32, float4 *scone32, hipStream_t st);
 // mode 0: exact nearest leaf centre; mode 1: greedy descent + 3x3 leaf neighbourhood;
-// mode 2: mode 1, then one Gauss-Newton step from the leaf's seed with its stored Jacobian
+// mode 2: the greedy descent's leaf, then one Gauss-Newton step from the leaf's seed with its
+// stored Jacobian; mode 3: the seed grid's leaf, then the same step
 int launch_seed(const GnSearchDev &sd, int mode, const double *x, const double *y,
                 const double *z, size_t n, int32_t *leaf, double *su, double *sv,
                 hipStream_t st);

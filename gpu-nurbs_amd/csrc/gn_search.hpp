@@ -172,7 +172,22 @@ __device__ inline uint32_t greedy_leaf(const GnSearchDev &sd, const double P[3],
     return best_c;
 }
 
-// Seed refinement of GN_SEED_GREEDY_STEP: one Gauss-Newton step from leaf c's seed (u, v) with
+// The seed grid's leaf for P (GN_SEED_GRID_STEP): the cell holding P rounded to FP32, clamped
+// into the grid (points outside it take the nearest border cell; NaN coordinates cell 0).
+// (p - org) * inv has no multiply-add to contract, so it rounds twice, as the oracle's does.
+__device__ __forceinline__ uint32_t grid_leaf(const GnSearchDev &sd, const double P[3]) {
+    int id[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const float t = ((float)P[k] - sd.gorg[k]) * sd.ginv;
+        id[k] = (int)fminf(fmaxf(floorf(t), 0.f), (float)(sd.gdim[k] - 1));
+    }
+    const size_t cell = ((size_t)id[0] * sd.gdim[1] + id[1]) * sd.gdim[2] + id[2];
+    return sd.grid_wide ? static_cast<const uint32_t *>(sd.grid)[cell]
+                        : static_cast<const uint16_t *>(sd.grid)[cell];
+}
+
+// Seed refinement of GN_SEED_GREEDY_STEP and GN_SEED_GRID_STEP: one Gauss-Newton step from leaf c's seed (u, v) with
 // the Jacobian J = [S_u S_v] stored for the leaf (evaluated at its seed, whose surface point is
 // the leaf's centre C), so no surface evaluation is needed:
 //   (du, dv) = (J^T J)^-1 J^T (C - P),

@@ -22,7 +22,8 @@ int launch_invert(const GnSurfDev &d, const GnSearchDev &sd, const GnTreeNode *t
     } else {
         if (o.seed_mode != GN_SEED_GIVEN) {
             const int mode = o.seed_mode == GN_SEED_NEAREST_LEAF ? 0
-                           : o.seed_mode == GN_SEED_GREEDY_STEP  ? 2 : 1;
+                           : o.seed_mode == GN_SEED_GREEDY_STEP  ? 2
+                           : o.seed_mode == GN_SEED_GRID_STEP    ? 3 : 1;
             rc = launch_seed(sd, mode, x, y, z, n, nullptr, u, v, st);
             if (rc) return rc;
         }

@@ -415,10 +415,89 @@ __global__ __launch_bounds__(256) void k_seed(GnSearchDev sd, int mode,
     }
 }
 
+// GN_SEED_GRID_STEP: one lane per query, the grid cell's leaf (one 2- or 4-byte gather), then
+// step_seed from it.  The leaf's reference index is the w bits of its FP32 record.
+__global__ __launch_bounds__(256) void k_seed_grid(GnSearchDev sd, const double *__restrict__ x,
+                                                   const double *__restrict__ y,
+                                                   const double *__restrict__ z, size_t n,
+                                                   int32_t *__restrict__ leaf,
+                                                   double *__restrict__ su,
+                                                   double *__restrict__ sv) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x) {
+        const double P[3] = {x[i], y[i], z[i]};
+        const uint32_t c = grid_leaf(sd, P);
+        const float4 cq = sd.leaf32[c];
+        if (leaf) leaf[i] = __float_as_int(cq.w);
+        double u, v;
+        step_seed(sd, c, cq, P, u, v);
+        su[i] = u;
+        sv[i] = v;
+    }
+}
+
+// Seed grid build: one thread per cell against every leaf.  The leaves' FP32 centres pass through
+// LDS in chunks, in reference order (bx * side + by), so a strict "nearer" keeps the lower
+// reference index among equals; the winner is stored as its Morton index.  Distances are the
+// greedy refinement's: fmaf(ez, ez, fmaf(ey, ey, ex * ex)) in FP32.
+constexpr int kGridChunk = 2048;
+
+__global__ __launch_bounds__(256) void k_seed_grid_build(const float4 *__restrict__ leaf32,
+                                                         int depth, int nx, int ny, int nz,
+                                                         float ox, float oy, float oz, float cell,
+                                                         int wide, void *__restrict__ grid) {
+    __shared__ float4 sl[kGridChunk];
+    const long cells = (long)nx * ny * nz, leaves = 1L << (2 * depth);
+    const int side = 1 << depth;
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long cid = idx < cells ? idx : cells - 1;
+    const int k = (int)(cid % nz), j = (int)((cid / nz) % ny), i = (int)(cid / ((long)nz * ny));
+    const float cx = fmaf((float)i + 0.5f, cell, ox), cy = fmaf((float)j + 0.5f, cell, oy),
+                cz = fmaf((float)k + 0.5f, cell, oz);
+    float best = INFINITY;
+    long best_t = 0;
+    for (long t0 = 0; t0 < leaves; t0 += kGridChunk) {
+        const int m = (int)(leaves - t0 < kGridChunk ? leaves - t0 : kGridChunk);
+        __syncthreads();
+        for (int t = threadIdx.x; t < m; t += blockDim.x) {
+            const long r = t0 + t;
+            sl[t] = leaf32[morton2((uint32_t)(r >> depth), (uint32_t)(r & (side - 1)))];
+        }
+        __syncthreads();
+        for (int t = 0; t < m; t++) {
+            const float4 q = sl[t];
+            const float ex = cx - q.x, ey = cy - q.y, ez = cz - q.z;
+            const float dd = fmaf(ez, ez, fmaf(ey, ey, ex * ex));
+            if (dd < best) {
+                best = dd;
+                best_t = t0 + t;
+            }
+        }
+    }
+    if (idx >= cells) return;
+    const uint32_t c = morton2((uint32_t)(best_t >> depth), (uint32_t)(best_t & (side - 1)));
+    if (wide) static_cast<uint32_t *>(grid)[idx] = c;
+    else static_cast<uint16_t *>(grid)[idx] = (uint16_t)c;
+}
+
+int launch_seed_grid_build(const float4 *leaf32, int depth, const int dims[3], const float org[3],
+                           float cell, int wide, void *grid, hipStream_t st) {
+    const long cells = (long)dims[0] * dims[1] * dims[2];
+    k_seed_grid_build<<<(unsigned)((cells + 255) / 256), 256, 0, st>>>(
+        leaf32, depth, dims[0], dims[1], dims[2], org[0], org[1], org[2], cell, wide, grid);
+    GN_HIP(hipGetLastError());
+    return GN_OK;
+}
+
 int launch_seed(const GnSearchDev &sd, int mode, const double *x, const double *y,
                 const double *z, size_t n, int32_t *leaf, double *su, double *sv,
                 hipStream_t st) {
     if (n == 0) return GN_OK;
+    if (mode == 3) {
+        k_seed_grid<<<grid_for(n, 256, 256 * 32), 256, 0, st>>>(sd, x, y, z, n, leaf, su, sv);
+        GN_HIP(hipGetLastError());
+        return GN_OK;
+    }
     // greedy (with or without the step): a few long-lived workgroups per CU, each staging the
     // upper tree levels in LDS once (a workgroup per 256 points would copy 32 KB per 256 points)
     const unsigned cap = mode != 0 ? 256 * GN_SEED_WG_PER_CU : 256 * 32;

@@ -28,6 +28,7 @@ GN_OK, GN_EINVAL, GN_ENOMEM, GN_EDEVICE, GN_EIO, GN_ESTATE, GN_EBUDGET, GN_ENOTF
 GN_FP64, GN_FP32, GN_MIXED = 0, 1, 2
 PT_CONVERGED, PT_CONVERGED_DIST, PT_MAXITER, PT_SINGULAR, PT_NONFINITE, PT_NOSEED = range(6)
 SEED_NEAREST_LEAF, SEED_REF_FIRST_HIT, SEED_GIVEN, SEED_GREEDY_LEAF, SEED_GLOBAL, SEED_GREEDY_STEP = 0, 1, 2, 3, 4, 5
+SEED_GRID_STEP = 6
 TREE_REF_PARENT = 1
 NEWTON_FP32, NEWTON_FP64 = 1, 2
 
@@ -82,6 +83,8 @@ _SIGS = [
     ("gn_build_tree", _c_int, [_c_void, _c_int, _c_int, _c_void]),
     ("gn_get_tree", _c_int, [_c_void, _c_void, _c_void, _c_void, _c_size]),
     ("gn_get_normal_cones", _c_int, [_c_void, _c_void, _c_size]),
+    ("gn_build_seed_grid", _c_int, [_c_void, _c_int, _c_void]),
+    ("gn_get_seed_grid", _c_int, [_c_void, _c_void, _c_size, _c_void, _c_void, _c_void]),
     ("gn_seed", _c_int, [_c_void, _c_int, _c_void, _c_void, _c_void, _c_size, _c_void, _c_void, _c_void, _c_void]),
     ("gn_invert", _c_int, [_c_void, _c_void, _c_void, _c_void, _c_size, ctypes.POINTER(InvertOpts), _c_void, _c_void, _c_void, _c_void]),
     ("gn_invert_noalloc", _c_int, [_c_void, _c_void, _c_void, _c_void, _c_size, ctypes.POINTER(InvertOpts), _c_void, _c_void, _c_void, _c_void]),
@@ -330,11 +333,27 @@ class Surface:
         _check(lib().gn_get_normal_cones(self._h, out.ctypes.data, g * g))
         return out
 
+    def BuildSeedGrid(self, resolution: int = 0, stream=None):
+        """Seed grid for GN_SEED_GRID_STEP (after BuildTree; 0 picks the default resolution)."""
+        _check(lib().gn_build_seed_grid(self._h, resolution, _c_void(stream_handle(stream))))
+
+    def GetSeedGrid(self):
+        """(dims int32[3], origin float32[3], cell size, cells int32[dims] of leaf reference
+        indices), the layout of the oracle's seed_grid."""
+        dims = np.zeros(3, dtype=np.int32)
+        org = np.zeros(3, dtype=np.float32)
+        cell = np.zeros(1, dtype=np.float32)
+        _check(lib().gn_get_seed_grid(self._h, None, 0, dims.ctypes.data, org.ctypes.data, cell.ctypes.data))
+        cells = np.zeros(tuple(int(d) for d in dims), dtype=np.int32)
+        _check(lib().gn_get_seed_grid(self._h, cells.ctypes.data, cells.size, None, None, None))
+        return dims, org, cell[0], cells
+
     def Seed(self, x, y, z, exact=True, stream=None):
         """Seed search: exact=True the exact nearest leaf, False the greedy leaf, "step" the greedy
-        descent's leaf plus one Gauss-Newton step (GN_SEED_GREEDY_STEP).  Returns (leaf, u, v)."""
+        descent's leaf plus one Gauss-Newton step (GN_SEED_GREEDY_STEP), "grid" the seed grid's
+        leaf plus the same step (GN_SEED_GRID_STEP).  Returns (leaf, u, v)."""
         torch = _torch()
-        mode = 2 if exact == "step" else (1 if exact else 0)
+        mode = 2 if exact == "step" else 3 if exact == "grid" else (1 if exact else 0)
         xd, yd, zd = (to_device(a, torch.float64) for a in (x, y, z))
         n = xd.numel()
         leaf = torch.empty(n, dtype=torch.int32, device="cuda")

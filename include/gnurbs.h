@@ -68,11 +68,15 @@ typedef enum {
                                  wins.  iters_fp32 of gn_invert_iters receives the number of
                                  leaves solved.  The reference's normal-cone projection
                                  (src/kernel/kernel.cu:471-513, 623-704) is unfinished. */
-    GN_SEED_GREEDY_STEP = 5   /* the greedy descent's leaf (GN_SEED_GREEDY_LEAF without its 3 x 3
+    GN_SEED_GREEDY_STEP = 5,  /* the greedy descent's leaf (GN_SEED_GREEDY_LEAF without its 3 x 3
                                  neighbour refinement), then one Gauss-Newton step from the
                                  leaf's seed with the Jacobian stored for that leaf (S_u, S_v at
                                  its seed), limited to one leaf cell and clamped to the domain:
                                  no surface evaluation, and about one iteration fewer after it */
+    GN_SEED_GRID_STEP = 6     /* the leaf that gn_build_seed_grid's uniform grid holds for the
+                                 cell of P (one lookup instead of the descent), then the step of
+                                 GN_SEED_GREEDY_STEP from that leaf; single-surface gn_invert
+                                 and gn_seed only */
 } gn_seed_mode;
 
 /* tree build flags */
@@ -191,9 +195,24 @@ int gn_get_tree(const gn_surface *s, double *nodes, uint8_t *valid, uint8_t *div
  * samples enlarged by the tree's K bound applied to the normal's second derivatives.
  * count >= g*g.  Synchronous. */
 int gn_get_normal_cones(const gn_surface *s, float *cones, size_t count);
+/* Seed grid for GN_SEED_GRID_STEP (no reference counterpart: the reference seeds by scanning its
+ * tree, src/kernel/kernel.cu:433-469).  A uniform grid of cubic cells over the bounding box of
+ * the finest leaves' centres (FP32), cell size = its largest extent / resolution, with
+ * max(1, resolution/16) cells of margin on every side; each cell holds the leaf whose FP32 centre
+ * is nearest to the cell's centre (ties to the lower reference index).  The build compares every
+ * cell with every leaf, so a grid may have at most 2^25 cells and 2^36 cell-leaf pairs (about a
+ * second); a larger explicit resolution (at most 1024) is GN_EINVAL.  resolution 0 starts at
+ * 2 * 2^depth capped at 256 (128 for 3 levels) and halves while the grid is too large.  Needs the
+ * tree; gn_build_tree discards the grid.  Cells take 2 bytes (4 beyond 65,536 leaves). */
+int gn_build_seed_grid(gn_surface *s, int resolution, gn_stream stream);
+/* Host copy of the seed grid: dims, origin and cell size (any may be NULL), and, when leaf is not
+ * NULL, the reference index of each cell's leaf at (i * dims[1] + j) * dims[2] + k; count >= the
+ * cell count.  Synchronous. */
+int gn_get_seed_grid(const gn_surface *s, int32_t *leaf, size_t count, int dims[3],
+                     float origin[3], float *cell);
 /* Seed search alone: leaf index (reference numbering) and seed (u,v) per point; exact == 0
- * selects GN_SEED_GREEDY_LEAF, exact == 2 GN_SEED_GREEDY_STEP, any other value
- * GN_SEED_NEAREST_LEAF. */
+ * selects GN_SEED_GREEDY_LEAF, exact == 2 GN_SEED_GREEDY_STEP, exact == 3 GN_SEED_GRID_STEP,
+ * any other value GN_SEED_NEAREST_LEAF. */
 int gn_seed(const gn_surface *s, int exact, const double *x, const double *y, const double *z,
             size_t n, int32_t *leaf, double *seed_u, double *seed_v, gn_stream stream);
 

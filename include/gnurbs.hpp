@@ -135,8 +135,13 @@ class NurbsSurface {
         check(gn_get_normal_cones(s_, c.data(), g * g));
         return c;
     }
-    // seed search alone (exact: 0 greedy leaf, 2 greedy descent + one Gauss-Newton step, other
-    // values the exact nearest leaf) into device leaf indices and seeds
+    // seed grid for GN_SEED_GRID_STEP (after BuildTree; 0: the default resolution)
+    void BuildSeedGrid(int resolution = 0, gn_stream st = nullptr) {
+        check(gn_build_seed_grid(s_, resolution, st));
+    }
+    // seed search alone (exact: 0 greedy leaf, 2 greedy descent + one Gauss-Newton step, 3 seed
+    // grid leaf + the step, other values the exact nearest leaf) into device leaf indices and
+    // seeds
     void Seed(const double *x, const double *y, const double *z, size_t n, int32_t *leaf,
               double *su, double *sv, int exact = 0, gn_stream st = nullptr) const {
         check(gn_seed(s_, exact, x, y, z, n, leaf, su, sv, st));

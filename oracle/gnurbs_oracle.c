@@ -1002,13 +1002,15 @@ void or_seed_greedy_descent(const double *nodes, int levels, const double *x, co
  * FP32 from C, P and J rounded to FP32, scaled down uniformly to at most one leaf cell per
  * direction, applied to the FP64 seed and clamped to the knot domain; a singular J^T J keeps the
  * leaf seed.  Written with the same explicit fmaf steps as the library. */
-void or_seed_greedy_step(int p, int q, int nu, int nv, const double *ku, const double *kv,
-                         const double *MatrixN, const double *nodes, int levels, const double *x,
-                         const double *y, const double *z, long n, int *leaf_out, double *seed_u,
-                         double *seed_v, int nthreads)
+/* The step itself, from the leaves in leaf_out (reference indices) whose uv centres are in
+ * seed_u / seed_v. */
+static void seed_step_from_leaf(const or_surf *sfp, const double *nodes, int levels,
+                                const double *x, const double *y, const double *z, long n,
+                                const int *leaf_out, double *seed_u, double *seed_v, int nthreads)
 {
-    or_surf sf = {p, q, nu, nv, ku, kv, MatrixN};
-    or_seed_greedy_descent(nodes, levels, x, y, z, n, leaf_out, seed_u, seed_v, nthreads);
+    const or_surf sf = *sfp;
+    const double *ku = sf.ku, *kv = sf.kv;
+    const int p = sf.p, q = sf.q, nu = sf.nu, nv = sf.nv;
     double u0 = ku[p], u1 = ku[nu], v0 = kv[q], v1 = kv[nv];
     float cells = (float)(1L << (3 * (levels - 1)));
     float hu = (float)(u1 - u0) / cells, hv = (float)(v1 - v0) / cells;
@@ -1038,6 +1040,129 @@ void or_seed_greedy_step(int p, int q, int nu, int nv, const double *ku, const d
         seed_u[i] = fmin(fmax(seed_u[i] - (double)(du / sc), u0), u1);
         seed_v[i] = fmin(fmax(seed_v[i] - (double)(dv / sc), v0), v1);
     }
+}
+
+void or_seed_greedy_step(int p, int q, int nu, int nv, const double *ku, const double *kv,
+                         const double *MatrixN, const double *nodes, int levels, const double *x,
+                         const double *y, const double *z, long n, int *leaf_out, double *seed_u,
+                         double *seed_v, int nthreads)
+{
+    or_surf sf = {p, q, nu, nv, ku, kv, MatrixN};
+    or_seed_greedy_descent(nodes, levels, x, y, z, n, leaf_out, seed_u, seed_v, nthreads);
+    seed_step_from_leaf(&sf, nodes, levels, x, y, z, n, leaf_out, seed_u, seed_v, nthreads);
+}
+
+/* The library's seed grid (gn_build_seed_grid; no reference counterpart).  Geometry from the
+ * finest leaves' FP32 centres: lo / hi per axis, ext = the largest hi - lo (1 when it is not a
+ * positive finite number), R = resolution, pad = max(1, R / 16), cell = ext / R,
+ * origin = lo - pad * cell, dims = ceil((hi - lo) / cell) + 2 pad; every value a single FP32
+ * operation.  A grid is too big above 2^25 cells or 2^36 cell-leaf pairs.  resolution 0 starts
+ * at 2 * 2^D capped at 256 and halves while too big (down to 8).  Returns 0, or -1 when the
+ * centres are not finite or the grid is too big (the library's GN_EINVAL cases). */
+int or_seed_grid_geometry(const double *nodes, int levels, int resolution, int dims[3],
+                          float org[3], float *cell_out)
+{
+    int D = 3 * (levels - 1);
+    long side = 1L << D, base = or_tree_base(levels - 1);
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (long t = 0; t < side * side; t++)
+        for (int k = 0; k < 3; k++) {
+            float c = (float)nodes[(base + t) * 13 + 10 + k];
+            lo[k] = fminf(lo[k], c);
+            hi[k] = fmaxf(hi[k], c);
+        }
+    float ext = 0.f;
+    for (int k = 0; k < 3; k++) {
+        if (!isfinite(lo[k]) || !isfinite(hi[k])) return -1;
+        float w = hi[k] - lo[k];
+        ext = fmaxf(ext, w);
+    }
+    if (!(ext > 0.f) || !isfinite(ext)) ext = 1.f;
+    long leaves = side * side, max_pairs = 1L << 36, cells = 0;
+    float cell = 0.f;
+    int R = resolution ? resolution : ((2 << D) < 256 ? (2 << D) : 256);
+    for (;;) {
+        int pad = R / 16 > 1 ? R / 16 : 1;
+        cell = ext / (float)R;
+        cells = 1;
+        for (int k = 0; k < 3; k++) {
+            float w = hi[k] - lo[k];
+            float span = w / cell;
+            float off = (float)pad * cell;
+            org[k] = lo[k] - off;
+            dims[k] = (int)ceilf(span) + 2 * pad;
+            cells *= dims[k];
+        }
+        int big = cells > (1L << 25) || cells * leaves > max_pairs;
+        if (!big) break;
+        if (resolution || R <= 8) return -1;
+        R /= 2;
+    }
+    (void)base;
+    *cell_out = cell;
+    return 0;
+}
+
+/* Each cell's leaf: the reference index t of least FP32 squared distance
+ * fmaf(ez, ez, fmaf(ey, ey, ex * ex)) from the cell centre org + (i + 1/2) * cell (one fmaf per
+ * axis) to the leaf's FP32 centre, the lowest t among equals.  grid[(i * dims[1] + j) * dims[2] + k]. */
+void or_seed_grid_build(const double *nodes, int levels, const int dims[3], const float org[3],
+                        float cell, int *grid, int nthreads)
+{
+    int D = 3 * (levels - 1);
+    long side = 1L << D, base = or_tree_base(levels - 1), leaves = side * side;
+    long cells = (long)dims[0] * dims[1] * dims[2];
+    float *c32 = malloc(sizeof(float) * 3 * leaves);
+    for (long t = 0; t < leaves; t++)
+        for (int k = 0; k < 3; k++) c32[t * 3 + k] = (float)nodes[(base + t) * 13 + 10 + k];
+#ifdef _OPENMP
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel for schedule(static) num_threads(nthreads)
+#else
+    (void)nthreads;
+#endif
+    for (long id = 0; id < cells; id++) {
+        int k = (int)(id % dims[2]), j = (int)((id / dims[2]) % dims[1]),
+            i = (int)(id / ((long)dims[2] * dims[1]));
+        float cx = fmaf((float)i + 0.5f, cell, org[0]), cy = fmaf((float)j + 0.5f, cell, org[1]),
+              cz = fmaf((float)k + 0.5f, cell, org[2]);
+        float best = INFINITY;
+        long bt = 0;
+        for (long t = 0; t < leaves; t++) {
+            float ex = cx - c32[t * 3], ey = cy - c32[t * 3 + 1], ez = cz - c32[t * 3 + 2];
+            float dd = fmaf(ez, ez, fmaf(ey, ey, ex * ex));
+            if (dd < best) { best = dd; bt = t; }
+        }
+        grid[id] = (int)(base + bt);
+    }
+    free(c32);
+}
+
+/* GN_SEED_GRID_STEP: the cell of P rounded to FP32, index floor(((float)P - org) * (1 / cell))
+ * clamped to [0, dims - 1] (fmaxf takes a NaN to cell 0), its leaf, then the step of
+ * or_seed_greedy_step from that leaf's uv centre. */
+void or_seed_grid_step(int p, int q, int nu, int nv, const double *ku, const double *kv,
+                       const double *MatrixN, const double *nodes, int levels, const int dims[3],
+                       const float org[3], float cell, const int *grid, const double *x,
+                       const double *y, const double *z, long n, int *leaf_out, double *seed_u,
+                       double *seed_v, int nthreads)
+{
+    or_surf sf = {p, q, nu, nv, ku, kv, MatrixN};
+    float inv = 1.f / cell;
+    for (long i = 0; i < n; i++) {
+        const double P[3] = {x[i], y[i], z[i]};
+        int id[3];
+        for (int k = 0; k < 3; k++) {
+            float t = ((float)P[k] - org[k]) * inv;
+            id[k] = (int)fminf(fmaxf(floorf(t), 0.f), (float)(dims[k] - 1));
+        }
+        int t = grid[((long)id[0] * dims[1] + id[1]) * dims[2] + id[2]];
+        leaf_out[i] = t;
+        const double *nd = nodes + (long)t * 13;
+        seed_u[i] = (nd[0] + nd[2]) / 2;
+        seed_v[i] = (nd[1] + nd[3]) / 2;
+    }
+    seed_step_from_leaf(&sf, nodes, levels, x, y, z, n, leaf_out, seed_u, seed_v, nthreads);
 }
 
 /* ------------------------------------------------------------------------------------------ */

@@ -57,6 +57,11 @@ def lib():
         L.or_seed_greedy_leaf.argtypes = [_P, _I, _P, _P, _P, _L, _P, _P, _P, _I]
         L.or_seed_greedy_descent.argtypes = [_P, _I, _P, _P, _P, _L, _P, _P, _P, _I]
         L.or_seed_greedy_step.argtypes = [_I, _I, _I, _I, _P, _P, _P, _P, _I, _P, _P, _P, _L, _P, _P, _P, _I]
+        L.or_seed_grid_geometry.restype = _I
+        L.or_seed_grid_geometry.argtypes = [_P, _I, _I, _P, _P, _P]
+        L.or_seed_grid_build.argtypes = [_P, _I, _P, _P, ctypes.c_float, _P, _I]
+        L.or_seed_grid_step.argtypes = [_I, _I, _I, _I, _P, _P, _P, _P, _I, _P, _P, ctypes.c_float, _P,
+                                        _P, _P, _P, _L, _P, _P, _P, _I]
         L.or_curve_eval.argtypes = [_I, _I, _P, _P, _P, _P, _L, _I, _P, _I]
         L.or_curve_seed.argtypes = [_I, _I, _P, _P, _P, _P, _P, _P, _L, _P]
         L.or_curve_invert.argtypes = [_I, _I, _P, _P, _P, _P, _P, _P, _L, _D, _D, _I, _I, _I, _P, _P, _P,
@@ -165,6 +170,21 @@ class OSurf:
                                   z.ctypes.data, n, leaf.ctypes.data, su.ctypes.data, sv.ctypes.data, nthreads)
         return leaf, su, sv
 
+    def seed_grid_step(self, nodes, levels, grid, x, y, z, nthreads=0):
+        """The library's GN_SEED_GRID_STEP: the leaf of P's cell in `grid` (seed_grid's result),
+        then the step of seed_greedy_step from it.  Returns (leaf, su, sv)."""
+        dims, org, cell, cells = grid
+        nodes = _a(nodes)
+        x, y, z = (_a(a) for a in (x, y, z))
+        n = x.size
+        leaf = np.zeros(n, dtype=np.int32)
+        su, sv = np.zeros(n), np.zeros(n)
+        d, o, c = _a(dims, np.int32), _a(org, np.float32), _a(cells.ravel(), np.int32)
+        lib().or_seed_grid_step(*self._args(), nodes.ctypes.data, levels, d.ctypes.data, o.ctypes.data,
+                                float(cell), c.ctypes.data, x.ctypes.data, y.ctypes.data, z.ctypes.data,
+                                n, leaf.ctypes.data, su.ctypes.data, sv.ctypes.data, nthreads)
+        return leaf, su, sv
+
     def normal_ders(self, u, v):
         """(n, 4, 3): N = S_u x S_v, N_uu, N_uv, N_vv at (u, v)."""
         u, v = _a(u), _a(v)
@@ -232,6 +252,25 @@ def seed_greedy_leaf(nodes, levels, x, y, z, nthreads=0):
     lib().or_seed_greedy_leaf(nodes.ctypes.data, levels, x.ctypes.data, y.ctypes.data, z.ctypes.data, n,
                               leaf.ctypes.data, su.ctypes.data, sv.ctypes.data, nthreads)
     return leaf, su, sv
+
+
+def seed_grid(nodes, levels, resolution=0, nthreads=0, build=True):
+    """The library's seed grid (gn_build_seed_grid): (dims int32[3], origin float32[3], cell size
+    (numpy float32), cells int32[dims] of leaf reference indices; None when build is False), or
+    None where the library rejects the grid."""
+    nodes = _a(nodes)
+    dims = np.zeros(3, dtype=np.int32)
+    org = np.zeros(3, dtype=np.float32)
+    cell = np.zeros(1, dtype=np.float32)
+    if lib().or_seed_grid_geometry(nodes.ctypes.data, levels, resolution, dims.ctypes.data, org.ctypes.data,
+                                   cell.ctypes.data) != 0:
+        return None
+    if not build:
+        return dims, org, cell[0], None
+    cells = np.zeros(tuple(int(d) for d in dims), dtype=np.int32)
+    lib().or_seed_grid_build(nodes.ctypes.data, levels, dims.ctypes.data, org.ctypes.data, float(cell[0]),
+                             cells.ctypes.data, nthreads)
+    return dims, org, cell[0], cells
 
 
 def seed_greedy_descent(nodes, levels, x, y, z, nthreads=0):
