@@ -25,8 +25,8 @@ PEAK_F32 = 157.3e12
 PEAK_F64 = 78.6e12
 
 
-# The mixed form's options: the greedy descent's leaf + one Gauss-Newton step with the leaf's stored
-# Jacobian as the seed; FP32 exact-Hessian Newton iterations to a 3e-5 step, at most 8 (the few points
+# The mixed form's options: the seed grid's leaf (one lookup; gn_build_seed_grid at set-up, like the
+# tree) + one Gauss-Newton step with the leaf's stored Jacobian as the seed; FP32 exact-Hessian Newton iterations to a 3e-5 step, at most 8 (the few points
 # that need more are the long dependent chains every wave of the persistent kernel would wait for: a
 # cap of 20 cost 0.236 ms per 1M-point step against 0.205 ms, for 0.25% more points within 1e-9; a
 # 3e-5 step against 1e-5: 5.31 against 5.20 G/s in interleaved runs, the same points within 1e-9);
@@ -34,7 +34,7 @@ PEAK_F64 = 78.6e12
 # error by 1e-10, or a 1e-10 step.  At most 11 iterations in all (the workload allows 20).
 def mixed_options(G):
     return dict(precision=G.GN_MIXED, newton=G.NEWTON_FP32 | G.NEWTON_FP64, fp32_iters=8, fp64_tail=3,
-                tol_step32=3e-5, tol_step=1e-10, tol_quad=1e-10, seed_mode=G.SEED_GREEDY_STEP)
+                tol_step32=3e-5, tol_step=1e-10, tol_quad=1e-10, seed_mode=G.SEED_GRID_STEP)
 
 
 def contract_flops_per_iter(k1, k2):
@@ -493,7 +493,12 @@ def main():
     N = args.points
     sd = synth.surface_G(32, 32, 3, 3, seed=1002)
     gs = G.Surface(sd.p, sd.q, sd.ku, sd.kv, sd.cp, sd.w)
+    torch.cuda.synchronize()
+    t_setup = time.perf_counter()
     gs.BuildTree(3)
+    gs.BuildSeedGrid(0)  # set-up of the surface, not part of a step
+    torch.cuda.synchronize()
+    setup_ms = (time.perf_counter() - t_setup) * 1e3
     u0, v0 = synth.uniform_uv(N, seed=1003)
     u0_t = torch.from_numpy(u0).to(dev)
     v0_t = torch.from_numpy(v0).to(dev)
@@ -519,10 +524,12 @@ def main():
         # the same reference iteration from the mixed mode's stepped seeds, to separate the seed's
         # share of the gain
         "plain_fp64_step_seeds": G.default_opts(precision=G.GN_FP64, newton=0, max_iter=20, tol_step=1e-10,
-                                                seed_mode=G.SEED_GREEDY_STEP),
+                                                seed_mode=G.SEED_GRID_STEP),
         # the mixed mode's algorithm and seeds run entirely in FP64, to separate algorithm from precision
         "fp64_newton": G.default_opts(precision=G.GN_FP64, newton=G.NEWTON_FP64, max_iter=20, tol_step=1e-10,
-                                      tol_quad=1e-10, seed_mode=G.SEED_GREEDY_STEP),
+                                      tol_quad=1e-10, seed_mode=G.SEED_GRID_STEP),
+        # the mixed mode seeded by the greedy descent (no seed grid), as the batched path seeds
+        "mixed_greedy_step_seeds": G.default_opts(**dict(mixed_opts, seed_mode=G.SEED_GREEDY_STEP)),
     }
 
     def run(opts):
@@ -552,7 +559,7 @@ def main():
 
     # per-kernel timing of the dominant kernel (mixed Gauss-Newton from given seeds) and the seed
     # search, both with HIP events on the launch stream
-    leaf, su_t, sv_t = gs.Seed(x, y, z, exact="step")
+    leaf, su_t, sv_t = gs.Seed(x, y, z, exact="grid")
     torch.cuda.synchronize()
     mixed_given = G.default_opts(**mixed_opts)
     reps = max(3, args.steps)
@@ -578,7 +585,7 @@ def main():
 
     copy_ms = timed(lambda: (u.copy_(su_t), v.copy_(sv_t)))
     gn_ms = timed(gn_only) - copy_ms
-    seed_ms = timed(lambda: gs.Seed(x, y, z, exact="step"))
+    seed_ms = timed(lambda: gs.Seed(x, y, z, exact="grid"))
     _, _, _, it32, it64 = gs.NewInverse(x, y, z, G.default_opts(**{k: getattr(mixed_given, k) for k, _ in
                                                                    G.InvertOpts._fields_}),
                                         seeds=(su_t, sv_t), iters=True)
@@ -680,8 +687,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": "projection: bicubic rational surface, 32x32 net (generator G, seed 1002), "
                                f"{N} queries offset N(0,1e-2) along the normal, depth-6 tree (64x64 leaves) "
-                               "greedy-descent seeds + one Gauss-Newton step (leaf Jacobian), <=8 FP32 + <=3 FP64 "
-                               "iterations",
+                               "and seed grid, seed-grid leaf + one Gauss-Newton step (leaf Jacobian), <=8 FP32 + "
+                               "<=3 FP64 iterations",
                    "points": N, "surface": "p=q=3, 32x32", "tree_depth": 6},
         "roofline": roof,
         "cpu_baseline": cpu,
@@ -689,6 +696,8 @@ def main():
         "plain_fp64": res["plain_fp64"],
         "fp64_newton": res["fp64_newton"],
         "plain_fp64_step_seeds": res["plain_fp64_step_seeds"],
+        "mixed_greedy_step_seeds": res["mixed_greedy_step_seeds"],
+        "surface_setup_ms": setup_ms,
         "speedup_vs_plain_fp64": mixed["points_per_s"] / max(res["plain_fp64"]["points_per_s"], 1e-30),
         "seed_ms": seed_ms,
         "global_projection": global_res,

@@ -68,14 +68,14 @@ static void free_tree(gn_surface *s) {
     (void)hipFree(s->d_sbox32);
     (void)hipFree(s->d_sref);
     (void)hipFree(s->d_sleaf32);
-    (void)hipFree(s->d_sjac32);
+    (void)hipFree(s->d_srec);
     (void)hipFree(s->d_scone32);
     (void)hipFree(s->d_grid);
     s->d_grid = nullptr;
     s->device_bytes -= s->tree_bytes + s->grid_bytes;
     s->tree_bytes = s->grid_bytes = 0;
     s->d_sleaf32 = nullptr;
-    s->d_sjac32 = nullptr;
+    s->d_srec = nullptr;
     s->d_scone32 = nullptr;
     s->d_tree = nullptr;
     s->d_valid = s->d_divided = nullptr;
@@ -337,7 +337,7 @@ int gn_build_tree(gn_surface *s, int levels, int flags, gn_stream stream) {
         (rc = dalloc(&s->d_sseed, (size_t)leaves * 2, acc)) ||
         (rc = dalloc(&s->d_sref, (size_t)leaves, acc)) ||
         (rc = dalloc(&s->d_sleaf32, (size_t)leaves, acc)) ||
-        (rc = dalloc(&s->d_sjac32, (size_t)leaves * 2, acc)) ||
+        (rc = dalloc(&s->d_srec, (size_t)leaves * 4, acc)) ||
         (rc = dalloc(&s->d_scone32, (size_t)leaves, acc))) {
         free_tree(s);
         return rc;
@@ -348,7 +348,7 @@ int gn_build_tree(gn_surface *s, int levels, int flags, gn_stream stream) {
         if (rc) { free_tree(s); return rc; }
     }
     rc = gn::launch_search_build(d, levels, s->d_tree, s->d_sbox, s->d_sbox32, s->d_scentre,
-                                 s->d_sseed, s->d_sref, s->d_sleaf32, s->d_sjac32, s->d_scone32,
+                                 s->d_sseed, s->d_sref, s->d_sleaf32, s->d_srec, s->d_scone32,
                                  st);
     if (rc) { free_tree(s); return rc; }
     s->tree_levels = levels;

@@ -208,7 +208,7 @@ int gn_batch_create(gn_surface *const *surfaces, int count, gn_stream stream, gn
             {s->d_sseed, sizeof(double) * leaves * 2, 0},
             {s->d_sref, sizeof(int32_t) * leaves, 0},
             {s->d_sleaf32, sizeof(float4) * leaves, 0},
-            {s->d_sjac32, sizeof(float4) * leaves * 2, 0},
+            {s->d_srec, sizeof(float4) * leaves * 4, 0},
         };
         for (Part p : ps) {
             p.off = total;
@@ -253,7 +253,7 @@ int gn_batch_create(gn_surface *const *surfaces, int count, gn_stream stream, gn
         ob.sd.seed_uv = (const double *)(base + parts[i][8].off);
         ob.sd.ref_index = (const int32_t *)(base + parts[i][9].off);
         ob.sd.leaf32 = (const float4 *)(base + parts[i][10].off);
-        ob.sd.jac32 = (const float4 *)(base + parts[i][11].off);
+        ob.sd.rec = (const float4 *)(base + parts[i][11].off);
         ob.sd.cone32 = nullptr;  // normal cones serve the global projection only
         ob.sd.grid = nullptr;    // seed grids serve single-surface inversion only
         ob.cls = degree_class(s->p, s->q);

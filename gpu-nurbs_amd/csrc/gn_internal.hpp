@@ -43,8 +43,9 @@ struct GnSearchDev {
     const int32_t *ref_index; // per leaf (Morton order) reference tree index
     const float4 *leaf32;     // per leaf (Morton order) centre in FP32 and the reference index
                               // (bits of w), one 16-byte load for the greedy 3 x 3 search
-    const float4 *jac32;      // per leaf (Morton order) S_u, S_v at the leaf's seed (u, v) in
-                              // FP32, two 16-byte records (x, y, z, 0)
+    const float4 *rec;        // per leaf (Morton order) one 64-byte record for the stepped
+                              // seeds, one cache line fetch: [0] as leaf32, [1] the seed (u, v)
+                              // as two doubles, [2] S_u and [3] S_v at the seed in FP32 (w = 0)
     const float4 *cone32;     // per leaf (Morton order) normal cone: unit axis, half-angle
                               // (radians, rounded up; 4 when unbounded); null: no cone pruning
     double u0, u1, v0, v1;    // the knot domain the leaves tile
@@ -129,7 +130,7 @@ struct gn_surface {
     float *d_sbox32 = nullptr;
     int32_t *d_sref = nullptr;
     float4 *d_sleaf32 = nullptr;
-    float4 *d_sjac32 = nullptr;
+    float4 *d_srec = nullptr;
     float4 *d_scone32 = nullptr;
     // seed grid (gn_build_seed_grid), freed with the tree
     void *d_grid = nullptr;
@@ -151,7 +152,7 @@ struct gn_surface {
         s.box = d_sbox; s.box32 = d_sbox32; s.centre = d_scentre; s.seed_uv = d_sseed;
         s.ref_index = d_sref;
         s.leaf32 = d_sleaf32;
-        s.jac32 = d_sjac32;
+        s.rec = d_srec;
         s.cone32 = d_scone32;
         s.grid = d_grid;
         s.grid_wide = grid_wide;
@@ -212,7 +213,7 @@ int launch_seed_grid_build(const float4 *leaf32, int depth, const int dims[3], c
                            float cell, int wide, void *grid, hipStream_t st);
 int launch_search_build(const GnSurfDev &d, int levels, const GnTreeNode *tree, double *sbox,
                         float *sbox32, double *scentre, double *sseed, int32_t *sref,
-                        float4 *sleaf32, float4 *sjac32, float4 *scone32, hipStream_t st);
+                        float4 *sleaf32, float4 *srec, float4 *scone32, hipStream_t st);
 // mode 0: exact nearest leaf centre; mode 1: greedy descent + 3x3 leaf neighbourhood;
 // mode 2: the greedy descent's leaf, then one Gauss-Newton step from the leaf's seed with its
 // stored Jacobian; mode 3: the seed grid's leaf, then the same step

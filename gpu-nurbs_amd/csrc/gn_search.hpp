@@ -135,7 +135,7 @@ __device__ inline uint32_t greedy_leaf(const GnSearchDev &sd, const double P[3],
     for (; l < lds_levels && l < D; l++) c = greedy_child(lds + (qoff(l + 1) - 1) * 6, c, p);
     for (; l < D; l++) c = greedy_child(sd.box32 + qoff(l + 1) * 6, c, p);
     if (!refine) {
-        if (cbest) *cbest = sd.leaf32[c];
+        if (cbest) *cbest = sd.rec[(size_t)c * 4];  // the leaf32 entry, on the step's cache line
         return c;
     }
     const int side = 1 << D;
@@ -175,16 +175,22 @@ __device__ inline uint32_t greedy_leaf(const GnSearchDev &sd, const double P[3],
 // The seed grid's leaf for P (GN_SEED_GRID_STEP): the cell holding P rounded to FP32, clamped
 // into the grid (points outside it take the nearest border cell; NaN coordinates cell 0).
 // (p - org) * inv has no multiply-add to contract, so it rounds twice, as the oracle's does.
-__device__ __forceinline__ uint32_t grid_leaf(const GnSearchDev &sd, const double P[3]) {
+__device__ __forceinline__ uint32_t grid_cell(const GnSearchDev &sd, const double P[3]) {
     int id[3];
 #pragma unroll
     for (int k = 0; k < 3; k++) {
         const float t = ((float)P[k] - sd.gorg[k]) * sd.ginv;
         id[k] = (int)fminf(fmaxf(floorf(t), 0.f), (float)(sd.gdim[k] - 1));
     }
-    const size_t cell = ((size_t)id[0] * sd.gdim[1] + id[1]) * sd.gdim[2] + id[2];
-    return sd.grid_wide ? static_cast<const uint32_t *>(sd.grid)[cell]
-                        : static_cast<const uint16_t *>(sd.grid)[cell];
+    return ((uint32_t)id[0] * (uint32_t)sd.gdim[1] + (uint32_t)id[1]) * (uint32_t)sd.gdim[2] +
+           (uint32_t)id[2];  // below 2^25 cells
+}
+
+template <bool WIDE>
+__device__ __forceinline__ uint32_t grid_leaf(const GnSearchDev &sd, const double P[3]) {
+    const uint32_t cell = grid_cell(sd, P);
+    return WIDE ? static_cast<const uint32_t *>(sd.grid)[cell]
+                : static_cast<const uint16_t *>(sd.grid)[cell];
 }
 
 // Seed refinement of GN_SEED_GREEDY_STEP and GN_SEED_GRID_STEP: one Gauss-Newton step from leaf c's seed (u, v) with
@@ -196,11 +202,10 @@ __device__ __forceinline__ uint32_t grid_leaf(const GnSearchDev &sd, const doubl
 // seed.  C is the leaf's FP32 centre (the one the greedy search compared), P the query rounded to
 // FP32.  Every multiply-add is an explicit fmaf and the divisions are IEEE, so the oracle can
 // restate the step operation for operation.
-__device__ __forceinline__ void step_seed(const GnSearchDev &sd, uint32_t c, const float4 &C,
-                                          const double P[3], double &su, double &sv) {
-    su = sd.seed_uv[c * 2 + 0];
-    sv = sd.seed_uv[c * 2 + 1];
-    const float4 ju = sd.jac32[c * 2], jv = sd.jac32[c * 2 + 1];
+// step_seed_from: the same step from the leaf's records already loaded (seed (u, v) in su, sv).
+__device__ __forceinline__ void step_seed_from(const GnSearchDev &sd, const float4 &C,
+                                               const float4 &ju, const float4 &jv,
+                                               const double P[3], double &su, double &sv) {
     const float r0 = C.x - (float)P[0], r1 = C.y - (float)P[1], r2 = C.z - (float)P[2];
     const float a = fmaf(ju.z, ju.z, fmaf(ju.y, ju.y, ju.x * ju.x));
     const float b = fmaf(ju.z, jv.z, fmaf(ju.y, jv.y, ju.x * jv.x));
@@ -208,15 +213,27 @@ __device__ __forceinline__ void step_seed(const GnSearchDev &sd, uint32_t c, con
     const float gu = fmaf(ju.z, r2, fmaf(ju.y, r1, ju.x * r0));
     const float gv = fmaf(jv.z, r2, fmaf(jv.y, r1, jv.x * r0));
     const float det = fmaf(a, e, -(b * b));
-    if (!(det > 0.f)) return;
+    // computed whatever det is and kept only when det > 0 and the scale is finite: no branch,
+    // so the compiler cannot sink the leaf-record loads behind one
     const float du = fmaf(e, gu, -(b * gv)) / det;
     const float dv = fmaf(a, gv, -(b * gu)) / det;
     const float cells = (float)(1u << sd.depth);
     const float hu = (float)(sd.u1 - sd.u0) / cells, hv = (float)(sd.v1 - sd.v0) / cells;
     const float sc = fmaxf(1.f, fmaxf(fabsf(du) / hu, fabsf(dv) / hv));
-    if (!(sc < INFINITY)) return;
-    su = fmin(fmax(su - (double)(du / sc), sd.u0), sd.u1);
-    sv = fmin(fmax(sv - (double)(dv / sc), sd.v0), sd.v1);
+    const bool ok = det > 0.f && sc < INFINITY;
+    const double nu = fmin(fmax(su - (double)(du / sc), sd.u0), sd.u1);
+    const double nv = fmin(fmax(sv - (double)(dv / sc), sd.v0), sd.v1);
+    su = ok ? nu : su;
+    sv = ok ? nv : sv;
+}
+
+__device__ __forceinline__ void step_seed(const GnSearchDev &sd, uint32_t c, const float4 &C,
+                                          const double P[3], double &su, double &sv) {
+    const float4 *r = sd.rec + (size_t)c * 4;
+    const double2 uv = *reinterpret_cast<const double2 *>(r + 1);
+    su = uv.x;
+    sv = uv.y;
+    step_seed_from(sd, C, r[2], r[3], P, su, sv);
 }
 
 }  // namespace gn
