@@ -32,9 +32,11 @@ PEAK_F64 = 78.6e12
 # 3e-5 step against 1e-5: 5.31 against 5.20 G/s in interleaved runs, the same points within 1e-9);
 # then at most 3 FP64 exact-Hessian closing steps until the Newton certificate bounds the (u,v)
 # error by 1e-10, or a 1e-10 step.  At most 11 iterations in all (the workload allows 20).
-def mixed_options(G):
+# Batches and the server seed by the greedy descent instead (seed grids are single-surface).
+def mixed_options(G, batched=False):
     return dict(precision=G.GN_MIXED, newton=G.NEWTON_FP32 | G.NEWTON_FP64, fp32_iters=8, fp64_tail=3,
-                tol_step32=3e-5, tol_step=1e-10, tol_quad=1e-10, seed_mode=G.SEED_GRID_STEP)
+                tol_step32=3e-5, tol_step=1e-10, tol_quad=1e-10,
+                seed_mode=G.SEED_GREEDY_STEP if batched else G.SEED_GRID_STEP)
 
 
 def contract_flops_per_iter(k1, k2):
@@ -252,7 +254,7 @@ def bench_batch(G, synth, torch, steps, count=1024):
     u = torch.empty(n, dtype=torch.float64, device="cuda")
     v = torch.empty_like(u)
     st = torch.empty(n, dtype=torch.uint8, device="cuda")
-    mixed = mixed_options(G)
+    mixed = mixed_options(G, batched=True)
     shape = b.tune(segs, x, y, z, G.default_opts(**mixed), out=(u, v, st))
     stream = torch.cuda.Stream()
     res = {"workload": f"{count} surfaces (degrees 2..5, nets 4..48, jittered knots), {n} points "
@@ -397,7 +399,7 @@ def bench_serve(G, synth, torch, duration=1.0, loads=(0.5, 0.8), budget=32 << 30
                 "mean_step_ms": float(dur.mean() * 1e3), "fixed_ms": float(a * 1e3),
                 "in_step_points_per_s": float(1.0 / b) if b > 0 else None}
 
-    mixed = mixed_options(G)
+    mixed = mixed_options(G, batched=True)
     plain = dict(precision=G.GN_FP64, max_iter=20, tol_step=1e-10)
     try:
         for name, opts in (("mixed", mixed), ("plain_fp64", plain)):
