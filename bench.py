@@ -32,11 +32,14 @@ PEAK_F64 = 78.6e12
 # 3e-5 step against 1e-5: 5.31 against 5.20 G/s in interleaved runs, the same points within 1e-9);
 # then at most 3 FP64 exact-Hessian closing steps until the Newton certificate bounds the (u,v)
 # error by 1e-10, or a 1e-10 step.  At most 11 iterations in all (the workload allows 20).
-# Batches and the server seed by the greedy descent instead (seed grids are single-surface).
-def mixed_options(G, batched=False):
+# The batch and serving objects carry 64-cell seed grids (0.4 MB each; 128 cells, the default, would
+# take 3.4 MB and 1.5 ms to build per object).
+def mixed_options(G):
     return dict(precision=G.GN_MIXED, newton=G.NEWTON_FP32 | G.NEWTON_FP64, fp32_iters=8, fp64_tail=3,
-                tol_step32=3e-5, tol_step=1e-10, tol_quad=1e-10,
-                seed_mode=G.SEED_GREEDY_STEP if batched else G.SEED_GRID_STEP)
+                tol_step32=3e-5, tol_step=1e-10, tol_quad=1e-10, seed_mode=G.SEED_GRID_STEP)
+
+
+OBJECT_GRID_RES = 64
 
 
 def contract_flops_per_iter(k1, k2):
@@ -234,6 +237,7 @@ def bench_batch(G, synth, torch, steps, count=1024):
     for sd, c in zip(defs, counts):
         s = G.Surface(sd.p, sd.q, sd.ku, sd.kv, sd.cp, sd.w)
         s.BuildTree(3)
+        s.BuildSeedGrid(OBJECT_GRID_RES)
         surfs.append(s)
         u0 = torch.rand(int(c), dtype=torch.float64, device="cuda", generator=gen)
         v0 = torch.rand(int(c), dtype=torch.float64, device="cuda", generator=gen)
@@ -254,11 +258,12 @@ def bench_batch(G, synth, torch, steps, count=1024):
     u = torch.empty(n, dtype=torch.float64, device="cuda")
     v = torch.empty_like(u)
     st = torch.empty(n, dtype=torch.uint8, device="cuda")
-    mixed = mixed_options(G, batched=True)
+    mixed = mixed_options(G)
     shape = b.tune(segs, x, y, z, G.default_opts(**mixed), out=(u, v, st))
     stream = torch.cuda.Stream()
     res = {"workload": f"{count} surfaces (degrees 2..5, nets 4..48, jittered knots), {n} points "
-                       "(power law 256..65536 per surface), offsets N(0,1e-2)",
+                       "(power law 256..65536 per surface), offsets N(0,1e-2); depth-6 trees and 64-cell seed "
+                       "grids",
            "points": n, "tile_points": shape[0], "blocks_per_cu": shape[1],
            "batch_device_bytes": b.device_bytes()}
 
@@ -315,6 +320,7 @@ def bench_serve(G, synth, torch, duration=1.0, loads=(0.5, 0.8), budget=32 << 30
     for sd in defs:
         s = G.Surface(sd.p, sd.q, sd.ku, sd.kv, sd.cp, sd.w)
         s.BuildTree(3)
+        s.BuildSeedGrid(OBJECT_GRID_RES)
         surfs.append(s)
         u0 = torch.rand(pool, dtype=torch.float64, device="cuda", generator=gen)
         v0 = torch.rand(pool, dtype=torch.float64, device="cuda", generator=gen)
@@ -340,8 +346,9 @@ def bench_serve(G, synth, torch, duration=1.0, loads=(0.5, 0.8), budget=32 << 30
     ptrs = np.array([[t.data_ptr() for t in p] for p in pools], dtype=np.uint64)
     lo, hi = 64, 100_000
     mean_size = (hi - lo) / np.log(hi / lo)
-    res = {"workload": "224 surfaces (degrees 2..5, nets 4..48) + 32 cubic rational curves (64 cps); "
-                       "requests log-uniform 64..100000 points, uniform object, Poisson arrivals",
+    res = {"workload": "224 surfaces (degrees 2..5, nets 4..48; depth-6 trees, 64-cell seed grids) + 32 cubic "
+                       "rational curves (64 cps); requests log-uniform 64..100000 points, uniform object, "
+                       "Poisson arrivals",
            "hbm_budget_bytes": budget, "step_slots": streams, "objects_device_bytes": batch.device_bytes() + sum(c.device_bytes() for c in curves)}
     old_switch = sys.getswitchinterval()
     sys.setswitchinterval(1e-4)
@@ -399,7 +406,7 @@ def bench_serve(G, synth, torch, duration=1.0, loads=(0.5, 0.8), budget=32 << 30
                 "mean_step_ms": float(dur.mean() * 1e3), "fixed_ms": float(a * 1e3),
                 "in_step_points_per_s": float(1.0 / b) if b > 0 else None}
 
-    mixed = mixed_options(G, batched=True)
+    mixed = mixed_options(G)
     plain = dict(precision=G.GN_FP64, max_iter=20, tol_step=1e-10)
     try:
         for name, opts in (("mixed", mixed), ("plain_fp64", plain)):
@@ -530,7 +537,7 @@ def main():
         # the mixed mode's algorithm and seeds run entirely in FP64, to separate algorithm from precision
         "fp64_newton": G.default_opts(precision=G.GN_FP64, newton=G.NEWTON_FP64, max_iter=20, tol_step=1e-10,
                                       tol_quad=1e-10, seed_mode=G.SEED_GRID_STEP),
-        # the mixed mode seeded by the greedy descent (no seed grid), as the batched path seeds
+        # the mixed mode seeded by the greedy descent (no seed grid)
         "mixed_greedy_step_seeds": G.default_opts(**dict(mixed_opts, seed_mode=G.SEED_GREEDY_STEP)),
     }
 

@@ -40,7 +40,8 @@ struct BatchArgs {
 // (b % per_tile) * kSeedBlock .. + kSeedBlock - 1 of tile b / per_tile, one point per lane, so
 // every point has a lane of its own (the seed search is a chain of dependent loads).  All lanes
 // of a workgroup search one surface, so the greedy form first stages the FP32 boxes of its tree's
-// levels 1..kBatchSeedLdsLevels (340 boxes, 8 KB) in LDS and descends those levels there.
+// levels 1..kBatchSeedLdsLevels (340 boxes, 8 KB) in LDS and descends those levels there; the grid
+// form (GN_SEED_GRID_STEP) looks the surface's seed grid up instead and stages nothing.
 constexpr int kSeedBlock = 256;
 constexpr int kBatchSeedLdsLevels = 4;
 __global__ __launch_bounds__(kSeedBlock) void k_batch_seed(const GnBatchObj *__restrict__ objs,
@@ -52,8 +53,9 @@ __global__ __launch_bounds__(kSeedBlock) void k_batch_seed(const GnBatchObj *__r
     for (long long b = blockIdx.x; b < nblocks; b += gridDim.x) {  // workgroup-uniform
         const GnTile tl = tiles[b / per_tile];
         const GnSearchDev &sd = objs[tl.obj].sd;
-        const int ll = mode == GN_SEED_NEAREST_LEAF ? 0
-                       : (sd.depth < kBatchSeedLdsLevels ? sd.depth : kBatchSeedLdsLevels);
+        const int ll = mode == GN_SEED_NEAREST_LEAF || mode == GN_SEED_GRID_STEP
+                           ? 0
+                           : (sd.depth < kBatchSeedLdsLevels ? sd.depth : kBatchSeedLdsLevels);
         __syncthreads();  // the previous tile's descents are done with lbox
         if (ll > 0) {
             // boxes of levels 1..ll start one box (24 bytes) into box32: float2 copies
@@ -68,6 +70,14 @@ __global__ __launch_bounds__(kSeedBlock) void k_batch_seed(const GnBatchObj *__r
         const GnReq &R = reqs[tl.req];
         const int64_t i = tl.start + j;
         const double P[3] = {R.x[i], R.y[i], R.z[i]};
+        if (mode == GN_SEED_GRID_STEP) {  // the object (and its grid's width) is workgroup-uniform
+            const uint32_t c = sd.grid_wide ? grid_leaf<true>(sd, P) : grid_leaf<false>(sd, P);
+            double u, v;
+            step_seed(sd, c, sd.rec[(size_t)c * 4], P, u, v);
+            R.u[i] = u;
+            R.v[i] = v;
+            continue;
+        }
         float4 cq;
         const uint32_t c = mode == GN_SEED_NEAREST_LEAF
                                ? nearest_leaf(sd, P)

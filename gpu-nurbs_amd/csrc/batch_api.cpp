@@ -24,6 +24,7 @@ struct gn_batch {
     size_t bytes = 0;
     int tile = 1024;  // points per tile
     int bpc = 0;      // workgroups per CU cap (0: occupancy limit)
+    bool grids = false;  // every surface brought its seed grid (GN_SEED_GRID_STEP allowed)
 };
 
 struct gn_graph {
@@ -132,16 +133,17 @@ int run_batch(gn_batch *b, const gn_segment *segs, int nseg, const double *x, co
     return rc;
 }
 
-int check_opts(const gn_invert_opts &o) {
+int check_opts(const gn_batch *b, const gn_invert_opts &o) {
     if (o.precision != GN_FP64 && o.precision != GN_MIXED)
         return fail(GN_EINVAL, "precision must be GN_FP64 or GN_MIXED");
     if (o.seed_mode == GN_SEED_REF_FIRST_HIT)
         return fail(GN_EINVAL, "first-hit seeding is a single-surface reference mode");
     if (o.seed_mode == GN_SEED_GLOBAL)
         return fail(GN_EINVAL, "the global projection is a single-surface mode");
-    if (o.seed_mode == GN_SEED_GRID_STEP)
-        return fail(GN_EINVAL, "seed grids serve single-surface inversion (gn_invert) only");
-    if (o.seed_mode < GN_SEED_NEAREST_LEAF || o.seed_mode > GN_SEED_GREEDY_STEP)
+    if (o.seed_mode == GN_SEED_GRID_STEP && !b->grids)
+        return fail(GN_ESTATE, "grid seeds need every surface's seed grid (gn_build_seed_grid) "
+                               "built before gn_batch_create");
+    if (o.seed_mode < GN_SEED_NEAREST_LEAF || o.seed_mode > GN_SEED_GRID_STEP)
         return fail(GN_EINVAL, "bad seed_mode");
     if (o.max_iter < 0 || o.fp32_iters < 0 || o.fp64_tail < 0 || o.max_iter > 10000 ||
         o.fp32_iters > 10000 || o.fp64_tail > 10000)
@@ -163,8 +165,9 @@ std::string signature(const gn_batch *b, const gn_segment *segs, int nseg, size_
 namespace gn {
 // Read-only view of a batch for the inversion server.
 int batch_view(const gn_batch *b, const GnBatchObj **objs, int *count, std::vector<int> *cls,
-               int *tile, int *bpc, size_t *bytes) {
+               int *tile, int *bpc, size_t *bytes, bool *grids) {
     if (!b) return fail(GN_EINVAL, "null batch");
+    *grids = b->grids;
     *objs = b->d_objs;
     *count = b->count;
     *cls = b->cls;
@@ -209,6 +212,7 @@ int gn_batch_create(gn_surface *const *surfaces, int count, gn_stream stream, gn
             {s->d_sref, sizeof(int32_t) * leaves, 0},
             {s->d_sleaf32, sizeof(float4) * leaves, 0},
             {s->d_srec, sizeof(float4) * leaves * 4, 0},
+            {s->d_grid, s->grid_bytes, 0},  // none (0 bytes) without a seed grid
         };
         for (Part p : ps) {
             p.off = total;
@@ -219,6 +223,8 @@ int gn_batch_create(gn_surface *const *surfaces, int count, gn_stream stream, gn
     gn_batch *b = new gn_batch();
     b->count = count;
     b->cls.resize(count);
+    b->grids = true;
+    for (int i = 0; i < count; i++) b->grids = b->grids && surfaces[i]->d_grid != nullptr;
     hipError_t e = hipMalloc((void **)&b->d_pack, total);
     if (e != hipSuccess) {
         delete b;
@@ -235,7 +241,7 @@ int gn_batch_create(gn_surface *const *surfaces, int count, gn_stream stream, gn
     for (int i = 0; i < count && e == hipSuccess; i++) {
         const gn_surface *s = surfaces[i];
         for (const Part &p : parts[i])
-            if (e == hipSuccess)
+            if (e == hipSuccess && p.bytes)
                 e = hipMemcpyAsync(b->d_pack + p.off, p.src, p.bytes, hipMemcpyDeviceToDevice, st);
         char *base = b->d_pack;
         GnBatchObj &ob = objs[i];
@@ -255,7 +261,7 @@ int gn_batch_create(gn_surface *const *surfaces, int count, gn_stream stream, gn
         ob.sd.leaf32 = (const float4 *)(base + parts[i][10].off);
         ob.sd.rec = (const float4 *)(base + parts[i][11].off);
         ob.sd.cone32 = nullptr;  // normal cones serve the global projection only
-        ob.sd.grid = nullptr;    // seed grids serve single-surface inversion only
+        ob.sd.grid = s->d_grid ? (const void *)(base + parts[i][12].off) : nullptr;
         ob.cls = degree_class(s->p, s->q);
         ob.pad_ = 0;
         b->cls[i] = ob.cls;
@@ -308,7 +314,7 @@ int gn_batch_invert(gn_batch *b, const gn_segment *segs, int nseg, const double 
     gn_invert_opts o;
     if (opts) o = *opts;
     else gn_invert_opts_default(&o);
-    int rc = check_opts(o);
+    int rc = check_opts(b, o);
     if (rc) return rc;
     if ((rc = check_segments(b, segs, nseg, n))) return rc;
     if (n && (!x || !y || !z || !u || !v || !status)) return fail(GN_EINVAL, "null buffer");
@@ -323,7 +329,7 @@ int gn_batch_tune(gn_batch *b, const gn_segment *segs, int nseg, const double *x
     gn_invert_opts o;
     if (opts) o = *opts;
     else gn_invert_opts_default(&o);
-    int rc = check_opts(o);
+    int rc = check_opts(b, o);
     if (rc) return rc;
     if ((rc = check_segments(b, segs, nseg, n))) return rc;
     if (n && (!x || !y || !z || !u || !v || !status)) return fail(GN_EINVAL, "null buffer");
@@ -406,7 +412,7 @@ int gn_batch_graph_create(gn_batch *b, const gn_segment *segs, int nseg, const d
     gn_invert_opts o;
     if (opts) o = *opts;
     else gn_invert_opts_default(&o);
-    int rc = check_opts(o);
+    int rc = check_opts(b, o);
     if (rc) return rc;
     if ((rc = check_segments(b, segs, nseg, n))) return rc;
     if (n && (!x || !y || !z || !u || !v || !status)) return fail(GN_EINVAL, "null buffer");

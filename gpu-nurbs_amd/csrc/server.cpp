@@ -118,7 +118,7 @@ struct gn_server {
 // Helpers exported by batch_api.cpp for the server (batch internals).
 namespace gn {
 int batch_view(const gn_batch *b, const GnBatchObj **objs, int *count, std::vector<int> *cls,
-               int *tile, int *bpc, size_t *bytes);
+               int *tile, int *bpc, size_t *bytes, bool *grids);
 }
 
 void gn_server::admit_locked() {
@@ -402,9 +402,9 @@ int gn_server_create(gn_batch *batch, gn_curve *const *curves, int ncurves,
     if (o.precision != GN_FP64 && o.precision != GN_MIXED)
         return fail(GN_EINVAL, "precision must be GN_FP64 or GN_MIXED");
     if (o.seed_mode != GN_SEED_GREEDY_LEAF && o.seed_mode != GN_SEED_NEAREST_LEAF &&
-        o.seed_mode != GN_SEED_GREEDY_STEP)
-        return fail(GN_EINVAL,
-                    "server seeds: GN_SEED_GREEDY_LEAF, GN_SEED_GREEDY_STEP or GN_SEED_NEAREST_LEAF");
+        o.seed_mode != GN_SEED_GREEDY_STEP && o.seed_mode != GN_SEED_GRID_STEP)
+        return fail(GN_EINVAL, "server seeds: GN_SEED_GREEDY_LEAF, GN_SEED_GREEDY_STEP, "
+                               "GN_SEED_GRID_STEP or GN_SEED_NEAREST_LEAF");
     if (o.max_iter < 0 || o.fp32_iters < 0 || o.fp64_tail < 0 || o.max_iter > 10000 ||
         o.fp32_iters > 10000 || o.fp64_tail > 10000)
         return fail(GN_EINVAL, "iteration counts must be 0..10000");
@@ -419,9 +419,16 @@ int gn_server_create(gn_batch *batch, gn_curve *const *curves, int ncurves,
     size_t batch_bytes = 0;
     if (batch) {
         const GnBatchObj *objs = nullptr;
-        if ((rc = gn::batch_view(batch, &objs, &s->nsurf, &s->cls, &s->tile, &s->bpc, &batch_bytes))) {
+        bool grids = false;
+        if ((rc = gn::batch_view(batch, &objs, &s->nsurf, &s->cls, &s->tile, &s->bpc, &batch_bytes,
+                                 &grids))) {
             delete s;
             return rc;
+        }
+        if (o.seed_mode == GN_SEED_GRID_STEP && !grids) {
+            delete s;
+            return fail(GN_ESTATE, "grid seeds need every surface's seed grid (gn_build_seed_grid) "
+                                   "built before gn_batch_create");
         }
         s->batch = batch;
         s->d_objs = const_cast<GnBatchObj *>(objs);

@@ -75,8 +75,7 @@ typedef enum {
                                  no surface evaluation, and about one iteration fewer after it */
     GN_SEED_GRID_STEP = 6     /* the leaf that gn_build_seed_grid's uniform grid holds for the
                                  cell of P (one lookup instead of the descent), then the step of
-                                 GN_SEED_GREEDY_STEP from that leaf; single-surface gn_invert
-                                 and gn_seed only */
+                                 GN_SEED_GREEDY_STEP from that leaf; needs the grid */
 } gn_seed_mode;
 
 /* tree build flags */
@@ -254,8 +253,10 @@ typedef struct {
 
 typedef struct gn_graph gn_graph; /* an instantiated hipGraph of one batched inversion */
 
-/* Packs copies of the surfaces' span tables, knots and seed trees into one device allocation.
- * Every surface needs its tree (gn_build_tree).  The surfaces may be destroyed afterwards. */
+/* Packs copies of the surfaces' span tables, knots and seed trees (and seed grids, where built)
+ * into one device allocation.  Every surface needs its tree (gn_build_tree); GN_SEED_GRID_STEP
+ * needs every surface's seed grid (gn_build_seed_grid) built before this call.  The surfaces may
+ * be destroyed afterwards. */
 int gn_batch_create(gn_surface *const *surfaces, int count, gn_stream stream, gn_batch **out);
 int gn_batch_destroy(gn_batch *b);
 size_t gn_batch_device_bytes(const gn_batch *b);
@@ -264,8 +265,9 @@ size_t gn_batch_device_bytes(const gn_batch *b);
 int gn_batch_set_shape(gn_batch *b, int tile_points, int blocks_per_cu);
 int gn_batch_get_shape(const gn_batch *b, int *tile_points, int *blocks_per_cu);
 /* Inverts the points of all segments (device arrays x, y, z, u, v, status of length n) with one
- * persistent launch per precision phase; seeds: GN_SEED_GREEDY_LEAF, GN_SEED_NEAREST_LEAF (each
- * point on its own surface's tree) or GN_SEED_GIVEN. */
+ * persistent launch per precision phase; seeds: GN_SEED_GREEDY_LEAF, GN_SEED_NEAREST_LEAF,
+ * GN_SEED_GREEDY_STEP, GN_SEED_GRID_STEP (each point on its own surface's tree or grid) or
+ * GN_SEED_GIVEN. */
 int gn_batch_invert(gn_batch *b, const gn_segment *segs, int nseg, const double *x,
                     const double *y, const double *z, size_t n, const gn_invert_opts *opts,
                     double *u, double *v, uint8_t *status, gn_stream stream);

@@ -150,12 +150,18 @@ def test_grid_errors_and_lifetime():
     with pytest.raises(G.GnError) as e:
         gs.NewInverse(x, x, x, opts)
     assert e.value.code == G.GN_ESTATE
-    # batches do not take grid seeds
-    gs.BuildSeedGrid()
+    # a batch takes grid seeds only when every surface had its grid when the batch was made
     b = G.Batch([gs])
     with pytest.raises(G.GnError) as e:
         b.invert(G.segments(np.array([0]), np.array([8])), x, x, x, opts)
-    assert e.value.code == G.GN_EINVAL
+    assert e.value.code == G.GN_ESTATE
+    with pytest.raises(G.GnError) as e:
+        G.Server(b, opts=opts)
+    assert e.value.code == G.GN_ESTATE
+    gs.BuildSeedGrid()
+    b = G.Batch([gs])
+    _, _, st = b.invert(G.segments(np.array([0]), np.array([8])), x + 0.5, x + 0.5, x, opts)
+    assert _np(st).max() <= G.PT_MAXITER
 
 
 def _fp32_d2(c, q):
@@ -197,3 +203,46 @@ def test_wide_grid_of_a_four_level_tree():
         gd = d2[np.arange(d2.shape[0]), got[a:a + 50] - base]
         assert np.all(gd <= d2.min(axis=1) * (1 + 1e-5) + 1e-30)
     assert agree >= 396
+
+
+def test_batch_and_server_grid_seeds_equal_single_surface():
+    """Surfaces of several degree classes, each with its seed grid, in one batch: grid-seeded
+    mixed inversion bit-identical to each surface's own gn_invert; the server's results for the
+    same requests bit-identical to the batch's."""
+    defs = [synth.surface_G(12, 10, 2, 3, seed=31, jitter=0.2), synth.surface_G(32, 32, 3, 3, seed=1002),
+            synth.surface_G(14, 12, 4, 5, seed=33, jitter=0.2), synth.surface_G(9, 9, 6, 2, seed=34)]
+    surfs = []
+    for i, sd in enumerate(defs):
+        s = G.Surface(sd.p, sd.q, sd.ku, sd.kv, sd.cp, sd.w)
+        s.BuildTree(3 if i % 2 else 2)
+        s.BuildSeedGrid(0 if i else 40)
+        surfs.append(s)
+    counts = np.array([3000, 20000, 5000, 700])
+    P = []
+    for i, sd in enumerate(defs):
+        os_ = O.OSurf.from_def(sd)
+        u0, v0, Pi = _queries(sd, sd.ku, os_, int(counts[i]), 40 + i)
+        P.append(Pi)
+    P = np.concatenate(P, axis=1)
+    opts = G.default_opts(precision=G.GN_MIXED, newton=G.NEWTON_FP32 | G.NEWTON_FP64,
+                          seed_mode=G.SEED_GRID_STEP)
+    b = G.Batch(surfs)
+    segs = G.segments(np.arange(len(defs)), counts)
+    bu, bv, bst = (_np(t) for t in b.invert(segs, P[0], P[1], P[2], opts))
+    for i, s in enumerate(surfs):
+        a, c = segs["start"][i], segs["start"][i] + counts[i]
+        su, sv, sst = (_np(t) for t in s.NewInverse(P[0][a:c], P[1][a:c], P[2][a:c], opts))
+        assert np.array_equal(bu[a:c], su) and np.array_equal(bv[a:c], sv) and np.array_equal(bst[a:c], sst), i
+    srv = G.Server(b, opts=opts)
+    try:
+        ids = [srv.submit(i, P[0][a:a + n], P[1][a:a + n], P[2][a:a + n])
+               for i, (a, n) in enumerate(zip(segs["start"], counts))]
+        for i, rid in enumerate(ids):
+            a, c = segs["start"][i], segs["start"][i] + counts[i]
+            info = srv.wait(rid, 60000)
+            assert info is not None, "request did not finish"
+            ru, rv, rst = (_np(t) for t in G.Server.results(info))
+            srv.release(rid)
+            assert np.array_equal(ru, bu[a:c]) and np.array_equal(rv, bv[a:c]) and np.array_equal(rst, bst[a:c]), i
+    finally:
+        srv.close()
