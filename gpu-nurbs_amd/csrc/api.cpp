@@ -337,7 +337,7 @@ int gn_build_tree(gn_surface *s, int levels, int flags, gn_stream stream) {
         (rc = dalloc(&s->d_sseed, (size_t)leaves * 2, acc)) ||
         (rc = dalloc(&s->d_sref, (size_t)leaves, acc)) ||
         (rc = dalloc(&s->d_sleaf32, (size_t)leaves, acc)) ||
-        (rc = dalloc(&s->d_srec, (size_t)leaves * 4, acc)) ||
+        (rc = dalloc(&s->d_srec, (size_t)leaves * kRecVec, acc)) ||
         (rc = dalloc(&s->d_scone32, (size_t)leaves, acc))) {
         free_tree(s);
         return rc;

@@ -73,7 +73,7 @@ __global__ __launch_bounds__(kSeedBlock) void k_batch_seed(const GnBatchObj *__r
         if (mode == GN_SEED_GRID_STEP) {  // the object (and its grid's width) is workgroup-uniform
             const uint32_t c = sd.grid_wide ? grid_leaf<true>(sd, P) : grid_leaf<false>(sd, P);
             double u, v;
-            step_seed(sd, c, sd.rec[(size_t)c * 4], P, u, v);
+            step_seed(sd, c, sd.rec[(size_t)c * kRecVec], P, u, v);
             R.u[i] = u;
             R.v[i] = v;
             continue;

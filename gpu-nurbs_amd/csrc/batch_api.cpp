@@ -211,7 +211,7 @@ int gn_batch_create(gn_surface *const *surfaces, int count, gn_stream stream, gn
             {s->d_sseed, sizeof(double) * leaves * 2, 0},
             {s->d_sref, sizeof(int32_t) * leaves, 0},
             {s->d_sleaf32, sizeof(float4) * leaves, 0},
-            {s->d_srec, sizeof(float4) * leaves * 4, 0},
+            {s->d_srec, sizeof(float4) * leaves * kRecVec, 0},
             {s->d_grid, s->grid_bytes, 0},  // none (0 bytes) without a seed grid
         };
         for (Part p : ps) {

@@ -34,6 +34,8 @@ struct GnTreeNode {
 
 // Quadtree search structure used by the nearest-leaf seed search: levels 0..D of 4^l nodes in
 // Morton order; boxes contain every leaf centre below them.
+constexpr int kRecVec = 8;  // 16-byte vectors per leaf record (GnSearchDev::rec)
+
 struct GnSearchDev {
     int depth;                // D = 3*(levels-1)
     const double *box;        // per node 6 doubles lo.xyz hi.xyz, level offsets (4^l-1)/3
@@ -43,9 +45,10 @@ struct GnSearchDev {
     const int32_t *ref_index; // per leaf (Morton order) reference tree index
     const float4 *leaf32;     // per leaf (Morton order) centre in FP32 and the reference index
                               // (bits of w), one 16-byte load for the greedy 3 x 3 search
-    const float4 *rec;        // per leaf (Morton order) one 64-byte record for the stepped
-                              // seeds, one cache line fetch: [0] as leaf32, [1] the seed (u, v)
-                              // as two doubles, [2] S_u and [3] S_v at the seed in FP32 (w = 0)
+    const float4 *rec;        // per leaf (Morton order) one 128-byte record (kRecVec vectors) for
+                              // the stepped seeds, one cache line: [0] as leaf32, [1] the seed
+                              // (u, v) as two doubles, [2..6] S_u, S_v, S_uu, S_uv, S_vv at the
+                              // seed in FP32 (w = 0), [7] zero
     const float4 *cone32;     // per leaf (Morton order) normal cone: unit axis, half-angle
                               // (radians, rounded up; 4 when unbounded); null: no cone pruning
     double u0, u1, v0, v1;    // the knot domain the leaves tile

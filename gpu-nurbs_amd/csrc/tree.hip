@@ -176,9 +176,9 @@ __global__ void k_search_leaves(int levels, const GnTreeNode *__restrict__ tree,
     const float4 l32 = make_float4((float)nd.c[0], (float)nd.c[1], (float)nd.c[2],
                                    __int_as_float((int32_t)(base + idx)));
     sleaf32[c] = l32;
-    srec[c * 4] = l32;
+    srec[c * kRecVec] = l32;
     const long long bu = __double_as_longlong(su), bv = __double_as_longlong(sv);
-    srec[c * 4 + 1] = make_float4(__int_as_float((int)(bu & 0xffffffff)), __int_as_float((int)(bu >> 32)),
+    srec[c * kRecVec + 1] = make_float4(__int_as_float((int)(bu & 0xffffffff)), __int_as_float((int)(bu >> 32)),
                                   __int_as_float((int)(bv & 0xffffffff)), __int_as_float((int)(bv >> 32)));
 }
 
@@ -217,17 +217,19 @@ __global__ void k_search_box32(long total, const double *__restrict__ sbox,
     }
 }
 
-// S_u, S_v at every leaf's seed (u, v), for the stepped seeds: one thread per leaf, into words
-// 2 and 3 of the leaf's record.
+// S_u, S_v, S_uu, S_uv, S_vv at every leaf's seed (u, v), for the stepped seeds: one thread per
+// leaf, into vectors 2..6 of the leaf's record (7 zero).
 template <int K1, int K2, bool EXACT>
 __global__ void k_search_jac(GnSurfDev d, long leaves, const double *__restrict__ sseed,
                              float4 *__restrict__ srec) {
     const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= leaves) return;
     Cart<double> o;
-    eval_point<K1, K2, 1, EXACT, double>(d, d.ku, d.kv, sseed[c * 2], sseed[c * 2 + 1], o);
-    srec[c * 4 + 2] = make_float4((float)o.Su[0], (float)o.Su[1], (float)o.Su[2], 0.f);
-    srec[c * 4 + 3] = make_float4((float)o.Sv[0], (float)o.Sv[1], (float)o.Sv[2], 0.f);
+    eval_point<K1, K2, 2, EXACT, double>(d, d.ku, d.kv, sseed[c * 2], sseed[c * 2 + 1], o);
+    const double *f[5] = {o.Su, o.Sv, o.Suu, o.Suv, o.Svv};
+    float4 *r = srec + c * kRecVec;
+    for (int k = 0; k < 5; k++) r[2 + k] = make_float4((float)f[k][0], (float)f[k][1], (float)f[k][2], 0.f);
+    r[7] = make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
 // Normal cone of every leaf (NormalBoundingBox + NormalBoxToNormalTreeNode,
@@ -438,11 +440,11 @@ __global__ __launch_bounds__(256) void k_seed_grid(GnSearchDev sd, const double 
          i += (size_t)gridDim.x * blockDim.x) {
         const double P[3] = {x[i], y[i], z[i]};
         const uint32_t c = grid_leaf<WIDE>(sd, P);
-        const float4 *r = sd.rec + (size_t)c * 4;
-        const float4 cq = r[0], ju = r[2], jv = r[3];
+        const float4 *r = sd.rec + (size_t)c * kRecVec;
+        const float4 cq = r[0];
         const double2 uv = *reinterpret_cast<const double2 *>(r + 1);
         double u = uv.x, v = uv.y;
-        step_seed_from(sd, cq, ju, jv, P, u, v);
+        step_seed_from(sd, cq, r[2], r[3], r[4], r[5], r[6], P, u, v);
         if (leaf) leaf[i] = __float_as_int(cq.w);
         su[i] = u;
         sv[i] = v;

@@ -71,8 +71,10 @@ typedef enum {
     GN_SEED_GREEDY_STEP = 5,  /* the greedy descent's leaf (GN_SEED_GREEDY_LEAF without its 3 x 3
                                  neighbour refinement), then one Gauss-Newton step from the
                                  leaf's seed with the Jacobian stored for that leaf (S_u, S_v at
-                                 its seed), limited to one leaf cell and clamped to the domain:
-                                 no surface evaluation, and about one iteration fewer after it */
+                                 its seed), limited to one leaf cell, and two Gauss-Newton
+                                 iterations on the leaf's stored quadratic Taylor model (S_uu,
+                                 S_uv, S_vv too), limited to two cells, clamped to the domain:
+                                 no surface evaluation, and about 1.5 iterations fewer after it */
     GN_SEED_GRID_STEP = 6     /* the leaf that gn_build_seed_grid's uniform grid holds for the
                                  cell of P (one lookup instead of the descent), then the step of
                                  GN_SEED_GREEDY_STEP from that leaf; needs the grid */

@@ -1014,6 +1014,7 @@ static void seed_step_from_leaf(const or_surf *sfp, const double *nodes, int lev
     double u0 = ku[p], u1 = ku[nu], v0 = kv[q], v1 = kv[nv];
     float cells = (float)(1L << (3 * (levels - 1)));
     float hu = (float)(u1 - u0) / cells, hv = (float)(v1 - v0) / cells;
+    float ihu = 1.f / hu, ihv = 1.f / hv, ih2u = 0.5f * ihu, ih2v = 0.5f * ihv;
 #ifdef _OPENMP
     if (nthreads <= 0) nthreads = omp_get_max_threads();
 #pragma omp parallel for schedule(static) num_threads(nthreads)
@@ -1021,24 +1022,62 @@ static void seed_step_from_leaf(const or_surf *sfp, const double *nodes, int lev
     for (long i = 0; i < n; i++) {
         const double *C = nodes + (long)leaf_out[i] * 13 + 10;
         or_pt o;
-        eval_one(&sf, seed_u[i], seed_v[i], 1, &o);
-        float ju[3] = {(float)o.Su[0], (float)o.Su[1], (float)o.Su[2]};
-        float jv[3] = {(float)o.Sv[0], (float)o.Sv[1], (float)o.Sv[2]};
-        float r0 = (float)C[0] - (float)x[i], r1 = (float)C[1] - (float)y[i],
-              r2 = (float)C[2] - (float)z[i];
+        eval_one(&sf, seed_u[i], seed_v[i], 2, &o);
+        float ju[3], jv[3], huu[3], huv[3], hvv[3], r[3];
+        for (int k = 0; k < 3; k++) {
+            ju[k] = (float)o.Su[k];
+            jv[k] = (float)o.Sv[k];
+            huu[k] = (float)o.Suu[k];
+            huv[k] = (float)o.Suv[k];
+            hvv[k] = (float)o.Svv[k];
+        }
+        r[0] = (float)C[0] - (float)x[i];
+        r[1] = (float)C[1] - (float)y[i];
+        r[2] = (float)C[2] - (float)z[i];
+        /* the linear step from the leaf's seed */
         float a = fmaf(ju[2], ju[2], fmaf(ju[1], ju[1], ju[0] * ju[0]));
         float b = fmaf(ju[2], jv[2], fmaf(ju[1], jv[1], ju[0] * jv[0]));
         float e = fmaf(jv[2], jv[2], fmaf(jv[1], jv[1], jv[0] * jv[0]));
-        float gu = fmaf(ju[2], r2, fmaf(ju[1], r1, ju[0] * r0));
-        float gv = fmaf(jv[2], r2, fmaf(jv[1], r1, jv[0] * r0));
+        float gu = fmaf(ju[2], r[2], fmaf(ju[1], r[1], ju[0] * r[0]));
+        float gv = fmaf(jv[2], r[2], fmaf(jv[1], r[1], jv[0] * r[0]));
         float det = fmaf(a, e, -(b * b));
-        if (!(det > 0.f)) continue;
-        float du = fmaf(e, gu, -(b * gv)) / det;
-        float dv = fmaf(a, gv, -(b * gu)) / det;
-        float sc = fmaxf(1.f, fmaxf(fabsf(du) / hu, fabsf(dv) / hv));
-        if (!(sc < INFINITY)) continue;
-        seed_u[i] = fmin(fmax(seed_u[i] - (double)(du / sc), u0), u1);
-        seed_v[i] = fmin(fmax(seed_v[i] - (double)(dv / sc), v0), v1);
+        float idet = 1.f / det;
+        float du = fmaf(e, gu, -(b * gv)) * idet;
+        float dv = fmaf(a, gv, -(b * gu)) * idet;
+        float sc = fmaxf(1.f, fmaxf(fabsf(du) * ihu, fabsf(dv) * ihv));
+        if (!(det > 0.f && sc < INFINITY)) continue;
+        float isc = 1.f / sc;
+        float tu = -(du * isc), tv = -(dv * isc);
+        /* Gauss-Newton iterations on the leaf's quadratic model
+         * m(t) = C + S_u tu + S_v tv + (S_uu tu^2 + 2 S_uv tu tv + S_vv tv^2) / 2 */
+        for (int it = 0; it < 2; it++) {
+            float tuu = tu * tu, tuv = tu * tv, tvv = tv * tv;
+            float m[3], au[3], av[3];
+            for (int k = 0; k < 3; k++) {
+                float t = fmaf(ju[k], tu, fmaf(jv[k], tv, r[k]));
+                float w = fmaf(huu[k], tuu, fmaf(2.f * huv[k], tuv, hvv[k] * tvv));
+                m[k] = fmaf(0.5f, w, t);
+                au[k] = fmaf(huu[k], tu, fmaf(huv[k], tv, ju[k]));
+                av[k] = fmaf(huv[k], tu, fmaf(hvv[k], tv, jv[k]));
+            }
+            float A = fmaf(au[2], au[2], fmaf(au[1], au[1], au[0] * au[0]));
+            float B = fmaf(au[2], av[2], fmaf(au[1], av[1], au[0] * av[0]));
+            float E = fmaf(av[2], av[2], fmaf(av[1], av[1], av[0] * av[0]));
+            float Gu = fmaf(au[2], m[2], fmaf(au[1], m[1], au[0] * m[0]));
+            float Gv = fmaf(av[2], m[2], fmaf(av[1], m[1], av[0] * m[0]));
+            float D = fmaf(A, E, -(B * B));
+            float iD = 1.f / D;
+            float nu_ = tu - fmaf(E, Gu, -(B * Gv)) * iD;
+            float nv_ = tv - fmaf(A, Gv, -(B * Gu)) * iD;
+            float s2 = fmaxf(1.f, fmaxf(fabsf(nu_) * ih2u, fabsf(nv_) * ih2v));
+            if (D > 0.f && s2 < INFINITY) {
+                float is2 = 1.f / s2;
+                tu = nu_ * is2;
+                tv = nv_ * is2;
+            }
+        }
+        seed_u[i] = fmin(fmax(seed_u[i] + (double)tu, u0), u1);
+        seed_v[i] = fmin(fmax(seed_v[i] + (double)tv, v0), v1);
     }
 }
 

@@ -95,11 +95,12 @@ def test_step_is_the_greedy_step_from_the_grid_leaf():
     same = leaf == gl
     assert same.mean() > 0.5
     assert np.array_equal(su[same], gsu[same]) and np.array_equal(sv[same], gsv[same])
-    # and the leaf's own uv centre where the step is singular is never left by more than a cell
+    # and the seeds never leave their leaf's uv centre by more than two cells (the model
+    # iterations' limit)
     nd = nodes[leaf]
     cu, cv = (nd[:, 0] + nd[:, 2]) / 2, (nd[:, 1] + nd[:, 3]) / 2
-    assert np.all(np.abs(su - cu) <= (nd[:, 2] - nd[:, 0]) * (1 + 1e-6))
-    assert np.all(np.abs(sv - cv) <= (nd[:, 3] - nd[:, 1]) * (1 + 1e-6))
+    assert np.all(np.abs(su - cu) <= 2 * (nd[:, 2] - nd[:, 0]) * (1 + 1e-6))
+    assert np.all(np.abs(sv - cv) <= 2 * (nd[:, 3] - nd[:, 1]) * (1 + 1e-6))
 
 
 def test_grid_seeds_save_iterations_and_keep_foot_points():

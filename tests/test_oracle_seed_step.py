@@ -73,32 +73,52 @@ def _descent(nodes, levels, P):
 
 
 def test_step_equals_an_independent_gauss_newton_step():
+    """The step restated in FP64 numpy with the derivatives from scipy's basis: one Gauss-Newton
+    step from the leaf's seed (at most one leaf cell), then two Gauss-Newton iterations on the
+    leaf's quadratic Taylor model (each result at most two cells from the seed)."""
     for scale in (1.0, 5.0):  # also on a knot domain other than [0, 1]
         sd, ku, os_, nodes, u0, v0, P = _case(3, ku_scale=scale)
         leaf, su, sv = os_.seed_greedy_step(nodes, 3, P[0], P[1], P[2])
         gl, gu_, gv_ = O.seed_greedy_descent(nodes, 3, P[0], P[1], P[2])
         assert np.array_equal(leaf, gl)
         assert np.array_equal(gl, _descent(nodes, 3, P))
-        # numpy: J at the leaf's seed from scipy's basis, C from the tree node
-        d = scipy_eval(sd.p, sd.q, ku, sd.kv, sd.cp, sd.w, gu_, gv_, order=1)
+        # numpy: J and the second derivatives at the leaf's seed from scipy's basis, C from the node
+        d = scipy_eval(sd.p, sd.q, ku, sd.kv, sd.cp, sd.w, gu_, gv_, order=2)
+        Su, Sv, Suu, Suv, Svv = d["Su"], d["Sv"], d["Suu"], d["Suv"], d["Svv"]
         C = nodes[leaf, 10:13].T
         r = C - P
-        a, b, e = (d["Su"] ** 2).sum(0), (d["Su"] * d["Sv"]).sum(0), (d["Sv"] ** 2).sum(0)
-        g1, g2 = (d["Su"] * r).sum(0), (d["Sv"] * r).sum(0)
+        a, b, e = (Su ** 2).sum(0), (Su * Sv).sum(0), (Sv ** 2).sum(0)
+        g1, g2 = (Su * r).sum(0), (Sv * r).sum(0)
         det = a * e - b * b
         du, dv = (e * g1 - b * g2) / det, (a * g2 - b * g1) / det
         hu, hv = (ku[-1] - ku[0]) / 64, (sd.kv[-1] - sd.kv[0]) / 64
         sc = np.maximum(1.0, np.maximum(np.abs(du) / hu, np.abs(dv) / hv))
-        eu = np.clip(gu_ - du / sc, ku[0], ku[-1])
-        ev = np.clip(gv_ - dv / sc, sd.kv[0], sd.kv[-1])
-        # the oracle's step is FP32 (as the library's): within FP32 rounding of the FP64 step,
-        # whose residual C - P rounds to about 1e-7 of the coordinates (1e-5 of a leaf cell)
-        su_step, sv_step = np.abs(du / sc), np.abs(dv / sc)
-        assert np.all(np.abs(su - eu) <= 2e-6 * su_step + 1e-5 * hu)
-        assert np.all(np.abs(sv - ev) <= 2e-6 * sv_step + 1e-5 * hv)
-        # at most one leaf cell from the leaf's seed, inside the domain
-        assert np.all(np.abs(su - gu_) <= hu * (1 + 1e-12)) and np.all(np.abs(sv - gv_) <= hv * (1 + 1e-12))
+        tu, tv = -du / sc, -dv / sc
+        for _ in range(2):
+            m = r + Su * tu + Sv * tv + 0.5 * (Suu * tu ** 2 + 2 * Suv * tu * tv + Svv * tv ** 2)
+            Au, Av = Su + Suu * tu + Suv * tv, Sv + Suv * tu + Svv * tv
+            A, B, E = (Au ** 2).sum(0), (Au * Av).sum(0), (Av ** 2).sum(0)
+            G1, G2 = (Au * m).sum(0), (Av * m).sum(0)
+            D = A * E - B * B
+            nu, nv = tu - (E * G1 - B * G2) / D, tv - (A * G2 - B * G1) / D
+            s2 = np.maximum(1.0, np.maximum(np.abs(nu) / (2 * hu), np.abs(nv) / (2 * hv)))
+            ok = D > 0
+            tu, tv = np.where(ok, nu / s2, tu), np.where(ok, nv / s2, tv)
+        eu = np.clip(gu_ + tu, ku[0], ku[-1])
+        ev = np.clip(gv_ + tv, sd.kv[0], sd.kv[-1])
+        # the oracle's step is FP32 (as the library's): within FP32 rounding of the FP64 one, whose
+        # residual C - P rounds to about 1e-7 of the coordinates (1e-5 of a leaf cell), amplified
+        # where the model's system is poorly conditioned
+        err = np.maximum(np.abs(su - eu) / hu, np.abs(sv - ev) / hv)
+        assert np.median(err) < 5e-6 and np.percentile(err, 99) < 2e-5 and err.max() < 1e-4, err.max()
+        # at most two leaf cells from the leaf's seed, inside the domain
+        assert np.all(np.abs(su - gu_) <= 2 * hu * (1 + 1e-6)) and np.all(np.abs(sv - gv_) <= 2 * hv * (1 + 1e-6))
         assert su.min() >= ku[0] and su.max() <= ku[-1]
+        # the model iterations pay: seeds nearer the foot points than the linear step's
+        lu, lv = np.clip(gu_ - du / sc, ku[0], ku[-1]), np.clip(gv_ - dv / sc, sd.kv[0], sd.kv[-1])
+        e_lin = np.maximum(np.abs(lu - u0), np.abs(lv - v0))
+        e_mod = np.maximum(np.abs(su - u0), np.abs(sv - v0))
+        assert np.median(e_mod) < 0.3 * np.median(e_lin)
 
 
 def test_step_brings_seeds_nearer_and_saves_iterations():
