@@ -149,6 +149,14 @@ __device__ __forceinline__ void project_point(const GnSurfDev &d, const GnSearch
     double bu = sd.seed_uv[2 * c0], bv = sd.seed_uv[2 * c0 + 1], best;
     int bst = solve_in<K1, K2, EXACT>(d, P, bu, bv, full, max_iter, tol_step, best);
     int ncand = 1;
+    // The bound solve stands for leaf c0's own solve only if it ended inside c0's cell; a solve
+    // that wandered off may have passed a nearer minimum in the cell, so c0 then stays a candidate.
+    bool c0_done;
+    {
+        const uint32_t ix = compact2(c0 >> 1), iy = compact2(c0);
+        c0_done = bu >= full.u0 + wu * (ix * cell) && bu <= full.u0 + wu * ((ix + 1) * cell) &&
+                  bv >= full.v0 + wv * (iy * cell) && bv <= full.v0 + wv * ((iy + 1) * cell);
+    }
     if (D > 0) {
         const float p[3] = {(float)P[0], (float)P[1], (float)P[2]};
         // stackless depth-first walk (Morton-code successor), pruned by the current best.  The
@@ -170,7 +178,7 @@ __device__ __forceinline__ void project_point(const GnSurfDev &d, const GnSearch
                 if (lb <= best) {
                     if (l < D) {
                         descend = true;
-                    } else if (c != c0 &&
+                    } else if ((c != c0 || !c0_done) &&
                                (!sd.cone32 || compact2(c >> 1) - 1u >= side - 2u ||
                                 compact2(c) - 1u >= side - 2u ||
                                 cone_may_hold(sd.cone32[c], sd.box32 + (qoff(l) + c) * 6, P))) {

@@ -1217,9 +1217,11 @@ void or_seed_grid_step(int p, int q, int nu, int nv, const double *ku, const dou
  * branch and bound: exact-Hessian projected Newton (a coordinate held on its bound while the
  * descent points outward) from the greedy seed leaf (or_seed_greedy_leaf) over the whole domain, then
  * from the centre of EVERY leaf with the iterate clamped to that leaf's uv cell (leaf_iter
- * steps), the nearest foot point kept (ties to the earlier candidate), then a full-domain Newton
- * polish from it that is kept unless it is farther by more than rounding (1e-9 relative).  The tree is the reference's (uv cells of
- * [0,1]^2, nodes as or_build_tree); leaves in Morton order of their cell (x bits odd). */
+ * steps; the seed leaf itself only if the first solve ended outside its cell), the nearest foot
+ * point kept (ties to the earlier candidate), then a full-domain Newton polish from it that is
+ * kept unless it is farther by more than rounding (1e-9 relative).  The tree is the reference's
+ * (uv cells of [0,1]^2, nodes as or_build_tree); leaves in Morton order of their cell (x bits
+ * odd). */
 static double dist2_at(const or_surf *sf, const double P[3], double u, double v)
 {
     or_pt o;
@@ -1255,8 +1257,10 @@ void or_project_global(int p, int q, int nu, int nv, const double *ku, const dou
         double bu = (nd0[0] + nd0[2]) / 2, bv = (nd0[1] + nd0[3]) / 2;
         int bst = gn_one_in(&sf, P, &full, &bu, &bv, NULL, NULL);
         double best = dist2_at(&sf, P, bu, bv);
+        /* the bound solve stands for the seed leaf's own solve only if it ended in that cell */
+        int bl_done = bu >= nd0[0] && bu <= nd0[2] && bv >= nd0[1] && bv <= nd0[3];
         for (long t = 0; t < side * side; t++) {
-            if (t == bl) continue;
+            if (t == bl && bl_done) continue;
             const double *nd = nodes + (base + t) * 13;
             double dom[4] = {nd[0], nd[2], nd[1], nd[3]};
             double cu = (nd[0] + nd[2]) / 2, cv = (nd[1] + nd[3]) / 2;
