@@ -139,6 +139,20 @@ class NurbsSurface {
     void BuildSeedGrid(int resolution = 0, gn_stream st = nullptr) {
         check(gn_build_seed_grid(s_, resolution, st));
     }
+    // host copy of the seed grid: cells per axis, origin, cell size and each cell's leaf
+    // (reference index) at (i * dims[1] + j) * dims[2] + k
+    struct SeedGrid {
+        int dims[3] = {0, 0, 0};
+        float origin[3] = {0.f, 0.f, 0.f}, cell = 0.f;
+        std::vector<int32_t> leaf;
+    };
+    SeedGrid GetSeedGrid() const {
+        SeedGrid g;
+        check(gn_get_seed_grid(s_, nullptr, 0, g.dims, g.origin, &g.cell));
+        g.leaf.resize((size_t)g.dims[0] * g.dims[1] * g.dims[2]);
+        check(gn_get_seed_grid(s_, g.leaf.data(), g.leaf.size(), nullptr, nullptr, nullptr));
+        return g;
+    }
     // seed search alone (exact: 0 greedy leaf, 2 greedy descent + one Gauss-Newton step, 3 seed
     // grid leaf + the step, other values the exact nearest leaf) into device leaf indices and
     // seeds
