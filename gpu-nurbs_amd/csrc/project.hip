@@ -145,8 +145,12 @@ __device__ __forceinline__ void project_point(const GnSurfDev &d, const GnSearch
     const int max_iter = a.o.max_iter;
     const double tol_step = a.o.tol_step;
     const double P[3] = {a.x[i], a.y[i], a.z[i]};
-    const uint32_t c0 = greedy_leaf(sd, P);
-    double bu = sd.seed_uv[2 * c0], bv = sd.seed_uv[2 * c0 + 1], best;
+    // the bound solve starts from the greedy leaf's stepped seed (step_seed: one Gauss-Newton
+    // step and two on the leaf's quadratic model), a few Newton iterations nearer than its centre
+    float4 cq;
+    const uint32_t c0 = greedy_leaf(sd, P, nullptr, 0, &cq);
+    double bu, bv, best;
+    step_seed(sd, c0, cq, P, bu, bv);
     int bst = solve_in<K1, K2, EXACT>(d, P, bu, bv, full, max_iter, tol_step, best);
     int ncand = 1;
     // The bound solve stands for leaf c0's own solve only if it ended inside c0's cell; a solve

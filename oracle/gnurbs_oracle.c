@@ -1215,7 +1215,8 @@ void or_seed_grid_step(int p, int q, int nu, int nv, const double *ku, const dou
 /* Global projection (the library's GN_SEED_GLOBAL; the reference's normal-cone projection,
  * src/kernel/kernel.cu:471-513, 623-704, is unfinished).  Exhaustive form of the library's
  * branch and bound: exact-Hessian projected Newton (a coordinate held on its bound while the
- * descent points outward) from the greedy seed leaf (or_seed_greedy_leaf) over the whole domain, then
+ * descent points outward) from the greedy seed leaf's stepped seed (or_seed_greedy_leaf, then
+ * seed_step_from_leaf) over the whole domain, then
  * from the centre of EVERY leaf with the iterate clamped to that leaf's uv cell (leaf_iter
  * steps; the seed leaf itself only if the first solve ended outside its cell), the nearest foot
  * point kept (ties to the earlier candidate), then a full-domain Newton polish from it that is
@@ -1243,6 +1244,7 @@ void or_project_global(int p, int q, int nu, int nv, const double *ku, const dou
     int *gleaf = malloc(sizeof(int) * (n > 0 ? n : 1));
     double *gsu = malloc(sizeof(double) * (n > 0 ? n : 1)), *gsv = malloc(sizeof(double) * (n > 0 ? n : 1));
     greedy_leaf_impl(nodes, levels, x, y, z, n, gleaf, gsu, gsv, nthreads, 1);
+    seed_step_from_leaf(&sf, nodes, levels, x, y, z, n, gleaf, gsu, gsv, nthreads);
 #ifdef _OPENMP
     if (nthreads <= 0) nthreads = omp_get_max_threads();
 #pragma omp parallel for schedule(dynamic, 16) num_threads(nthreads)
@@ -1251,10 +1253,11 @@ void or_project_global(int p, int q, int nu, int nv, const double *ku, const dou
 #endif
     for (long i = 0; i < n; i++) {
         double P[3] = {x[i], y[i], z[i]};
-        /* the greedy seed leaf (descent + 3 x 3, as or_seed_greedy_leaf) starts the bound solve */
+        /* the greedy seed leaf (descent + 3 x 3, as or_seed_greedy_leaf) and its stepped seed
+         * (seed_step_from_leaf) start the bound solve */
         long bl = gleaf[i] - base;
         const double *nd0 = nodes + (base + bl) * 13;
-        double bu = (nd0[0] + nd0[2]) / 2, bv = (nd0[1] + nd0[3]) / 2;
+        double bu = gsu[i], bv = gsv[i];
         int bst = gn_one_in(&sf, P, &full, &bu, &bv, NULL, NULL);
         double best = dist2_at(&sf, P, bu, bv);
         /* the bound solve stands for the seed leaf's own solve only if it ended in that cell */
