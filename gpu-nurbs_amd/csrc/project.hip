@@ -213,7 +213,12 @@ __device__ __forceinline__ void project_point(const GnSurfDev &d, const GnSearch
                 const uint32_t ix = compact2(lc >> 1), iy = compact2(lc);
                 const Domain cr{full.u0 + wu * (ix * cell), full.u0 + wu * ((ix + 1) * cell),
                                 full.v0 + wv * (iy * cell), full.v0 + wv * ((iy + 1) * cell)};
-                double cu = sd.seed_uv[2 * lc], cv = sd.seed_uv[2 * lc + 1], d2;
+                // from the leaf's stepped seed for P (step_seed from its record), clamped into
+                // its cell: fewer Newton steps than from the cell centre
+                double cu, cv, d2;
+                step_seed(sd, lc, sd.rec[(size_t)lc * kRecVec], P, cu, cv);
+                cu = fmin(fmax(cu, cr.u0), cr.u1);
+                cv = fmin(fmax(cv, cr.v0), cr.v1);
                 const int s2 = solve_in<K1, K2, EXACT>(d, P, cu, cv, cr, kLeafIter, tol_step, d2);
                 ncand++;
                 if (d2 < best) {

@@ -1002,82 +1002,121 @@ void or_seed_greedy_descent(const double *nodes, int levels, const double *x, co
  * FP32 from C, P and J rounded to FP32, scaled down uniformly to at most one leaf cell per
  * direction, applied to the FP64 seed and clamped to the knot domain; a singular J^T J keeps the
  * leaf seed.  Written with the same explicit fmaf steps as the library. */
-/* The step itself, from the leaves in leaf_out (reference indices) whose uv centres are in
- * seed_u / seed_v. */
+/* The step itself.  or_leafrec: what the library's 128-byte leaf record holds (FP32 centre C and
+ * S_u, S_v, S_uu, S_uv, S_vv at the leaf's seed, the seed (u, v)); step_from_rec: the step from
+ * it for one query P (leaves the seed when the linear step's system is singular). */
+typedef struct {
+    float C[3], ju[3], jv[3], huu[3], huv[3], hvv[3];
+    double su, sv;
+} or_leafrec;
+
+static void leaf_rec(const or_surf *sf, const double *nodes, long t, or_leafrec *L)
+{
+    const double *nd = nodes + t * 13;
+    L->su = (nd[0] + nd[2]) / 2;
+    L->sv = (nd[1] + nd[3]) / 2;
+    or_pt o;
+    eval_one(sf, L->su, L->sv, 2, &o);
+    for (int k = 0; k < 3; k++) {
+        L->C[k] = (float)nd[10 + k];
+        L->ju[k] = (float)o.Su[k];
+        L->jv[k] = (float)o.Sv[k];
+        L->huu[k] = (float)o.Suu[k];
+        L->huv[k] = (float)o.Suv[k];
+        L->hvv[k] = (float)o.Svv[k];
+    }
+}
+
+typedef struct {
+    float ihu, ihv, ih2u, ih2v;
+    double u0, u1, v0, v1;
+} or_stepdom;
+
+static or_stepdom step_domain(const or_surf *sf, int levels)
+{
+    or_stepdom g;
+    g.u0 = sf->ku[sf->p];
+    g.u1 = sf->ku[sf->nu];
+    g.v0 = sf->kv[sf->q];
+    g.v1 = sf->kv[sf->nv];
+    float cells = (float)(1L << (3 * (levels - 1)));
+    float hu = (float)(g.u1 - g.u0) / cells, hv = (float)(g.v1 - g.v0) / cells;
+    g.ihu = 1.f / hu;
+    g.ihv = 1.f / hv;
+    g.ih2u = 0.5f * g.ihu;
+    g.ih2v = 0.5f * g.ihv;
+    return g;
+}
+
+static void step_from_rec(const or_leafrec *L, const or_stepdom *g, const double P[3], double *su,
+                          double *sv)
+{
+    const float *ju = L->ju, *jv = L->jv, *huu = L->huu, *huv = L->huv, *hvv = L->hvv;
+    float r[3];
+    *su = L->su;
+    *sv = L->sv;
+    for (int k = 0; k < 3; k++) r[k] = L->C[k] - (float)P[k];
+    /* the linear step from the leaf's seed */
+    float a = fmaf(ju[2], ju[2], fmaf(ju[1], ju[1], ju[0] * ju[0]));
+    float b = fmaf(ju[2], jv[2], fmaf(ju[1], jv[1], ju[0] * jv[0]));
+    float e = fmaf(jv[2], jv[2], fmaf(jv[1], jv[1], jv[0] * jv[0]));
+    float gu = fmaf(ju[2], r[2], fmaf(ju[1], r[1], ju[0] * r[0]));
+    float gv = fmaf(jv[2], r[2], fmaf(jv[1], r[1], jv[0] * r[0]));
+    float det = fmaf(a, e, -(b * b));
+    float idet = 1.f / det;
+    float du = fmaf(e, gu, -(b * gv)) * idet;
+    float dv = fmaf(a, gv, -(b * gu)) * idet;
+    float sc = fmaxf(1.f, fmaxf(fabsf(du) * g->ihu, fabsf(dv) * g->ihv));
+    if (!(det > 0.f && sc < INFINITY)) return;
+    float isc = 1.f / sc;
+    float tu = -(du * isc), tv = -(dv * isc);
+    /* Gauss-Newton iterations on the leaf's quadratic model
+     * m(t) = C + S_u tu + S_v tv + (S_uu tu^2 + 2 S_uv tu tv + S_vv tv^2) / 2 */
+    for (int it = 0; it < 2; it++) {
+        float tuu = tu * tu, tuv = tu * tv, tvv = tv * tv;
+        float m[3], au[3], av[3];
+        for (int k = 0; k < 3; k++) {
+            float t = fmaf(ju[k], tu, fmaf(jv[k], tv, r[k]));
+            float w = fmaf(huu[k], tuu, fmaf(2.f * huv[k], tuv, hvv[k] * tvv));
+            m[k] = fmaf(0.5f, w, t);
+            au[k] = fmaf(huu[k], tu, fmaf(huv[k], tv, ju[k]));
+            av[k] = fmaf(huv[k], tu, fmaf(hvv[k], tv, jv[k]));
+        }
+        float A = fmaf(au[2], au[2], fmaf(au[1], au[1], au[0] * au[0]));
+        float B = fmaf(au[2], av[2], fmaf(au[1], av[1], au[0] * av[0]));
+        float E = fmaf(av[2], av[2], fmaf(av[1], av[1], av[0] * av[0]));
+        float Gu = fmaf(au[2], m[2], fmaf(au[1], m[1], au[0] * m[0]));
+        float Gv = fmaf(av[2], m[2], fmaf(av[1], m[1], av[0] * m[0]));
+        float D = fmaf(A, E, -(B * B));
+        float iD = 1.f / D;
+        float nu_ = tu - fmaf(E, Gu, -(B * Gv)) * iD;
+        float nv_ = tv - fmaf(A, Gv, -(B * Gu)) * iD;
+        float s2 = fmaxf(1.f, fmaxf(fabsf(nu_) * g->ih2u, fabsf(nv_) * g->ih2v));
+        if (D > 0.f && s2 < INFINITY) {
+            float is2 = 1.f / s2;
+            tu = nu_ * is2;
+            tv = nv_ * is2;
+        }
+    }
+    *su = fmin(fmax(*su + (double)tu, g->u0), g->u1);
+    *sv = fmin(fmax(*sv + (double)tv, g->v0), g->v1);
+}
+
+/* The step for the points whose leaves (reference indices) are in leaf_out. */
 static void seed_step_from_leaf(const or_surf *sfp, const double *nodes, int levels,
                                 const double *x, const double *y, const double *z, long n,
                                 const int *leaf_out, double *seed_u, double *seed_v, int nthreads)
 {
-    const or_surf sf = *sfp;
-    const double *ku = sf.ku, *kv = sf.kv;
-    const int p = sf.p, q = sf.q, nu = sf.nu, nv = sf.nv;
-    double u0 = ku[p], u1 = ku[nu], v0 = kv[q], v1 = kv[nv];
-    float cells = (float)(1L << (3 * (levels - 1)));
-    float hu = (float)(u1 - u0) / cells, hv = (float)(v1 - v0) / cells;
-    float ihu = 1.f / hu, ihv = 1.f / hv, ih2u = 0.5f * ihu, ih2v = 0.5f * ihv;
+    const or_stepdom g = step_domain(sfp, levels);
 #ifdef _OPENMP
     if (nthreads <= 0) nthreads = omp_get_max_threads();
 #pragma omp parallel for schedule(static) num_threads(nthreads)
 #endif
     for (long i = 0; i < n; i++) {
-        const double *C = nodes + (long)leaf_out[i] * 13 + 10;
-        or_pt o;
-        eval_one(&sf, seed_u[i], seed_v[i], 2, &o);
-        float ju[3], jv[3], huu[3], huv[3], hvv[3], r[3];
-        for (int k = 0; k < 3; k++) {
-            ju[k] = (float)o.Su[k];
-            jv[k] = (float)o.Sv[k];
-            huu[k] = (float)o.Suu[k];
-            huv[k] = (float)o.Suv[k];
-            hvv[k] = (float)o.Svv[k];
-        }
-        r[0] = (float)C[0] - (float)x[i];
-        r[1] = (float)C[1] - (float)y[i];
-        r[2] = (float)C[2] - (float)z[i];
-        /* the linear step from the leaf's seed */
-        float a = fmaf(ju[2], ju[2], fmaf(ju[1], ju[1], ju[0] * ju[0]));
-        float b = fmaf(ju[2], jv[2], fmaf(ju[1], jv[1], ju[0] * jv[0]));
-        float e = fmaf(jv[2], jv[2], fmaf(jv[1], jv[1], jv[0] * jv[0]));
-        float gu = fmaf(ju[2], r[2], fmaf(ju[1], r[1], ju[0] * r[0]));
-        float gv = fmaf(jv[2], r[2], fmaf(jv[1], r[1], jv[0] * r[0]));
-        float det = fmaf(a, e, -(b * b));
-        float idet = 1.f / det;
-        float du = fmaf(e, gu, -(b * gv)) * idet;
-        float dv = fmaf(a, gv, -(b * gu)) * idet;
-        float sc = fmaxf(1.f, fmaxf(fabsf(du) * ihu, fabsf(dv) * ihv));
-        if (!(det > 0.f && sc < INFINITY)) continue;
-        float isc = 1.f / sc;
-        float tu = -(du * isc), tv = -(dv * isc);
-        /* Gauss-Newton iterations on the leaf's quadratic model
-         * m(t) = C + S_u tu + S_v tv + (S_uu tu^2 + 2 S_uv tu tv + S_vv tv^2) / 2 */
-        for (int it = 0; it < 2; it++) {
-            float tuu = tu * tu, tuv = tu * tv, tvv = tv * tv;
-            float m[3], au[3], av[3];
-            for (int k = 0; k < 3; k++) {
-                float t = fmaf(ju[k], tu, fmaf(jv[k], tv, r[k]));
-                float w = fmaf(huu[k], tuu, fmaf(2.f * huv[k], tuv, hvv[k] * tvv));
-                m[k] = fmaf(0.5f, w, t);
-                au[k] = fmaf(huu[k], tu, fmaf(huv[k], tv, ju[k]));
-                av[k] = fmaf(huv[k], tu, fmaf(hvv[k], tv, jv[k]));
-            }
-            float A = fmaf(au[2], au[2], fmaf(au[1], au[1], au[0] * au[0]));
-            float B = fmaf(au[2], av[2], fmaf(au[1], av[1], au[0] * av[0]));
-            float E = fmaf(av[2], av[2], fmaf(av[1], av[1], av[0] * av[0]));
-            float Gu = fmaf(au[2], m[2], fmaf(au[1], m[1], au[0] * m[0]));
-            float Gv = fmaf(av[2], m[2], fmaf(av[1], m[1], av[0] * m[0]));
-            float D = fmaf(A, E, -(B * B));
-            float iD = 1.f / D;
-            float nu_ = tu - fmaf(E, Gu, -(B * Gv)) * iD;
-            float nv_ = tv - fmaf(A, Gv, -(B * Gu)) * iD;
-            float s2 = fmaxf(1.f, fmaxf(fabsf(nu_) * ih2u, fabsf(nv_) * ih2v));
-            if (D > 0.f && s2 < INFINITY) {
-                float is2 = 1.f / s2;
-                tu = nu_ * is2;
-                tv = nv_ * is2;
-            }
-        }
-        seed_u[i] = fmin(fmax(seed_u[i] + (double)tu, u0), u1);
-        seed_v[i] = fmin(fmax(seed_v[i] + (double)tv, v0), v1);
+        or_leafrec L;
+        leaf_rec(sfp, nodes, leaf_out[i], &L);
+        const double P[3] = {x[i], y[i], z[i]};
+        step_from_rec(&L, &g, P, &seed_u[i], &seed_v[i]);
     }
 }
 
@@ -1217,8 +1256,9 @@ void or_seed_grid_step(int p, int q, int nu, int nv, const double *ku, const dou
  * branch and bound: exact-Hessian projected Newton (a coordinate held on its bound while the
  * descent points outward) from the greedy seed leaf's stepped seed (or_seed_greedy_leaf, then
  * seed_step_from_leaf) over the whole domain, then
- * from the centre of EVERY leaf with the iterate clamped to that leaf's uv cell (leaf_iter
- * steps; the seed leaf itself only if the first solve ended outside its cell), the nearest foot
+ * from EVERY leaf's stepped seed for P (seed_step_from_leaf's step from that leaf, clamped into
+ * its cell) with the iterate clamped to that leaf's uv cell (leaf_iter steps; the seed leaf
+ * itself only if the first solve ended outside its cell), the nearest foot
  * point kept (ties to the earlier candidate), then a full-domain Newton polish from it that is
  * kept unless it is farther by more than rounding (1e-9 relative).  The tree is the reference's
  * (uv cells of [0,1]^2, nodes as or_build_tree); leaves in Morton order of their cell (x bits
@@ -1245,6 +1285,9 @@ void or_project_global(int p, int q, int nu, int nv, const double *ku, const dou
     double *gsu = malloc(sizeof(double) * (n > 0 ? n : 1)), *gsv = malloc(sizeof(double) * (n > 0 ? n : 1));
     greedy_leaf_impl(nodes, levels, x, y, z, n, gleaf, gsu, gsv, nthreads, 1);
     seed_step_from_leaf(&sf, nodes, levels, x, y, z, n, gleaf, gsu, gsv, nthreads);
+    const or_stepdom g = step_domain(&sf, levels);
+    or_leafrec *recs = malloc(sizeof(or_leafrec) * side * side);
+    for (long t = 0; t < side * side; t++) leaf_rec(&sf, nodes, base + t, &recs[t]);
 #ifdef _OPENMP
     if (nthreads <= 0) nthreads = omp_get_max_threads();
 #pragma omp parallel for schedule(dynamic, 16) num_threads(nthreads)
@@ -1266,7 +1309,11 @@ void or_project_global(int p, int q, int nu, int nv, const double *ku, const dou
             if (t == bl && bl_done) continue;
             const double *nd = nodes + (base + t) * 13;
             double dom[4] = {nd[0], nd[2], nd[1], nd[3]};
-            double cu = (nd[0] + nd[2]) / 2, cv = (nd[1] + nd[3]) / 2;
+            /* the leaf's stepped seed for P, clamped into its cell */
+            double cu, cv;
+            step_from_rec(&recs[t], &g, P, &cu, &cv);
+            cu = fmin(fmax(cu, nd[0]), nd[2]);
+            cv = fmin(fmax(cv, nd[1]), nd[3]);
             int st = gn_one_in(&sf, P, &leaf, &cu, &cv, NULL, dom);
             double d2 = dist2_at(&sf, P, cu, cv);
             if (d2 < best) { best = d2; bu = cu; bv = cv; bst = st; }
@@ -1278,6 +1325,7 @@ void or_project_global(int p, int q, int nu, int nv, const double *ku, const dou
         v_out[i] = bv;
         status[i] = (unsigned char)bst;
     }
+    free(recs);
     free(gleaf);
     free(gsu);
     free(gsv);
