@@ -5,7 +5,7 @@ Checks: perm is a permutation of the input indices, each block of 65,536 consecu
 keeps its own output range, grouped by span pair in increasing key order (keys from the oracle's
 findSpan, src/NURBS/Evaluate.cu:504-533); every field
 at sorted point k equals the FP64 oracle at input point perm[k] within 1e-5 of the field scale
-(the north star's FP32 bound), for degrees 1..3 on the matrix cores and higher degrees on the
+(the FP32 evaluation tolerance), for degrees 1..3 on the matrix cores and higher degrees on the
 per-point fallback, for orders 0..2, on jittered and repeated knots, knots outside [0,1], a span
 count beyond the LDS histogram (global-atomic path), ragged sizes; and at the 16,777,216-point
 workload, against the per-point FP32 kernel over all points and the oracle on a subset."""
