@@ -289,6 +289,9 @@ struct SlotPool {
 };
 size_t slot_words();    // 64-bit words per counter slot
 size_t list_entries();  // 32-bit indices per continuation list
+// Whether the calling thread's launches go to a pool (a graph capture or the no-allocation entry
+// point): such launches must not allocate.
+bool slot_pool_active();
 // Routes the calling thread's launches to `pool` for the lifetime of the object.
 class PoolScope {
   public:

@@ -261,13 +261,85 @@ __device__ __forceinline__ void project_point(const GnSurfDev &d, const GnSearch
 template <int K1, int K2, bool EXACT>
 constexpr int project_waves() { return (!EXACT || K1 * K2 <= 20) ? 2 : 1; }
 
+// order (may be null): the queries in the order to take them, so that a wave's lanes hold
+// queries near one another in space and walk much the same part of the tree (k_proj_order).
 template <int K1, int K2, bool EXACT>
 __global__ __launch_bounds__(256, (project_waves<K1, K2, EXACT>())) void k_project_global(
-    GnSurfDev d, GnSearchDev sd, InvArgs a) {
+    GnSurfDev d, GnSearchDev sd, InvArgs a, const uint32_t *__restrict__ order) {
     __shared__ uint32_t cand[256 * kCand];
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < a.n;
          i += (size_t)gridDim.x * blockDim.x)
-        project_point<K1, K2, EXACT>(d, sd, a, i, true, cand);
+        project_point<K1, K2, EXACT>(d, sd, a, order ? (size_t)order[i] : i, true, cand);
+}
+
+// Spatial order of the queries: the key is the 16 x 16 x 16 cell of P within the tree's root box
+// (Morton order, 12 bits; points outside the box take its border cells, NaN coordinates cell 0).
+// One workgroup per block of kOrdBlock consecutive queries counts its keys in LDS, scans them and
+// writes each query's index to its key's next slot within the block's own range of `order`.  The
+// order within one key is unspecified; each query's result does not depend on it (project_point
+// is per query).
+constexpr int kOrdThreads = 1024;
+constexpr size_t kOrdBlock = 65536;
+constexpr int kOrdKeys = 4096;
+
+__device__ __forceinline__ uint32_t proj_key(const float *__restrict__ root, const double P[3]) {
+    uint32_t q[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const float lo = root[k], w = root[3 + k] - root[k];
+        const float t = w > 0.f ? ((float)P[k] - lo) * (16.f / w) : 0.f;
+        q[k] = (uint32_t)fminf(fmaxf(t, 0.f), 15.f);
+    }
+    uint32_t m = 0;
+#pragma unroll
+    for (int b = 0; b < 4; b++)
+        m |= ((q[0] >> b & 1u) << (3 * b + 2)) | ((q[1] >> b & 1u) << (3 * b + 1)) |
+             ((q[2] >> b & 1u) << (3 * b));
+    return m;
+}
+
+__global__ __launch_bounds__(kOrdThreads) void k_proj_order(const float *__restrict__ root,
+                                                            const double *__restrict__ x,
+                                                            const double *__restrict__ y,
+                                                            const double *__restrict__ z, size_t n,
+                                                            uint32_t *__restrict__ order) {
+    __shared__ uint32_t cnt[kOrdKeys];
+    __shared__ uint32_t part[kOrdThreads];
+    for (int k = threadIdx.x; k < kOrdKeys; k += kOrdThreads) cnt[k] = 0u;
+    __syncthreads();
+    const size_t i0 = (size_t)blockIdx.x * kOrdBlock;
+    const size_t i1 = i0 + kOrdBlock < n ? i0 + kOrdBlock : n;
+    for (size_t i = i0 + threadIdx.x; i < i1; i += kOrdThreads) {
+        const double P[3] = {x[i], y[i], z[i]};
+        atomicAdd(&cnt[proj_key(root, P)], 1u);
+    }
+    __syncthreads();
+    // exclusive scan: each thread a run of kOrdKeys / kOrdThreads keys, then the runs' sums
+    constexpr int per = kOrdKeys / kOrdThreads;
+    const int k0 = threadIdx.x * per;
+    uint32_t sum = 0;
+#pragma unroll
+    for (int k = 0; k < per; k++) sum += cnt[k0 + k];
+    part[threadIdx.x] = sum;
+    __syncthreads();
+    for (int o = 1; o < kOrdThreads; o <<= 1) {  // inclusive Hillis-Steele scan of the sums
+        const uint32_t v = threadIdx.x >= (unsigned)o ? part[threadIdx.x - o] : 0u;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    uint32_t run = (uint32_t)i0 + (threadIdx.x ? part[threadIdx.x - 1] : 0u);
+#pragma unroll
+    for (int k = 0; k < per; k++) {
+        const uint32_t c = cnt[k0 + k];
+        cnt[k0 + k] = run;
+        run += c;
+    }
+    __syncthreads();
+    for (size_t i = i0 + threadIdx.x; i < i1; i += kOrdThreads) {
+        const double P[3] = {x[i], y[i], z[i]};
+        order[atomicAdd(&cnt[proj_key(root, P)], 1u)] = (uint32_t)i;
+    }
 }
 
 // The fallback: the queries listed in list[0 .. *count) (iteration counts left as they were).
@@ -297,19 +369,38 @@ struct ProjectLaunch {
     const GnSurfDev &d;
     const GnSearchDev &sd;
     const InvArgs &a;
+    const uint32_t *order;
     hipStream_t st;
     template <int K1, int K2, bool EXACT>
     int operator()() {
-        k_project_global<K1, K2, EXACT><<<grid_for(a.n, 256, 256 * 16), 256, 0, st>>>(d, sd, a);
+        k_project_global<K1, K2, EXACT><<<grid_for(a.n, 256, 256 * 16), 256, 0, st>>>(d, sd, a,
+                                                                                    order);
         GN_HIP(hipGetLastError());
         return GN_OK;
     }
 };
 
+// Calls of at least kOrdMin queries (fewer than 2^32) take them in spatial order: a stream-ordered
+// scratch index array of n words, filled by k_proj_order.  Calls that must not allocate (the
+// no-allocation entry point) take them in input order.
+constexpr size_t kOrdMin = 16384;
+
 int launch_project_global(const GnSurfDev &d, const GnSearchDev &sd, const InvArgs &a,
                           hipStream_t st) {
     if (a.n == 0) return GN_OK;
-    return dispatch_degrees(d.p, d.q, ProjectLaunch{d, sd, a, st});
+    uint32_t *order = nullptr;
+    if (a.n >= kOrdMin && a.n < 0xffffffffull && sd.depth > 0 && !slot_pool_active()) {
+        GN_HIP(hipMallocAsync((void **)&order, sizeof(uint32_t) * a.n, st));
+        k_proj_order<<<(unsigned)((a.n + kOrdBlock - 1) / kOrdBlock), kOrdThreads, 0, st>>>(
+            sd.box32, a.x, a.y, a.z, a.n, order);
+        if (hipGetLastError() != hipSuccess) {
+            (void)hipFreeAsync(order, st);
+            return fail(GN_EDEVICE, "query ordering launch");
+        }
+    }
+    const int rc = dispatch_degrees(d.p, d.q, ProjectLaunch{d, sd, a, order, st});
+    if (order) GN_HIP(hipFreeAsync(order, st));
+    return rc;
 }
 
 struct ListLaunch {

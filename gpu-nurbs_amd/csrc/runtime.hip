@@ -36,6 +36,7 @@ size_t list_entries() { return kListCap; }
 
 PoolScope::PoolScope(SlotPool *pool) : prev_(t_pool) { t_pool = pool; }
 PoolScope::~PoolScope() { t_pool = prev_; }
+bool slot_pool_active() { return t_pool != nullptr; }
 
 int ensure_runtime() {
     int dev = 0;
