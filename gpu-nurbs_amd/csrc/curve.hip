@@ -45,7 +45,8 @@ __global__ void k_curve_table(GnCurveDev c, const double *__restrict__ xyz,
 template <typename T>
 struct CurvePt {
     T C[3], C1[3], C2[3];
-    T ih;  // 1 / length of the span evaluated
+    T ih;            // 1 / length of the span evaluated
+    double lo, hi;   // the span evaluated, [k[m], k[m+1]]
 };
 
 template <int ORDER, typename T>
@@ -77,6 +78,8 @@ __device__ __forceinline__ void curve_eval(const GnCurveDev &c, double t, CurveP
     }
     const T iw = recip(R.w), h1 = T(ih);
     o.ih = h1;
+    o.lo = c.k[m];
+    o.hi = c.k[m + 1];
     const T A[3] = {R.x, R.y, R.z}, A1[3] = {R1.x, R1.y, R1.z}, A2[3] = {R2.x, R2.y, R2.z};
 #pragma unroll
     for (int q = 0; q < 3; q++) o.C[q] = A[q] * iw;
@@ -172,8 +175,10 @@ __device__ int curve_loop(const GnCurveDev &c, const double P[3], double &t, int
         if (tol_step > 0 && fabs(dt) < T(tol_step)) { st = GN_PT_CONVERGED; break; }
         // certificate of a Newton step dt: the error left is at most L dt^2 / (2 f''), with
         // L = 3 |C'| |C''| + |r| |C'''| bounding f''' and |C'''| taken as |C''| (p+1)/h (one more
-        // derivative of the span's polynomial); squared, with L^2 <= 2 |C''|^2 (9 |C'|^2 + |r|^2 m^2)
-        if (NEWTON && exact && tol_quad > 0) {
+        // derivative of the span's polynomial); squared, with L^2 <= 2 |C''|^2 (9 |C'|^2 + |r|^2 m^2).
+        // The bound holds on one polynomial piece: a step that leaves the span crosses a knot,
+        // where C'' may jump, and is not certified.
+        if (NEWTON && exact && tol_quad > 0 && t >= o.lo && t <= o.hi) {
             const T c2 = o.C2[0] * o.C2[0] + o.C2[1] * o.C2[1] + o.C2[2] * o.C2[2];
             const T m = T(c.p + 1) * o.ih, t2 = dt * dt, tq = T(tol_quad);
             if (T(0.5) * c2 * (T(9) * c1 + rr * m * m) * (t2 * t2) <= tq * tq * fp * fp) {

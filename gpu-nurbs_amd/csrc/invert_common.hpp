@@ -199,7 +199,11 @@ __device__ __forceinline__ int gn_step(const GnSurfDev &d, const double *__restr
     // A flat minimum (small lambda_min) therefore needs a smaller step than a well-conditioned one.
     // Where the Hessian is not positive definite (a saddle of the distance: the query is farther
     // than the curvature radius) the Gauss-Newton step converges linearly and only tol_step stops it.
-    if (NEWTON && hexact && tol_quad > 0) {
+    // The bound holds on one polynomial piece only: a step that ends outside the span pair it was
+    // evaluated in crosses a knot, where S'' may jump (C1 surfaces of degree 2), and is not
+    // certified; the next iteration evaluates in the new span and certifies there.
+    if (NEWTON && hexact && tol_quad > 0 && cu >= sc.ulo && cu <= sc.uhi && cv >= sc.vlo &&
+        cv <= sc.vhi) {
         const T h2 = o.Suu[0] * o.Suu[0] + o.Suu[1] * o.Suu[1] + o.Suu[2] * o.Suu[2] +
                      T(2) * (o.Suv[0] * o.Suv[0] + o.Suv[1] * o.Suv[1] + o.Suv[2] * o.Suv[2]) +
                      o.Svv[0] * o.Svv[0] + o.Svv[1] * o.Svv[1] + o.Svv[2] * o.Svv[2];

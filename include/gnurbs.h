@@ -102,7 +102,8 @@ typedef struct {
                         /* of |S-P|^2/2 from S', S'' and the span length, so a flat minimum    */
                         /* (small eigenvalue of H) needs a smaller step; points that never     */
                         /* meet it or tol_step end GN_PT_MAXITER.  Not applied to a step that   */
-                        /* fell back to J^T J.  Default 1e-10; <= 0 off                        */
+                        /* fell back to J^T J, nor to a step that leaves the knot span pair it */
+                        /* was evaluated in (S'' may jump at a knot).  Default 1e-10; <= 0 off */
     int fallback;       /* gn_invert: points that end unconverged are solved again by the      */
                         /* global projection (GN_SEED_GLOBAL, FP64, same max_iter / tol_step), */
                         /* keeping the nearer of the two results -- the role of the legacy     */

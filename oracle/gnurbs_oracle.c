@@ -438,7 +438,8 @@ typedef struct {
     double tol_quad; /* certificate of an exact-Hessian step (the library's rule, not in the
                         reference): stop when |H^-1| L |d|^2 / 2 <= tol_quad, L = |S''|
                         (3 |J| + |r| (k1/h_u + k2/h_v)), evaluated in the squared form
-                        0.5 |S''|^2 (9 |J|^2 + |r|^2 m^2) |d|^4 (a+c)^2 <= tol_quad^2 det^2
+                        0.5 |S''|^2 (9 |J|^2 + |r|^2 m^2) |d|^4 (a+c)^2 <= tol_quad^2 det^2,
+                        for a step that ends inside the span pair it was evaluated in
                         (<= 0: off) */
 } or_gn_opts;
 
@@ -518,7 +519,10 @@ static int gn_one_in(const or_surf *sf, const double P[3], const or_gn_opts *op,
             it++;
             break;
         }
-        if (op->newton && hexact && op->tol_quad > 0) {
+        /* the bound holds on one polynomial piece: a step that leaves the span pair it was
+           evaluated in crosses a knot, where S'' may jump, and is not certified */
+        if (op->newton && hexact && op->tol_quad > 0 && cu >= sf->ku[o.m] &&
+            cu <= sf->ku[o.m + 1] && cv >= sf->kv[o.n] && cv <= sf->kv[o.n + 1]) {
             double h2 = 0;
             for (int k = 0; k < 3; k++)
                 h2 += o.Suu[k] * o.Suu[k] + 2 * o.Suv[k] * o.Suv[k] + o.Svv[k] * o.Svv[k];

@@ -128,3 +128,49 @@ def test_step_size_rule_alone_is_not_enough_on_flat_minima():
                                    tol_quad=TOL)
     cert = st == O.CONV_STEP
     assert not np.any(cert & (np.maximum(np.abs(u - u0), np.abs(v - v0)) > 1e-9))
+
+
+def test_no_certificate_for_a_step_across_a_knot():
+    """The bound holds on one polynomial piece.  On a degree-2 surface S'' jumps at every interior
+    knot, so a Newton step that crosses one can land farther than tol_quad from the minimum while
+    the certificate, computed from the span the step started in, passes.  Foot points here sit
+    just beside an interior knot and the seeds just on its other side; a step that ends outside
+    its evaluation span is never certified, and every certified point is within tol_quad."""
+    sd = synth.surface_G(11, 10, 2, 2, seed=97, jitter=0.4)
+    os_ = O.OSurf.from_def(sd)
+    rng = np.random.default_rng(98)
+    n = 4000
+    ku = np.asarray(sd.ku)[3:-3]
+    kv = np.asarray(sd.kv)[3:-3]
+    k0 = ku[rng.integers(0, len(ku), n)]
+    off = rng.uniform(1e-7, 3e-5, n) * rng.choice([-1.0, 1.0], n)
+    u0 = k0 + off
+    v0 = np.where(rng.random(n) < 0.5, kv[rng.integers(0, len(kv), n)] + rng.uniform(-3e-5, 3e-5, n),
+                  rng.uniform(0.05, 0.95, n))
+    b = blocks(os_.eval(u0, v0, 1))
+    P, _ = synth.offset_queries(b["S"], b["Su"], b["Sv"], 2e-2, 99)
+    su = k0 - off * rng.uniform(0.2, 3.0, n)  # across the knot from the foot point
+    sv = v0 + rng.uniform(-3e-5, 3e-5, n)
+    u, v, st, _ = os_.gauss_newton(P[0], P[1], P[2], su, sv, tol_step=0.0, max_iter=40, newton=1,
+                                   tol_quad=TOL)
+    cert = st == O.CONV_STEP
+    assert cert.mean() > 0.8
+    tu, tv, _, _ = os_.gauss_newton(P[0], P[1], P[2], u, v, tol_step=0.0, max_iter=60, newton=1)
+    err = np.maximum(np.abs(u - tu), np.abs(v - tv))[cert]
+    assert np.max(err) <= TOL, float(np.max(err))
+
+
+def test_knot_crossing_point_of_a_c1_batch_surface():
+    """A point of the 224-surface uneven batch (surface 42, degree 2 x 2) whose FP32 bulk result
+    lies across a u knot (0.44547494865471027) from the foot point: a certified Newton step from
+    there once ended 1.4e-8 from the minimum."""
+    defs, _ = synth.uneven_batch(224, seed=1004)
+    os_ = O.OSurf.from_def(defs[42])
+    P = np.array([[0.45269181064699265], [0.2427569024800431], [0.0012385542991139222]])
+    su, sv = np.array([0.4447364762891084]), np.array([0.23347128368914127])
+    bu, bv, _, _ = os_.gauss_newton(P[0], P[1], P[2], su, sv, tol_step=1e-4, max_iter=20, newton=1)
+    assert bu[0] < 0.44547494865471027 < 0.4454750193089489  # the closing step crosses the knot
+    u, v, st, _ = os_.gauss_newton(P[0], P[1], P[2], bu, bv, tol_step=0.0, max_iter=3, newton=1,
+                                   tol_quad=TOL)
+    assert st[0] == O.CONV_STEP
+    assert abs(u[0] - 0.4454750193089489) <= TOL and abs(v[0] - 0.23427055545943834) <= TOL
