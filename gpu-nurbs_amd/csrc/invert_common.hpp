@@ -202,8 +202,7 @@ __device__ __forceinline__ int gn_step(const GnSurfDev &d, const double *__restr
     // The bound holds on one polynomial piece only: a step that ends outside the span pair it was
     // evaluated in crosses a knot, where S'' may jump (C1 surfaces of degree 2), and is not
     // certified; the next iteration evaluates in the new span and certifies there.
-    if (NEWTON && hexact && tol_quad > 0 && cu >= sc.ulo && cu <= sc.uhi && cv >= sc.vlo &&
-        cv <= sc.vhi) {
+    if (NEWTON && hexact && tol_quad > 0) {
         const T h2 = o.Suu[0] * o.Suu[0] + o.Suu[1] * o.Suu[1] + o.Suu[2] * o.Suu[2] +
                      T(2) * (o.Suv[0] * o.Suv[0] + o.Suv[1] * o.Suv[1] + o.Suv[2] * o.Suv[2]) +
                      o.Svv[0] * o.Svv[0] + o.Svv[1] * o.Svv[1] + o.Svv[2] * o.Svv[2];
@@ -211,7 +210,9 @@ __device__ __forceinline__ int gn_step(const GnSurfDev &d, const double *__restr
         const T d2 = du * du + dv * dv, tr = a + c;
         const T lhs = T(0.5) * h2 * (T(9) * j2 + rr * m * m) * (d2 * d2) * (tr * tr);
         const T tq = T(tol_quad);
-        if (lhs <= tq * tq * det * det) return GN_PT_CONVERGED;
+        if (lhs <= tq * tq * det * det && cu >= sc.ulo && cu <= sc.uhi && cv >= sc.vlo &&
+            cv <= sc.vhi)
+            return GN_PT_CONVERGED;
     }
     return kStepContinue;
 }
